@@ -1,0 +1,1360 @@
+// Host engine behind the C ABI (include/cellflow.h).
+//
+// One Engine owns the device state of one or more simulations (a plain simulation is an
+// ensemble of one, so both paths run the same kernels in the same order).  All work goes
+// to one non-blocking HIP stream per engine at default priority.  The reference's host
+// class is ParticleSimulation (cuda-native/src/ParticleSimulation.cu:426-754); each C
+// entry point at the bottom of this file names the member it replaces.
+
+#include "../../include/cellflow.h"
+#include "cf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+namespace cf {
+
+// ------------------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPR(call)                                                                         \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) return fail(CF_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CFR(call)            \
+  do {                       \
+    int rc_ = (call);        \
+    if (rc_ != CF_OK) return rc_; \
+  } while (0)
+
+// ------------------------------------------------------------------------------------
+// host math shared with the kernels' contract
+
+// ParticleSimulation.cu:108-110 in FP32 (this file is compiled with -ffp-contract=off).
+static float eff_radius(const cf_params& p, const float* rbt, int ti, int tj) {
+  const float ri = p.radius * (1.0f + rbt[ti] * p.ratio_with_lfo);
+  const float rj = p.radius * (1.0f + rbt[tj] * p.ratio_with_lfo);
+  return (ri + rj) * 0.5f;
+}
+
+float cutoff_threshold(float e) {
+  if (!(e > 0.0f)) return 0.0f;  // effR <= 0 (or NaN): sqrtf(s + 1e-4f) < effR never holds
+  if (std::isinf(e)) return INFINITY;
+  float t = e * e;
+  while (!(sqrtf(t) >= e)) t = nextafterf(t, INFINITY);
+  while (t > 0.0f && sqrtf(nextafterf(t, 0.0f)) >= e) t = nextafterf(t, 0.0f);
+  return t;
+}
+
+static float max_cutoff(const cf_params& p, const float* rbt, int T) {
+  float m = 0.0f;
+  for (int a = 0; a < T; ++a)
+    for (int b = 0; b < T; ++b) m = std::max(m, eff_radius(p, rbt, a, b));
+  return m;
+}
+
+// The grid rule.  Cells per axis n = floor(L / (cutmax * (1 + 2^-10) / divisor)), at least
+// 1, at most 1024; the total is capped near 8 cells per particle by coarsening the
+// largest axis.  Any n with L/n >= that target keeps every interacting pair within
+// `divisor` cells of each other, so the rule only trades speed, never correctness.
+int grid_dims(const cf_params& p, const float* rbt, int T, int n, int divisor, int dims[3]) {
+  const float L[3] = {p.canvas_width, p.canvas_height, p.canvas_depth};
+  const float cut = max_cutoff(p, rbt, T);
+  if (!std::isfinite(cut)) return fail(CF_ERR_ARG, "effective radius is not finite");
+  for (int a = 0; a < 3; ++a) {
+    if (!(L[a] > 0.0f) || !std::isfinite(L[a])) return fail(CF_ERR_ARG, "canvas size must be > 0");
+    if (cut <= 0.0f) {
+      dims[a] = 1;
+      continue;
+    }
+    const double target = (double)cut * (1.0 + 1.0 / 1024.0) / (double)divisor;
+    double q = std::floor((double)L[a] / target);
+    dims[a] = (int)std::max(1.0, std::min(q, 1024.0));
+  }
+  const long long cap = std::max<long long>(8LL * n, 4096);
+  while ((long long)dims[0] * dims[1] * dims[2] > cap) {
+    int a = 0;
+    if (dims[1] > dims[a]) a = 1;
+    if (dims[2] > dims[a]) a = 2;
+    dims[a] = std::max(1, dims[a] - 1);
+  }
+  return CF_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// engine
+
+struct SimHost {
+  int n = 0;
+  int T = 1;
+  std::vector<float> table;  // T*T final table
+  std::vector<float> raw;    // T*T raw table (updateForceTable input)
+  std::vector<float> rbt;    // T radius modifiers
+  cf_params prm{};
+  bool have_prm = false;
+  int dims[3] = {1, 1, 1};
+  int reach = 1;
+  int p0 = 0;
+  int c0 = 0;
+};
+
+struct EdgeReq {
+  bool on = false;
+  float distance = 0.0f;
+  int maxc = 0;
+};
+
+template <class T>
+static int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  HIPR(hipMalloc((void**)p, count * sizeof(T)));
+  return CF_OK;
+}
+
+template <class T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+struct Engine {
+  int kind = 0;  // 1: cf_sim, 2: cf_ensemble
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::vector<SimHost> sims;
+  long P = 0;      // padded slots
+  int nwaves = 0;
+  long C = 0;      // cells incl. one pad cell per simulation
+  long Ccap = 0;
+  DevBufs b{};
+  int cur = 0;
+  bool load_pending = false;
+  std::vector<int> grid_sig;       // dims/reach of the binning A is sorted for
+  std::vector<SimDesc> hdesc;      // uploaded descriptors (kept alive for async copies)
+  std::vector<PairConst> hpairs;
+  std::vector<uint32_t> hhist;
+  // scan plans: 0 = cells, 1 = cells + edge counts, 2 = edge counts
+  unsigned long long* tickets = nullptr;
+  unsigned long long* flags[3] = {nullptr, nullptr, nullptr};
+  int plan_tiles[3] = {0, 0, 0};
+  // edges
+  int edge_cap_alloc = 0;   // candidate rows allocated (2*maxc)
+  int edge_maxc_alloc = 0;
+  long long edges_count = -1;  // -1: no edge list built
+  int edges_maxc = 0;
+  uint32_t* slot_of = nullptr;
+  cf_tuning tun{1, 256, 32};
+  uint32_t* hsmall = nullptr;  // pinned readback words
+
+  ~Engine() { release(); }
+
+  void release() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (int i = 0; i < 2; ++i) {
+      dfree(b.pos[i]);
+      dfree(b.vel[i]);
+      dfree(b.id[i]);
+      dfree(b.start[i]);
+    }
+    dfree(b.tpos); dfree(b.tvel); dfree(b.tid); dfree(b.key); dfree(b.arrival);
+    dfree(b.hist); dfree(b.tmp_id); dfree(b.acc); dfree(b.pad);
+    dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.eoff);
+    dfree(b.edges); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
+    dfree(slot_of); dfree(tickets);
+    for (auto& f : flags) dfree(f);
+    if (hsmall) (void)hipHostFree(hsmall);
+    hsmall = nullptr;
+    if (st) (void)hipStreamDestroy(st);
+    st = nullptr;
+  }
+
+  int init(int nsims, const int* counts, const int* types) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return fail(CF_ERR_DEVICE, "no HIP device available");
+    HIPR(hipGetDevice(&device));
+    HIPR(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPR(hipHostMalloc((void**)&hsmall, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    sims.assign(nsims, SimHost{});
+    long p = 0;
+    for (int s = 0; s < nsims; ++s) {
+      SimHost& h = sims[s];
+      h.n = counts[s];
+      h.T = types[s];
+      h.table.assign(h.T * h.T, 0.0f);
+      h.raw.assign(h.T * h.T, 0.0f);
+      h.rbt.assign(h.T, 0.0f);
+      h.p0 = (int)p;
+      p += ((long)h.n + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
+    }
+    P = std::max<long>(p, kSlotAlign);
+    nwaves = (int)(P / kWave);
+    for (int i = 0; i < 2; ++i) {
+      CFR(dalloc(&b.pos[i], P));
+      CFR(dalloc(&b.vel[i], P));
+      CFR(dalloc(&b.id[i], P));
+    }
+    CFR(dalloc(&b.tpos, P)); CFR(dalloc(&b.tvel, P)); CFR(dalloc(&b.tid, P));
+    CFR(dalloc(&b.key, P)); CFR(dalloc(&b.arrival, P)); CFR(dalloc(&b.tmp_id, P));
+    CFR(dalloc(&b.acc, P)); CFR(dalloc(&b.pad, P));
+    CFR(dalloc(&b.ecount, P + 1)); CFR(dalloc(&b.eoff, P + 1));
+    CFR(dalloc(&b.sims, nsims)); CFR(dalloc(&b.wave_sim, nwaves));
+    CFR(dalloc(&b.pairs, (size_t)nsims * kMaxTypes * kMaxTypes));
+    CFR(dalloc(&b.flag, 4)); CFR(dalloc(&tickets, 3)); CFR(dalloc(&slot_of, P));
+    HIPR(hipMemsetAsync(b.ecount, 0, (P + 1) * sizeof(uint32_t), st));
+    HIPR(hipMemsetAsync(b.tpos, 0, P * sizeof(float4), st));
+    HIPR(hipMemsetAsync(b.tvel, 0, P * sizeof(float4), st));
+    HIPR(hipMemsetAsync(b.acc, 0, P * sizeof(float4), st));
+    HIPR(hipMemsetAsync(b.pad, 0, P * sizeof(float), st));
+    HIPR(hipMemsetAsync(b.flag, 0, 4 * sizeof(uint32_t), st));
+    std::vector<uint32_t> ws(nwaves, 0);
+    for (int s = 0; s < nsims; ++s) {
+      const int w0 = sims[s].p0 / kWave;
+      const int w1 = (int)((sims[s].p0 + ((long)sims[s].n + kWave - 1) / kWave * kWave) / kWave);
+      for (int w = w0; w < w1 && w < nwaves; ++w) ws[w] = (uint32_t)s;
+    }
+    HIPR(hipMemcpyAsync(b.wave_sim, ws.data(), ws.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, st));
+    // identity indices in slot order for the initial (all zero) state
+    std::vector<uint32_t> ids(P);
+    for (long i = 0; i < P; ++i) ids[i] = (uint32_t)i;
+    HIPR(hipMemcpyAsync(b.tid, ids.data(), P * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPR(hipStreamSynchronize(st));
+    load_pending = true;
+    return flush();
+  }
+
+  // ---- descriptors and grid --------------------------------------------------------
+
+  int build_descs(const EdgeReq& er, bool* grid_changed) {
+    hdesc.assign(sims.size(), SimDesc{});
+    hpairs.assign(sims.size() * kMaxTypes * kMaxTypes, PairConst{});
+    std::vector<int> sig;
+    long c = 0;
+    for (size_t s = 0; s < sims.size(); ++s) {
+      SimHost& h = sims[s];
+      SimDesc& d = hdesc[s];
+      cf_params p = h.prm;
+      if (!h.have_prm) {
+        // No parameters yet: one cell holds everything (only loads/gets happen).
+        p = cf_params{};
+        p.radius = 0.0f;
+        p.canvas_width = p.canvas_height = p.canvas_depth = 1.0f;
+        p.num_particle_types = h.T;
+        p.max_expected_neighbors = 1;
+      }
+      const int divisor = std::max(1, std::min(2, tun.cell_divisor));
+      if (h.have_prm) {
+        CFR(grid_dims(p, h.rbt.data(), h.T, h.n, divisor, h.dims));
+        h.reach = divisor;
+      } else {
+        h.dims[0] = h.dims[1] = h.dims[2] = 1;
+        h.reach = 1;
+      }
+      h.c0 = (int)c;
+      d.n = h.n;
+      d.p0 = h.p0;
+      d.c0 = h.c0;
+      d.nx = h.dims[0];
+      d.ny = h.dims[1];
+      d.nz = h.dims[2];
+      d.ncells = d.nx * d.ny * d.nz;
+      d.T = h.T;
+      d.reach = h.reach;
+      d.generic = (d.nx < 2 * d.reach + 1 || d.ny < 2 * d.reach + 1 || d.nz < 2 * d.reach + 1);
+      d.tab = (int)(s * kMaxTypes * kMaxTypes);
+      d.Lx = p.canvas_width;
+      d.Ly = p.canvas_height;
+      d.Lz = p.canvas_depth;
+      d.ihx = (float)d.nx / d.Lx;
+      d.ihy = (float)d.ny / d.Ly;
+      d.ihz = (float)d.nz / d.Lz;
+      d.dt = p.delta_t;
+      d.friction = p.friction;
+      d.fm = p.force_multiplier;
+      d.one_minus_balance = 1.0f - p.balance;
+      d.max_exp = (float)p.max_expected_neighbors;
+      d.rep = p.repulsion;
+      d.att = p.attraction;
+      d.kexp2 = -p.k * 1.44269504088896340736f;
+      for (int a = 0; a < h.T; ++a)
+        for (int bb = 0; bb < h.T; ++bb) {
+          PairConst& pc = hpairs[d.tab + a * h.T + bb];
+          const float e = h.have_prm ? eff_radius(p, h.rbt.data(), a, bb) : 0.0f;
+          pc.thr = cutoff_threshold(e);
+          pc.fv = h.table[a * h.T + bb];
+          pc.inv_eff = (e > 0.0f) ? 1.0f / e : 0.0f;
+          pc.pad = 0.0f;
+        }
+      d.edge_on = er.on ? 1 : 0;
+      d.edge_d2 = er.distance * er.distance;
+      d.edge_maxc = er.maxc;
+      const float ih[3] = {d.ihx, d.ihy, d.ihz};
+      int re[3];
+      for (int a = 0; a < 3; ++a) {
+        const double q = std::ceil((double)er.distance * (1.0 + 1.0 / 1024.0) * (double)ih[a]);
+        re[a] = (int)std::max(1.0, std::min(q, 1024.0));
+      }
+      d.edge_rx = re[0];
+      d.edge_ry = re[1];
+      d.edge_rz = re[2];
+      c += (long)d.ncells + 1;
+      sig.push_back(d.nx);
+      sig.push_back(d.ny);
+      sig.push_back(d.nz);
+      sig.push_back(d.reach);
+    }
+    C = c;
+    *grid_changed = (sig != grid_sig);
+    if (*grid_changed) {
+      grid_sig = sig;
+      CFR(ensure_cells());
+    }
+    HIPR(hipStreamSynchronize(st));  // descriptors may still be read by queued work
+    HIPR(hipMemcpyAsync(b.sims, hdesc.data(), hdesc.size() * sizeof(SimDesc),
+                        hipMemcpyHostToDevice, st));
+    HIPR(hipMemcpyAsync(b.pairs, hpairs.data(), hpairs.size() * sizeof(PairConst),
+                        hipMemcpyHostToDevice, st));
+    return CF_OK;
+  }
+
+  // Cell arrays for the current C; the histogram holds zeros and each pad cell's size.
+  int ensure_cells() {
+    HIPR(hipStreamSynchronize(st));
+    if (C > Ccap) {
+      dfree(b.hist);
+      dfree(b.start[0]);
+      dfree(b.start[1]);
+      const long cap = C + C / 4 + 64;
+      CFR(dalloc(&b.hist, cap));
+      CFR(dalloc(&b.start[0], cap + 1));
+      CFR(dalloc(&b.start[1], cap + 1));
+      Ccap = cap;
+    }
+    hhist.assign(C, 0u);
+    for (const SimHost& h : sims) {
+      const long pad = ((long)h.n + kSlotAlign - 1) / kSlotAlign * kSlotAlign - h.n;
+      hhist[h.c0 + (long)h.dims[0] * h.dims[1] * h.dims[2]] = (uint32_t)pad;
+    }
+    HIPR(hipMemcpyAsync(b.hist, hhist.data(), C * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    // scan plans: tile counts change with C, so tickets and flags restart
+    const int tc = (int)((C + kScanTile - 1) / kScanTile);
+    const int te = (int)((P + kScanTile - 1) / kScanTile);
+    const int tiles[3] = {tc, tc + te, te};
+    for (int i = 0; i < 3; ++i) {
+      if (tiles[i] > plan_tiles[i]) {
+        dfree(flags[i]);
+        CFR(dalloc(&flags[i], tiles[i]));
+      }
+      plan_tiles[i] = tiles[i];
+      HIPR(hipMemsetAsync(flags[i], 0, tiles[i] * sizeof(unsigned long long), st));
+    }
+    HIPR(hipMemsetAsync(tickets, 0, 3 * sizeof(unsigned long long), st));
+    HIPR(hipStreamSynchronize(st));
+    return CF_OK;
+  }
+
+  ScanPlan plan(int which, uint32_t* cell_out) {
+    ScanPlan sp{};
+    const int tc = (int)((C + kScanTile - 1) / kScanTile);
+    const int te = (int)((P + kScanTile - 1) / kScanTile);
+    ScanSeg cells{b.hist, cell_out, (int)C, tc};
+    ScanSeg edges{b.ecount, b.eoff, (int)P, te};
+    if (which == 0) {
+      sp.seg[0] = cells;
+      sp.nseg = 1;
+      sp.tiles_total = tc;
+    } else if (which == 1) {
+      sp.seg[0] = cells;
+      sp.seg[1] = edges;
+      sp.nseg = 2;
+      sp.tiles_total = tc + te;
+    } else {
+      sp.seg[0] = edges;
+      sp.nseg = 1;
+      sp.tiles_total = te;
+    }
+    sp.ticket = tickets + which;
+    sp.flags = flags[which];
+    return sp;
+  }
+
+  // ---- sorting ---------------------------------------------------------------------
+
+  // Bin `src` (slot order `src_id`) with the current grid and write the (cell, index)
+  // sorted state into buffer `dst`, including its cell starts.
+  int sort_into(const float4* src_pos, const float4* src_vel, const uint32_t* src_id, int dst) {
+    BinArgs ba{src_pos, b.key, b.arrival, b.hist, b.sims, b.wave_sim, nwaves};
+    HIPR(launch_bin(ba, st));
+    HIPR(launch_scan(plan(0, b.start[dst]), st));
+    SortArgs sa{};
+    sa.src_pos = src_pos;
+    sa.src_vel = src_vel;
+    sa.src_id = src_id;
+    sa.key = b.key;
+    sa.arrival = b.arrival;
+    sa.start = b.start[dst];
+    sa.hist = b.hist;
+    sa.tmp_id = b.tmp_id;
+    sa.dst_pos = b.pos[dst];
+    sa.dst_vel = b.vel[dst];
+    sa.dst_id = b.id[dst];
+    sa.sims = b.sims;
+    sa.wave_sim = b.wave_sim;
+    sa.P = P;
+    sa.nwaves = nwaves;
+    HIPR(launch_fill(sa, st));
+    HIPR(launch_rank_scatter(sa, st));
+    return CF_OK;
+  }
+
+  // Loads write index-order records of one simulation into the temporaries; the other
+  // simulations' current state is copied there first, then everything is sorted once.
+  int begin_load() {
+    if (load_pending) return CF_OK;
+    HIPR(hipMemcpyAsync(b.tpos, b.pos[cur], P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPR(hipMemcpyAsync(b.tvel, b.vel[cur], P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPR(hipMemcpyAsync(b.tid, b.id[cur], P * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    load_pending = true;
+    return CF_OK;
+  }
+
+  int flush() {
+    if (!load_pending) return CF_OK;
+    bool changed = false;
+    EdgeReq none;
+    CFR(build_descs(none, &changed));
+    CFR(sort_into(b.tpos, b.tvel, b.tid, cur));
+    load_pending = false;
+    edges_count = -1;
+    return CF_OK;
+  }
+
+  // Re-sort the current state when the grid changes (new parameters).
+  int prepare(const EdgeReq& er) {
+    CFR(flush());
+    bool changed = false;
+    CFR(build_descs(er, &changed));
+    if (changed) {
+      CFR(sort_into(b.pos[cur], b.vel[cur], b.id[cur], 1 - cur));
+      cur = 1 - cur;
+    }
+    return CF_OK;
+  }
+
+  int ensure_edge_bufs(int maxc) {
+    const int cap = 2 * maxc;
+    if (cap > edge_cap_alloc) {
+      HIPR(hipStreamSynchronize(st));
+      dfree(b.ecand_d2);
+      dfree(b.ecand_id);
+      CFR(dalloc(&b.ecand_d2, (size_t)cap * P));
+      CFR(dalloc(&b.ecand_id, (size_t)cap * P));
+      edge_cap_alloc = cap;
+    }
+    if (maxc > edge_maxc_alloc) {
+      HIPR(hipStreamSynchronize(st));
+      dfree(b.estage);
+      dfree(b.edges);
+      CFR(dalloc(&b.estage, (size_t)maxc * P));
+      CFR(dalloc(&b.edges, (size_t)maxc * P));
+      edge_maxc_alloc = maxc;
+    }
+    return CF_OK;
+  }
+
+  // ---- the step ----------------------------------------------------------------------
+
+  ForceArgs force_args(bool edges, int cap) {
+    const int A = cur;
+    ForceArgs fa{};
+    fa.pos = b.pos[A];
+    fa.vel = b.vel[A];
+    fa.id = b.id[A];
+    fa.start = b.start[A];
+    fa.tpos = b.tpos;
+    fa.tvel = b.tvel;
+    fa.key = b.key;
+    fa.arrival = b.arrival;
+    fa.hist = b.hist;
+    fa.acc = b.acc;
+    fa.sims = b.sims;
+    fa.wave_sim = b.wave_sim;
+    fa.pairs = b.pairs;
+    fa.flag = b.flag;
+    fa.ecand_d2 = b.ecand_d2;
+    fa.ecand_id = b.ecand_id;
+    fa.estage = b.estage;
+    fa.ecount = b.ecount;
+    fa.P = P;
+    fa.nwaves = nwaves;
+    fa.edges = edges ? 1 : 0;
+    fa.edge_cap = cap;
+    return fa;
+  }
+
+  SortArgs step_sort_args(bool edges) {
+    const int A = cur, B = 1 - cur;
+    SortArgs sa{};
+    sa.src_pos = b.tpos;
+    sa.src_vel = b.tvel;
+    sa.src_id = b.id[A];
+    sa.key = b.key;
+    sa.arrival = b.arrival;
+    sa.start = b.start[B];
+    sa.hist = b.hist;
+    sa.tmp_id = b.tmp_id;
+    sa.dst_pos = b.pos[B];
+    sa.dst_vel = b.vel[B];
+    sa.dst_id = b.id[B];
+    sa.sims = b.sims;
+    sa.wave_sim = b.wave_sim;
+    if (edges) {
+      sa.estage = b.estage;
+      sa.eoff = b.eoff;
+      sa.edges = b.edges;
+    }
+    sa.P = P;
+    sa.nwaves = nwaves;
+    return sa;
+  }
+
+  // Kernel sequence of one step; `ev` (optional) receives an event after each kernel.
+  int step(bool fused_edges, int cap, hipEvent_t* ev = nullptr) {
+    const int B = 1 - cur;
+    ForceArgs fa = force_args(fused_edges, cap);
+    HIPR(launch_force(fa, fused_edges, tun.block_size, st));
+    if (ev) HIPR(hipEventRecord(ev[0], st));
+    HIPR(launch_scan(plan(fused_edges ? 1 : 0, b.start[B]), st));
+    if (ev) HIPR(hipEventRecord(ev[1], st));
+    SortArgs sa = step_sort_args(fused_edges);
+    HIPR(launch_fill(sa, st));
+    if (ev) HIPR(hipEventRecord(ev[2], st));
+    HIPR(launch_rank_scatter(sa, st));
+    if (ev) HIPR(hipEventRecord(ev[3], st));
+    cur = B;
+    return CF_OK;
+  }
+
+  // Standalone proximity-graph pass on the current state.
+  int edge_pass(int cap, hipEvent_t* ev = nullptr) {
+    EdgeArgs ea{};
+    ea.pos = b.pos[cur];
+    ea.id = b.id[cur];
+    ea.start = b.start[cur];
+    ea.sims = b.sims;
+    ea.wave_sim = b.wave_sim;
+    ea.ecand_d2 = b.ecand_d2;
+    ea.ecand_id = b.ecand_id;
+    ea.estage = b.estage;
+    ea.ecount = b.ecount;
+    ea.P = P;
+    ea.nwaves = nwaves;
+    ea.edge_cap = cap;
+    HIPR(launch_edges(ea, st));
+    if (ev) HIPR(hipEventRecord(ev[0], st));
+    HIPR(launch_scan(plan(2, nullptr), st));
+    if (ev) HIPR(hipEventRecord(ev[1], st));
+    SortArgs sa{};
+    sa.src_id = b.id[cur];
+    sa.sims = b.sims;
+    sa.wave_sim = b.wave_sim;
+    sa.estage = b.estage;
+    sa.eoff = b.eoff;
+    sa.edges = b.edges;
+    sa.P = P;
+    sa.nwaves = nwaves;
+    HIPR(launch_edge_write(sa, st));
+    if (ev) HIPR(hipEventRecord(ev[2], st));
+    return CF_OK;
+  }
+
+  bool fusable(const EdgeReq& er) const {
+    if (!er.on) return false;
+    for (const SimDesc& d : hdesc) {
+      const bool ax = (d.nx < 2 * d.reach + 1 || d.edge_rx <= d.reach) &&
+                      (d.ny < 2 * d.reach + 1 || d.edge_ry <= d.reach) &&
+                      (d.nz < 2 * d.reach + 1 || d.edge_rz <= d.reach);
+      if (!ax) return false;
+    }
+    return true;
+  }
+
+  int finish_call(bool edges_built) {
+    HIPR(hipMemcpyAsync(hsmall, b.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (edges_built)
+      HIPR(hipMemcpyAsync(hsmall + 1, b.eoff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    if (edges_built) edges_count = (long long)hsmall[1];
+    if (hsmall[0] & 1u) return fail(CF_ERR_NONFINITE, "a position or velocity became non-finite");
+    return CF_OK;
+  }
+
+  int simulate(int nsteps, const EdgeReq& er) {
+    for (const SimHost& h : sims)
+      if (!h.have_prm) return fail(CF_ERR_STATE, "parameters not set for every simulation");
+    if (er.on) CFR(ensure_edge_bufs(er.maxc));
+    CFR(prepare(er));
+    HIPR(hipMemsetAsync(b.flag, 0, sizeof(uint32_t), st));
+    const bool fuse = fusable(er);
+    const int cap = 2 * er.maxc;
+    edges_count = -1;
+    for (int k = 0; k < nsteps; ++k) {
+      CFR(step(fuse && k > 0, cap));
+      if (er.on && !fuse) CFR(edge_pass(cap));
+    }
+    if (er.on && (fuse || nsteps == 0)) CFR(edge_pass(cap));
+    if (er.on) edges_maxc = er.maxc;
+    return finish_call(er.on);
+  }
+
+  // ---- state I/O ---------------------------------------------------------------------
+
+  IoArgs io_args(int s) {
+    IoArgs a{};
+    a.pos = b.pos[cur];
+    a.vel = b.vel[cur];
+    a.id = b.id[cur];
+    a.tpos = b.tpos;
+    a.tvel = b.tvel;
+    a.tid = b.tid;
+    a.acc = b.acc;
+    a.pad = b.pad;
+    a.sims = b.sims;
+    a.sim = s;
+    return a;
+  }
+
+  int load_aos(int s, const cf_particle* parts) {
+    const SimHost& h = sims[s];
+    for (int i = 0; i < h.n; ++i)
+      if (parts[i].ptype >= (uint32_t)h.T)
+        return fail(CF_ERR_ARG, "particle %d has type %u >= num_types %d", i, parts[i].ptype, h.T);
+    CFR(begin_load());
+    AosParticle* d = nullptr;
+    CFR(dalloc(&d, std::max(1, h.n)));
+    HIPR(hipMemcpyAsync(d, parts, (size_t)h.n * sizeof(AosParticle), hipMemcpyHostToDevice, st));
+    IoArgs a = io_args(s);
+    a.aos = d;
+    HIPR(launch_io(0, a, h.n, st));
+    HIPR(hipStreamSynchronize(st));
+    dfree(d);
+    return CF_OK;
+  }
+
+  int unload_aos(int s, cf_particle* parts) {
+    CFR(flush());
+    const SimHost& h = sims[s];
+    AosParticle* d = nullptr;
+    CFR(dalloc(&d, std::max(1, h.n)));
+    IoArgs a = io_args(s);
+    a.aos = d;
+    HIPR(launch_io(1, a, h.n, st));
+    HIPR(hipMemcpyAsync(parts, d, (size_t)h.n * sizeof(AosParticle), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    dfree(d);
+    return CF_OK;
+  }
+
+  int get_counts(int s, int32_t* out) {
+    CFR(flush());
+    const SimHost& h = sims[s];
+    int32_t* d = nullptr;
+    CFR(dalloc(&d, std::max(1, h.n)));
+    IoArgs a = io_args(s);
+    a.counts_out = d;
+    HIPR(launch_io(2, a, h.n, st));
+    HIPR(hipMemcpyAsync(out, d, (size_t)h.n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    dfree(d);
+    return CF_OK;
+  }
+
+  int set_counts(int s, const int32_t* in) {
+    CFR(flush());
+    const SimHost& h = sims[s];
+    int32_t* d = nullptr;
+    CFR(dalloc(&d, std::max(1, h.n)));
+    HIPR(hipMemcpyAsync(d, in, (size_t)h.n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    IoArgs a = io_args(s);
+    a.counts_in = d;
+    a.tvel = b.vel[cur];
+    HIPR(launch_io(3, a, h.n, st));
+    HIPR(hipStreamSynchronize(st));
+    dfree(d);
+    return CF_OK;
+  }
+
+  int sort_order(int s, uint32_t* cells, uint32_t* index) {
+    CFR(flush());
+    const SimHost& h = sims[s];
+    uint32_t* d = nullptr;
+    CFR(dalloc(&d, 2 * (size_t)std::max(1, h.n)));
+    IoArgs a = io_args(s);
+    a.cell_out = d;
+    a.index_out = d + h.n;
+    HIPR(launch_io(4, a, h.n, st));
+    HIPR(hipMemcpyAsync(cells, d, (size_t)h.n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPR(hipMemcpyAsync(index, d + h.n, (size_t)h.n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    dfree(d);
+    return CF_OK;
+  }
+
+  int init_particles(int s, uint64_t seed, const float canvas[3], float spawn) {
+    CFR(begin_load());
+    // descriptors must carry this simulation's T before the init kernel runs
+    bool changed = false;
+    EdgeReq none;
+    CFR(build_descs(none, &changed));
+    InitArgs a{};
+    a.tpos = b.tpos;
+    a.tvel = b.tvel;
+    a.tid = b.tid;
+    a.acc = b.acc;
+    a.pad = b.pad;
+    a.sims = b.sims;
+    a.sim = s;
+    a.seed = seed;
+    a.cw = canvas[0];
+    a.ch = canvas[1];
+    a.cd = canvas[2];
+    a.spawn = spawn;
+    HIPR(launch_init(a, sims[s].n, st));
+    return flush();
+  }
+
+  int get_edges(int32_t* pairs, long long capacity, long long* count) {
+    if (edges_count < 0) return fail(CF_ERR_STATE, "no proximity graph has been built");
+    *count = edges_count;
+    if (capacity < edges_count)
+      return fail(CF_ERR_CAPACITY, "edge buffer holds %lld pairs, %lld needed", capacity,
+                  edges_count);
+    if (edges_count > 0)
+      HIPR(hipMemcpyAsync(pairs, b.edges, (size_t)edges_count * sizeof(int2),
+                          hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    return CF_OK;
+  }
+
+  int line_vertices(const cf_color* colors, int ncolors, float* out, long long cap_floats,
+                    long long* vcount) {
+    if (edges_count < 0) return fail(CF_ERR_STATE, "no proximity graph has been built");
+    *vcount = 2 * edges_count;
+    const long long need = 12 * edges_count;
+    if (cap_floats < need)
+      return fail(CF_ERR_CAPACITY, "vertex buffer holds %lld floats, %lld needed", cap_floats, need);
+    if (edges_count == 0) return CF_OK;
+    float* dcol = nullptr;
+    float* dout = nullptr;
+    CFR(dalloc(&dcol, 3 * (size_t)std::max(1, ncolors)));
+    CFR(dalloc(&dout, (size_t)need));
+    if (ncolors > 0)
+      HIPR(hipMemcpyAsync(dcol, colors, 3 * (size_t)ncolors * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPR(launch_slot_of(b.id[cur], slot_of, b.sims, 0, sims[0].n, st));
+    LineArgs la{};
+    la.edges = b.edges;
+    la.pos = b.pos[cur];
+    la.slot_of = slot_of;
+    la.colors = dcol;
+    la.out = dout;
+    la.nedges = edges_count;
+    la.ncolors = ncolors;
+    la.T = sims[0].T;
+    la.p0 = sims[0].p0;
+    HIPR(launch_line_vertices(la, st));
+    HIPR(hipMemcpyAsync(out, dout, (size_t)need * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    dfree(dcol);
+    dfree(dout);
+    return CF_OK;
+  }
+
+  int profile(int nsteps, const EdgeReq& er, double* ms_out, int max_k, int* nk, char* names,
+              int names_cap) {
+    for (const SimHost& h : sims)
+      if (!h.have_prm) return fail(CF_ERR_STATE, "parameters not set");
+    if (er.on) CFR(ensure_edge_bufs(er.maxc));
+    CFR(prepare(er));
+    const bool fuse = fusable(er);
+    const int cap = 2 * er.maxc;
+    const char* kn[] = {"force", "scan", "fill", "rank_scatter", "edges", "edge_scan", "edge_write"};
+    const int K = 7;
+    double acc[K] = {0, 0, 0, 0, 0, 0, 0};
+    int cnt[K] = {0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[8];
+    for (auto& e : ev) HIPR(hipEventCreate(&e));
+    for (int k = 0; k < nsteps; ++k) {
+      HIPR(hipEventRecord(ev[0], st));
+      CFR(step(er.on && fuse && k > 0, cap, ev + 1));
+      int ne = 4;
+      if (er.on && !fuse) {
+        CFR(edge_pass(cap, ev + 5));
+        ne = 7;
+      }
+      HIPR(hipEventSynchronize(ev[ne]));
+      for (int i = 0; i < ne; ++i) {
+        float ms = 0;
+        HIPR(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        acc[i] += ms;
+        cnt[i]++;
+      }
+    }
+    if (er.on && fuse) {
+      HIPR(hipEventRecord(ev[4], st));
+      CFR(edge_pass(cap, ev + 5));
+      HIPR(hipEventSynchronize(ev[7]));
+      for (int i = 4; i < 7; ++i) {
+        float ms = 0;
+        HIPR(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        acc[i] += ms;
+        cnt[i]++;
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    int w = 0;
+    std::string nm;
+    for (int i = 0; i < K && w < max_k; ++i) {
+      if (!cnt[i]) continue;
+      ms_out[w++] = acc[i] / cnt[i];
+      if (!nm.empty()) nm += ",";
+      nm += kn[i];
+    }
+    *nk = w;
+    if (names && names_cap > 0) {
+      std::snprintf(names, names_cap, "%s", nm.c_str());
+    }
+    if (er.on) edges_maxc = er.maxc;
+    return finish_call(er.on);
+  }
+};
+
+}  // namespace cf
+
+// ------------------------------------------------------------------------------------
+// handles
+
+struct cf_sim {
+  std::unique_ptr<cf::Engine> e;
+};
+struct cf_ensemble {
+  std::unique_ptr<cf::Engine> e;
+};
+
+namespace {
+std::mutex g_mu;
+
+template <class H>
+std::unordered_set<const H*>& live() {
+  static std::unordered_set<const H*> s;
+  return s;
+}
+
+template <class H>
+int check(H* h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!h || !live<H>().count(h)) return cf::fail(CF_ERR_HANDLE, "invalid or destroyed handle");
+  return CF_OK;
+}
+
+template <class H>
+void reg(H* h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  live<H>().insert(h);
+}
+
+template <class H>
+void unreg(H* h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  live<H>().erase(h);
+}
+
+int check_params(const cf_params* p, int T) {
+  if (!p) return cf::fail(CF_ERR_ARG, "params is null");
+  if (p->num_particle_types != T)
+    return cf::fail(CF_ERR_ARG, "params.num_particle_types %d != simulation types %d",
+                    p->num_particle_types, T);
+  const float f[] = {p->radius, p->delta_t, p->friction, p->repulsion, p->attraction, p->k,
+                     p->balance, p->ratio_with_lfo, p->force_multiplier};
+  for (float v : f)
+    if (!std::isfinite(v)) return cf::fail(CF_ERR_ARG, "non-finite parameter");
+  if (!(p->canvas_width > 0 && p->canvas_height > 0 && p->canvas_depth > 0) ||
+      !std::isfinite(p->canvas_width) || !std::isfinite(p->canvas_height) ||
+      !std::isfinite(p->canvas_depth))
+    return cf::fail(CF_ERR_ARG, "canvas sizes must be finite and > 0");
+  return CF_OK;
+}
+
+int edge_req(const cf_edge_opts* e, cf::EdgeReq* r) {
+  r->on = false;
+  if (!e) return CF_OK;
+  if (!(e->distance >= 0.0f) || !std::isfinite(e->distance))
+    return cf::fail(CF_ERR_ARG, "edge distance must be finite and >= 0");
+  if (e->max_connections > CF_MAX_CONNECTIONS)
+    return cf::fail(CF_ERR_ARG, "max_connections > %d", CF_MAX_CONNECTIONS);
+  r->on = e->max_connections > 0;
+  r->distance = e->distance;
+  r->maxc = std::max(0, std::min(e->max_connections, CF_MAX_CONNECTIONS));
+  if (!r->on) r->maxc = 0;
+  return CF_OK;
+}
+
+int make_engine(std::unique_ptr<cf::Engine>& out, int kind, int nsims, const int* counts,
+                const int* types) {
+  for (int s = 0; s < nsims; ++s) {
+    if (counts[s] < 1 || counts[s] > (1 << 28))
+      return cf::fail(CF_ERR_ARG, "particle count %d out of range", counts[s]);
+    if (types[s] < 1 || types[s] > CF_MAX_TYPES)
+      return cf::fail(CF_ERR_ARG, "num_types %d out of range 1..%d", types[s], CF_MAX_TYPES);
+  }
+  std::unique_ptr<cf::Engine> e(new cf::Engine());
+  e->kind = kind;
+  CFR(e->init(nsims, counts, types));
+  out = std::move(e);
+  return CF_OK;
+}
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char* cf_last_error_string(void) { return cf::g_err.c_str(); }
+int cf_version(void) { return 100; }
+
+int cf_device_count(int* count) {
+  if (!count) return cf::fail(CF_ERR_ARG, "count is null");
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return cf::fail(CF_ERR_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  return CF_OK;
+}
+
+float cf_cutoff_threshold(float eff_radius) { return cf::cutoff_threshold(eff_radius); }
+
+void cf_force_table_transform(const float* raw, int count, float range, float bias, float offset,
+                              float* out) {
+  // ParticleSimulation.cu:524-527 (float operands: the float overloads of tanh/fmax/fmin)
+  for (int i = 0; i < count; ++i) {
+    const float t = tanhf(raw[i] * offset) * range + bias;
+    out[i] = fmaxf(-1.0f, fminf(1.0f, t));
+  }
+}
+
+int cf_grid_dims(const cf_params* params, const float* rbt, int particle_count, int divisor,
+                 int dims[3]) {
+  if (!params || !rbt || !dims) return cf::fail(CF_ERR_ARG, "null argument");
+  if (params->num_particle_types < 1 || params->num_particle_types > CF_MAX_TYPES)
+    return cf::fail(CF_ERR_ARG, "num_particle_types out of range");
+  if (divisor != 1 && divisor != 2) return cf::fail(CF_ERR_ARG, "cell_divisor must be 1 or 2");
+  return cf::grid_dims(*params, rbt, params->num_particle_types, particle_count, divisor, dims);
+}
+
+int cf_create(int particle_count, int num_types, cf_sim** out) {
+  if (!out) return cf::fail(CF_ERR_ARG, "out is null");
+  *out = nullptr;
+  cf_sim* s = new cf_sim();
+  int rc = make_engine(s->e, 1, 1, &particle_count, &num_types);
+  if (rc != CF_OK) {
+    delete s;
+    return rc;
+  }
+  reg(s);
+  *out = s;
+  return CF_OK;
+}
+
+int cf_destroy(cf_sim* sim) {
+  CFR(check(sim));
+  unreg(sim);
+  delete sim;
+  return CF_OK;
+}
+
+int cf_get_particle_count(cf_sim* sim, int* out) {
+  CFR(check(sim));
+  if (!out) return cf::fail(CF_ERR_ARG, "out is null");
+  *out = sim->e->sims[0].n;
+  return CF_OK;
+}
+
+int cf_get_num_particle_types(cf_sim* sim, int* out) {
+  CFR(check(sim));
+  if (!out) return cf::fail(CF_ERR_ARG, "out is null");
+  *out = sim->e->sims[0].T;
+  return CF_OK;
+}
+
+int cf_set_particle_count(cf_sim* sim, int particle_count) {
+  CFR(check(sim));
+  const cf::SimHost old = sim->e->sims[0];
+  if (particle_count == old.n) return CF_OK;
+  std::unique_ptr<cf::Engine> fresh;
+  int T = old.T;
+  CFR(make_engine(fresh, 1, 1, &particle_count, &T));
+  fresh->tun = sim->e->tun;
+  cf::SimHost& h = fresh->sims[0];
+  h.table = old.table;
+  h.raw = old.raw;
+  h.rbt = old.rbt;
+  h.prm = old.prm;
+  h.have_prm = old.have_prm;
+  sim->e = std::move(fresh);
+  return CF_OK;
+}
+
+int cf_set_num_particle_types(cf_sim* sim, int num_types) {
+  CFR(check(sim));
+  if (num_types < 1 || num_types > CF_MAX_TYPES)
+    return cf::fail(CF_ERR_ARG, "num_types %d out of range 1..%d", num_types, CF_MAX_TYPES);
+  cf::SimHost& h = sim->e->sims[0];
+  h.T = num_types;
+  h.table.assign(num_types * num_types, 0.0f);
+  h.raw.assign(num_types * num_types, 0.0f);
+  h.rbt.assign(num_types, 0.0f);
+  h.have_prm = false;
+  // types of the current particles may now be out of range: reset the state to zero
+  std::vector<cf_particle> zero(h.n);
+  std::memset(zero.data(), 0, zero.size() * sizeof(cf_particle));
+  CFR(sim->e->load_aos(0, zero.data()));
+  return sim->e->flush();
+}
+
+int cf_set_particles(cf_sim* sim, const cf_particle* particles, int count) {
+  CFR(check(sim));
+  if (!particles) return cf::fail(CF_ERR_ARG, "particles is null");
+  if (count != sim->e->sims[0].n)
+    return cf::fail(CF_ERR_ARG, "count %d != particle count %d", count, sim->e->sims[0].n);
+  CFR(sim->e->load_aos(0, particles));
+  return sim->e->flush();
+}
+
+int cf_get_particles(cf_sim* sim, cf_particle* particles, int count) {
+  CFR(check(sim));
+  if (!particles) return cf::fail(CF_ERR_ARG, "particles is null");
+  if (count != sim->e->sims[0].n)
+    return cf::fail(CF_ERR_ARG, "count %d != particle count %d", count, sim->e->sims[0].n);
+  return sim->e->unload_aos(0, particles);
+}
+
+int cf_set_state_soa(cf_sim* sim, const float* pos, const float* vel, const uint32_t* type,
+                     int count) {
+  CFR(check(sim));
+  if (!pos || !type) return cf::fail(CF_ERR_ARG, "pos and type are required");
+  if (count != sim->e->sims[0].n)
+    return cf::fail(CF_ERR_ARG, "count %d != particle count %d", count, sim->e->sims[0].n);
+  std::vector<cf_particle> a(count);
+  for (int i = 0; i < count; ++i) {
+    cf_particle& r = a[i];
+    std::memset(&r, 0, sizeof(r));
+    for (int k = 0; k < 3; ++k) {
+      r.pos[k] = pos[3 * i + k];
+      r.vel[k] = vel ? vel[3 * i + k] : 0.0f;
+    }
+    r.ptype = type[i];
+  }
+  CFR(sim->e->load_aos(0, a.data()));
+  return sim->e->flush();
+}
+
+int cf_get_state_soa(cf_sim* sim, float* pos, float* vel, uint32_t* type, int count) {
+  CFR(check(sim));
+  if (count != sim->e->sims[0].n)
+    return cf::fail(CF_ERR_ARG, "count %d != particle count %d", count, sim->e->sims[0].n);
+  std::vector<cf_particle> a(count);
+  CFR(sim->e->unload_aos(0, a.data()));
+  for (int i = 0; i < count; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      if (pos) pos[3 * i + k] = a[i].pos[k];
+      if (vel) vel[3 * i + k] = a[i].vel[k];
+    }
+    if (type) type[i] = a[i].ptype;
+  }
+  return CF_OK;
+}
+
+int cf_set_neighbor_counts(cf_sim* sim, const int32_t* counts, int count) {
+  CFR(check(sim));
+  if (!counts || count != sim->e->sims[0].n) return cf::fail(CF_ERR_ARG, "bad counts");
+  return sim->e->set_counts(0, counts);
+}
+
+int cf_get_neighbor_counts(cf_sim* sim, int32_t* counts, int count) {
+  CFR(check(sim));
+  if (!counts || count != sim->e->sims[0].n) return cf::fail(CF_ERR_ARG, "bad counts");
+  return sim->e->get_counts(0, counts);
+}
+
+int cf_init_particles(cf_sim* sim, uint64_t seed, const float canvas[3], float spawn_region) {
+  CFR(check(sim));
+  if (!canvas) return cf::fail(CF_ERR_ARG, "canvas is null");
+  for (int a = 0; a < 3; ++a)
+    if (!(canvas[a] > 0.0f) || !std::isfinite(canvas[a])) return cf::fail(CF_ERR_ARG, "bad canvas");
+  if (!(spawn_region > 0.0f)) return cf::fail(CF_ERR_ARG, "spawn_region must be > 0");
+  return sim->e->init_particles(0, seed, canvas, spawn_region);
+}
+
+int cf_set_force_table(cf_sim* sim, const float* table, int num_types) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  if (!table || num_types != h.T) return cf::fail(CF_ERR_ARG, "table must be num_types^2 floats");
+  h.table.assign(table, table + h.T * h.T);
+  return CF_OK;
+}
+
+int cf_get_force_table(cf_sim* sim, float* table, int count) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  if (!table || count != h.T * h.T) return cf::fail(CF_ERR_ARG, "count must be num_types^2");
+  std::copy(h.table.begin(), h.table.end(), table);
+  return CF_OK;
+}
+
+int cf_set_raw_force_table(cf_sim* sim, const float* raw, int count) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  if (!raw || count != h.T * h.T) return cf::fail(CF_ERR_ARG, "count must be num_types^2");
+  h.raw.assign(raw, raw + count);
+  return CF_OK;
+}
+
+int cf_get_raw_force_table(cf_sim* sim, float* raw, int count) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  if (!raw || count != h.T * h.T) return cf::fail(CF_ERR_ARG, "count must be num_types^2");
+  std::copy(h.raw.begin(), h.raw.end(), raw);
+  return CF_OK;
+}
+
+int cf_update_force_table(cf_sim* sim, float range, float bias, float offset) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  cf_force_table_transform(h.raw.data(), h.T * h.T, range, bias, offset, h.table.data());
+  return CF_OK;
+}
+
+int cf_set_radio_by_type(cf_sim* sim, const float* values, int count) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  if (!values || count != h.T) return cf::fail(CF_ERR_ARG, "count must be num_types");
+  h.rbt.assign(values, values + count);
+  return CF_OK;
+}
+
+int cf_set_radio_by_type_value(cf_sim* sim, int index, float value) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  if (index >= 0 && index < h.T) h.rbt[index] = value;  // out of range: ignored (.cu:616)
+  return CF_OK;
+}
+
+int cf_get_radio_by_type(cf_sim* sim, float* values, int count) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  if (!values || count != h.T) return cf::fail(CF_ERR_ARG, "count must be num_types");
+  std::copy(h.rbt.begin(), h.rbt.end(), values);
+  return CF_OK;
+}
+
+int cf_rotate_radio_by_type(cf_sim* sim) {
+  CFR(check(sim));
+  cf::SimHost& h = sim->e->sims[0];
+  std::rotate(h.rbt.rbegin(), h.rbt.rbegin() + 1, h.rbt.rend());  // .cu:596-600
+  return CF_OK;
+}
+
+int cf_simulate(cf_sim* sim, const cf_params* params, int nsteps, const cf_edge_opts* edges) {
+  CFR(check(sim));
+  if (nsteps < 0) return cf::fail(CF_ERR_ARG, "nsteps < 0");
+  CFR(check_params(params, sim->e->sims[0].T));
+  cf::EdgeReq er;
+  CFR(edge_req(edges, &er));
+  sim->e->sims[0].prm = *params;
+  sim->e->sims[0].have_prm = true;
+  return sim->e->simulate(nsteps, er);
+}
+
+int cf_build_edges(cf_sim* sim, float distance, int max_connections) {
+  CFR(check(sim));
+  cf_edge_opts o{distance, max_connections};
+  cf::EdgeReq er;
+  CFR(edge_req(&o, &er));
+  cf::Engine& e = *sim->e;
+  if (!er.on) {
+    e.edges_count = 0;
+    return CF_OK;
+  }
+  if (!e.sims[0].have_prm) {
+    // no step parameters yet: bin with a unit-cell grid spanning the particles
+    return cf::fail(CF_ERR_STATE, "call cf_simulate (or set parameters) before building edges");
+  }
+  CFR(e.ensure_edge_bufs(er.maxc));
+  CFR(e.prepare(er));
+  CFR(e.edge_pass(2 * er.maxc));
+  e.edges_maxc = er.maxc;
+  return e.finish_call(true);
+}
+
+int cf_get_edges(cf_sim* sim, int32_t* pairs, long long capacity, long long* count) {
+  CFR(check(sim));
+  if (!count || (!pairs && capacity > 0)) return cf::fail(CF_ERR_ARG, "null argument");
+  return sim->e->get_edges(pairs, capacity, count);
+}
+
+int cf_get_line_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float* out,
+                         long long capacity_floats, long long* vertex_count) {
+  CFR(check(sim));
+  if (!vertex_count || (!out && capacity_floats > 0) || (!colors && ncolors > 0))
+    return cf::fail(CF_ERR_ARG, "null argument");
+  return sim->e->line_vertices(colors, ncolors, out, capacity_floats, vertex_count);
+}
+
+int cf_get_grid(cf_sim* sim, int dims[3], int* reach) {
+  CFR(check(sim));
+  CFR(sim->e->flush());
+  const cf::SimHost& h = sim->e->sims[0];
+  for (int a = 0; a < 3; ++a) dims[a] = h.dims[a];
+  if (reach) *reach = h.reach;
+  return CF_OK;
+}
+
+int cf_get_sort_order(cf_sim* sim, uint32_t* cell_of_slot, uint32_t* index_of_slot, int count) {
+  CFR(check(sim));
+  if (!cell_of_slot || !index_of_slot || count != sim->e->sims[0].n)
+    return cf::fail(CF_ERR_ARG, "bad arguments");
+  return sim->e->sort_order(0, cell_of_slot, index_of_slot);
+}
+
+int cf_profile_steps(cf_sim* sim, const cf_params* params, int nsteps, const cf_edge_opts* edges,
+                     double* ms_out, int max_kernels, int* nkernels, char* names_buf,
+                     int names_cap) {
+  CFR(check(sim));
+  if (!ms_out || !nkernels || nsteps < 1) return cf::fail(CF_ERR_ARG, "bad arguments");
+  CFR(check_params(params, sim->e->sims[0].T));
+  cf::EdgeReq er;
+  CFR(edge_req(edges, &er));
+  sim->e->sims[0].prm = *params;
+  sim->e->sims[0].have_prm = true;
+  return sim->e->profile(nsteps, er, ms_out, max_kernels, nkernels, names_buf, names_cap);
+}
+
+static int check_tuning(const cf_tuning* t) {
+  if (!t) return cf::fail(CF_ERR_ARG, "tuning is null");
+  if (t->cell_divisor != 1 && t->cell_divisor != 2)
+    return cf::fail(CF_ERR_ARG, "cell_divisor must be 1 or 2");
+  if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
+    return cf::fail(CF_ERR_ARG, "block_size must be 64, 128 or 256");
+  if (t->steps_per_graph < 1) return cf::fail(CF_ERR_ARG, "steps_per_graph must be >= 1");
+  return CF_OK;
+}
+
+int cf_set_tuning(cf_sim* sim, const cf_tuning* tuning) {
+  CFR(check(sim));
+  CFR(check_tuning(tuning));
+  sim->e->tun = *tuning;
+  return CF_OK;
+}
+
+int cf_get_tuning(cf_sim* sim, cf_tuning* tuning) {
+  CFR(check(sim));
+  if (!tuning) return cf::fail(CF_ERR_ARG, "tuning is null");
+  *tuning = sim->e->tun;
+  return CF_OK;
+}
+
+// ---- ensemble ------------------------------------------------------------------------
+
+int cf_ensemble_create(int nsims, const int* counts, const int* types, cf_ensemble** out) {
+  if (!out || !counts || !types || nsims < 1) return cf::fail(CF_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  cf_ensemble* e = new cf_ensemble();
+  int rc = make_engine(e->e, 2, nsims, counts, types);
+  if (rc != CF_OK) {
+    delete e;
+    return rc;
+  }
+  reg(e);
+  *out = e;
+  return CF_OK;
+}
+
+int cf_ensemble_destroy(cf_ensemble* ens) {
+  CFR(check(ens));
+  unreg(ens);
+  delete ens;
+  return CF_OK;
+}
+
+int cf_ensemble_set_sim(cf_ensemble* ens, int s, const cf_params* params, const float* table,
+                        const float* rbt, const cf_particle* particles, int count) {
+  CFR(check(ens));
+  cf::Engine& e = *ens->e;
+  if (s < 0 || s >= (int)e.sims.size()) return cf::fail(CF_ERR_ARG, "simulation index out of range");
+  cf::SimHost& h = e.sims[s];
+  CFR(check_params(params, h.T));
+  if (!table || !rbt || !particles || count != h.n) return cf::fail(CF_ERR_ARG, "bad arguments");
+  h.table.assign(table, table + h.T * h.T);
+  h.rbt.assign(rbt, rbt + h.T);
+  h.prm = *params;
+  h.have_prm = true;
+  return e.load_aos(s, particles);
+}
+
+int cf_ensemble_simulate(cf_ensemble* ens, int nsteps) {
+  CFR(check(ens));
+  if (nsteps < 0) return cf::fail(CF_ERR_ARG, "nsteps < 0");
+  cf::EdgeReq none;
+  return ens->e->simulate(nsteps, none);
+}
+
+int cf_ensemble_get_particles(cf_ensemble* ens, int s, cf_particle* particles, int count) {
+  CFR(check(ens));
+  cf::Engine& e = *ens->e;
+  if (s < 0 || s >= (int)e.sims.size() || !particles || count != e.sims[s].n)
+    return cf::fail(CF_ERR_ARG, "bad arguments");
+  return e.unload_aos(s, particles);
+}
+
+int cf_ensemble_get_neighbor_counts(cf_ensemble* ens, int s, int32_t* counts, int count) {
+  CFR(check(ens));
+  cf::Engine& e = *ens->e;
+  if (s < 0 || s >= (int)e.sims.size() || !counts || count != e.sims[s].n)
+    return cf::fail(CF_ERR_ARG, "bad arguments");
+  return e.get_counts(s, counts);
+}
+
+int cf_ensemble_set_tuning(cf_ensemble* ens, const cf_tuning* tuning) {
+  CFR(check(ens));
+  CFR(check_tuning(tuning));
+  ens->e->tun = *tuning;
+  return CF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,233 @@
+// Internal layout shared by the HIP kernels (cf_kernels.hip) and the host engine
+// (cf_engine.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cf {
+
+constexpr int kMaxTypes = 10;          // MAX_PARTICLE_TYPES (SimulationParams.h:64)
+constexpr int kWave = 64;              // CDNA wavefront
+constexpr int kSlotAlign = 64;         // every simulation's slot range starts on a wave
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 4096 elements per scan tile
+constexpr int kMaxEdgeCap = 64;        // 2 * CF_MAX_CONNECTIONS
+
+// Per-type-pair constants of one simulation, T*T entries, row = own type.
+struct PairConst {
+  float thr;       // smallest float t with sqrtf(t) >= effR (cutoff test: s + 1e-4f < thr)
+  float fv;        // force table value (ParticleSimulation.cu:116)
+  float inv_eff;   // 1 / effR (continuous part only)
+  float pad;
+};
+
+// Device descriptor of one simulation.  Every wave covers 64 slots of one simulation, so
+// the descriptor is wave-uniform and lives in scalar registers.
+struct SimDesc {
+  int n;            // particles
+  int p0;           // first slot (multiple of 64); slot p0 + k, id p0 + local index
+  int c0;           // first cell in the concatenated cell space (the pad cell is c0+ncells)
+  int ncells;       // nx * ny * nz
+  int nx, ny, nz;
+  int T;
+  int generic;      // 1: some axis has fewer than 2*reach+1 cells -> per-pair minimum image
+  int reach;        // force stencil reach in cells (1 or 2)
+  int tab;          // offset of this simulation's PairConst table
+  int edge_on;      // edges computed inside the force pass
+  float Lx, Ly, Lz;
+  float ihx, ihy, ihz;      // n / L in FP32 (binning)
+  float dt, friction, fm, one_minus_balance, max_exp;
+  float rep, att, kexp2;    // repulsion, attraction, -k*log2(e)
+  float edge_d2;            // proximityDistance^2 in FP32 (ParticleSimulation.cu:660)
+  int edge_maxc;            // min(maxConn, 32)
+  int edge_rx, edge_ry, edge_rz;  // non-periodic edge stencil reach (standalone edge pass)
+};
+
+// Decoupled look-back scan plan: one 64-bit ticket counter and one flag word per tile.
+// A flag word is (epoch:30 | status:2 | value:32); tickets never reset, so no kernel has
+// to clear scan state between launches (also under HIP graph replay).
+struct ScanSeg {
+  const uint32_t* in;
+  uint32_t* out;      // n + 1 entries, exclusive prefix, out[n] = total
+  int n;
+  int tiles;
+};
+
+struct ScanPlan {
+  ScanSeg seg[2];
+  int nseg;
+  int tiles_total;
+  unsigned long long* ticket;   // device counter
+  unsigned long long* flags;    // tiles_total words
+};
+
+// Buffers of one engine (all device pointers).
+struct DevBufs {
+  // sorted state, double-buffered (A = current)
+  float4* pos[2];       // x, y, z, type bits
+  float4* vel[2];       // vx, vy, vz, previous neighbour count bits
+  uint32_t* id[2];      // global index of the particle in each slot
+  uint32_t* start[2];   // cell start slots, C + 1 entries
+  // step temporaries (old-slot order)
+  float4* tpos;
+  float4* tvel;
+  uint32_t* tid;
+  uint32_t* key;        // new global cell of each old slot
+  uint32_t* arrival;    // arrival rank inside the new cell (atomic order)
+  uint32_t* hist;       // per-cell counts (pad cells hold their pad size)
+  uint32_t* tmp_id;     // particle index per new slot, cell ranges unsorted
+  float4* acc;          // acceleration by index (output-only, ParticleSimulation.cu:146)
+  float* pad;           // the Particle.pad field by index, carried unchanged
+  // edges
+  float* ecand_d2;      // [cap][P]
+  uint32_t* ecand_id;   // [cap][P]
+  uint32_t* estage;     // [maxc][P] kept neighbour index by rank
+  uint32_t* ecount;     // [P + 1] kept edges per index
+  uint32_t* eoff;       // [P + 1] exclusive prefix of ecount
+  int2* edges;          // canonical output pairs
+  // descriptors
+  SimDesc* sims;
+  uint32_t* wave_sim;   // simulation index of every 64-slot wave
+  PairConst* pairs;
+  uint32_t* flag;       // bit 0: non-finite state seen
+};
+
+// ---- kernel arguments ----
+struct AosParticle {
+  float px, py, pz, vx, vy, vz, ax, ay, az;
+  uint32_t ptype;
+  float pad;
+};
+static_assert(sizeof(AosParticle) == 44, "Particle record must be 44 bytes");
+
+struct ForceArgs {
+  const float4* __restrict__ pos;
+  const float4* __restrict__ vel;
+  const uint32_t* __restrict__ id;
+  const uint32_t* __restrict__ start;
+  float4* __restrict__ tpos;
+  float4* __restrict__ tvel;
+  uint32_t* __restrict__ key;
+  uint32_t* __restrict__ arrival;
+  uint32_t* hist;
+  float4* __restrict__ acc;
+  const SimDesc* __restrict__ sims;
+  const uint32_t* __restrict__ wave_sim;
+  const PairConst* __restrict__ pairs;
+  uint32_t* flag;
+  float* ecand_d2;
+  uint32_t* ecand_id;
+  uint32_t* estage;
+  uint32_t* ecount;
+  long P;
+  int nwaves;
+  int edges;        // fused edge candidates requested for this launch
+  int edge_cap;
+};
+
+struct EdgeArgs {
+  const float4* __restrict__ pos;
+  const uint32_t* __restrict__ id;
+  const uint32_t* __restrict__ start;
+  const SimDesc* __restrict__ sims;
+  const uint32_t* __restrict__ wave_sim;
+  float* ecand_d2;
+  uint32_t* ecand_id;
+  uint32_t* estage;
+  uint32_t* ecount;
+  long P;
+  int nwaves;
+  int edge_cap;
+};
+
+struct SortArgs {
+  const float4* __restrict__ src_pos;
+  const float4* __restrict__ src_vel;
+  const uint32_t* __restrict__ src_id;
+  const uint32_t* __restrict__ key;
+  const uint32_t* __restrict__ arrival;
+  const uint32_t* __restrict__ start;   // new cell starts
+  uint32_t* hist;
+  uint32_t* tmp_id;
+  float4* __restrict__ dst_pos;
+  float4* __restrict__ dst_vel;
+  uint32_t* __restrict__ dst_id;
+  const SimDesc* __restrict__ sims;
+  const uint32_t* __restrict__ wave_sim;
+  // canonical edge output (k_fill only; null when no edges this step)
+  const uint32_t* __restrict__ estage;
+  const uint32_t* __restrict__ eoff;
+  int2* __restrict__ edges;
+  long P;
+  int nwaves;
+};
+
+struct BinArgs {
+  const float4* __restrict__ pos;
+  uint32_t* key;
+  uint32_t* arrival;
+  uint32_t* hist;
+  const SimDesc* __restrict__ sims;
+  const uint32_t* __restrict__ wave_sim;
+  int nwaves;
+};
+
+struct IoArgs {
+  AosParticle* aos;        // count records of one simulation
+  const float4* pos;
+  const float4* vel;
+  const uint32_t* id;
+  float4* tpos;
+  float4* tvel;
+  uint32_t* tid;
+  float4* acc;
+  float* pad;
+  const int32_t* counts_in;   // optional prev counts (index order)
+  int32_t* counts_out;
+  uint32_t* cell_out;
+  uint32_t* index_out;
+  const SimDesc* sims;
+  int sim;
+};
+
+struct InitArgs {
+  float4* tpos;
+  float4* tvel;
+  uint32_t* tid;
+  float4* acc;
+  float* pad;
+  const SimDesc* __restrict__ sims;
+  int sim;
+  unsigned long long seed;
+  float cw, ch, cd, spawn;
+};
+
+struct LineArgs {
+  const int2* __restrict__ edges;
+  const float4* __restrict__ pos;
+  const uint32_t* __restrict__ slot_of;   // index -> slot
+  const float* __restrict__ colors;       // ncolors x 3
+  float* out;
+  long nedges;
+  int ncolors;
+  int T;
+  int p0;
+};
+
+// host-side launchers (cf_kernels.hip)
+hipError_t launch_force(const ForceArgs& a, bool edges, int block, hipStream_t st);
+hipError_t launch_edges(const EdgeArgs& a, hipStream_t st);
+hipError_t launch_edge_write(const SortArgs& a, hipStream_t st);
+hipError_t launch_scan(const ScanPlan& p, hipStream_t st);
+hipError_t launch_fill(const SortArgs& a, hipStream_t st);
+hipError_t launch_rank_scatter(const SortArgs& a, hipStream_t st);
+hipError_t launch_bin(const BinArgs& a, hipStream_t st);
+hipError_t launch_io(int which, const IoArgs& a, int n, hipStream_t st);
+hipError_t launch_init(const InitArgs& a, int n, hipStream_t st);
+hipError_t launch_slot_of(const uint32_t* id, uint32_t* slot_of, const SimDesc* sims, int sim, int n,
+                          hipStream_t st);
+hipError_t launch_line_vertices(const LineArgs& a, hipStream_t st);
+
+}  // namespace cf

@@ -1,0 +1,741 @@
+// HIP kernels of the particle-life step engine, written for gfx950 (CDNA4, wave64).
+//
+// One step (Jacobi: every read comes from the pre-step state A, every write goes to
+// temporaries and then to state B):
+//   k_force         27-cell (or 125-cell) stencil over the cell-sorted state: cutoff
+//                   predicate, neighbour count, smooth force, density factor, friction-
+//                   Euler integration, periodic wrap (ParticleSimulation.cu:69-166); then
+//                   the new cell of the particle and its arrival rank in that cell
+//                   (atomic histogram).  Optionally also the proximity-graph candidate
+//                   selection of the pre-step state (ParticleSimulation.cu:214-246).
+//   k_scan          decoupled look-back exclusive scan of the cell histogram (and of the
+//                   per-owner edge counts), wave64 prefix scans inside each tile.
+//   k_fill          scatters each particle index into its new cell range (slot =
+//                   start + arrival) and writes the canonical edge list.
+//   k_rank_scatter  ranks each particle by index inside its new cell and moves its state
+//                   there: the new order is (cell, index), independent of atomic order.
+// The state lives in HBM sorted by (cell, index) as float4 SoA, so the stencil reads
+// contiguous runs of slots: for each (dy, dz) row the x-neighbour cells are adjacent in
+// memory.
+//
+// Arithmetic contract: compiled with -ffp-contract=off; the squared distance is
+// fmaf(dz, dz, fmaf(dy, dy, dx*dx)) in FP32, the cutoff test is s + 1e-4f < thr with thr
+// the smallest float whose sqrtf reaches the effective radius (identical to the
+// reference's sqrtf(s + 0.0001f) < effR), and binning is floor(x * (n/L)) in FP32.  The
+// oracle evaluates the same FP32 expressions, so counts, cells, order and edges match it
+// bit for bit.
+
+#include "cf_internal.h"
+
+namespace cf {
+
+// ------------------------------------------------------------------------------------
+// small device helpers
+
+__device__ __forceinline__ int bin_axis(float x, float ih, int n) {
+  float u = x * ih;
+  if (!(u >= 0.0f)) return 0;
+  if (u >= (float)n) return n - 1;
+  return (int)u;
+}
+
+__device__ __forceinline__ float min_image(float d, float L) {
+  // ParticleSimulation.cu:97-102
+  if (d > L * 0.5f) d -= L;
+  if (d < -L * 0.5f) d += L;
+  return d;
+}
+
+// fmodf(x + L, L) of ParticleSimulation.cu:159-161; the two common cases are exact
+// without the library call (y - L is exact for y in [L, 2L) by Sterbenz's lemma).
+__device__ __forceinline__ float wrap_pos(float x, float L) {
+  float y = x + L;
+  if (y >= 0.0f && y < L) return y;
+  if (y >= L && y < 2.0f * L) return y - L;
+  return fmodf(y, L);
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t y = __shfl_up(v, d, kWave);
+    if (lane >= d) v += y;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------------------------------
+// proximity-graph candidate list of one owner: the `cap` smallest qualifying indices,
+// kept in a slot-major scratch array (coalesced across the wave).
+
+struct EdgeAcc {
+  int q;            // qualifying candidates seen (stored: min(q, cap))
+  uint32_t maxid;   // largest stored index
+  int maxk;         // its position
+};
+
+__device__ __forceinline__ void edge_push(EdgeAcc& e, float d2, uint32_t jid, int cap, float* cd2,
+                                          uint32_t* cid, long P, int p) {
+  if (e.q < cap) {
+    cd2[(long)e.q * P + p] = d2;
+    cid[(long)e.q * P + p] = jid;
+    if (e.q == 0 || jid > e.maxid) {
+      e.maxid = jid;
+      e.maxk = e.q;
+    }
+    e.q++;
+  } else {
+    e.q++;
+    if (jid < e.maxid) {
+      cd2[(long)e.maxk * P + p] = d2;
+      cid[(long)e.maxk * P + p] = jid;
+      uint32_t m = 0;
+      int mk = 0;
+      for (int k = 0; k < cap; ++k) {
+        uint32_t v = cid[(long)k * P + p];
+        if (k == 0 || v > m) {
+          m = v;
+          mk = k;
+        }
+      }
+      e.maxid = m;
+      e.maxk = mk;
+    }
+  }
+}
+
+// Stable sort of the kept candidates by squared distance (ties by index, which is the
+// order the reference's insertion sort leaves, ParticleSimulation.cu:235-243), truncated
+// to maxc (:246).  Writes estage[rank][p] and returns the kept count.
+__device__ __forceinline__ int edge_finish(const EdgeAcc& e, int cap, int maxc, const float* cd2,
+                                           const uint32_t* cid, uint32_t* estage, long P, int p) {
+  const int m = min(e.q, cap);
+  for (int k = 0; k < m; ++k) {
+    const float dk = cd2[(long)k * P + p];
+    const uint32_t ik = cid[(long)k * P + p];
+    int rank = 0;
+    for (int k2 = 0; k2 < m; ++k2) {
+      const float d2 = cd2[(long)k2 * P + p];
+      const uint32_t i2 = cid[(long)k2 * P + p];
+      rank += (d2 < dk || (d2 == dk && i2 < ik)) ? 1 : 0;
+    }
+    if (rank < maxc) estage[(long)rank * P + p] = ik;
+  }
+  return min(m, maxc);
+}
+
+// One axis of the force stencil as up to three runs of cells: the run below cell 0
+// (wrapped, image shift -L), the unwrapped run, and the run above cell n-1 (wrapped,
+// shift +L).  Axes with fewer than 2r+1 cells are one run over every cell (the minimum
+// image is then decided per pair).  Computed with selects so nothing lands in scratch.
+__device__ __forceinline__ bool axis_run(int k, int c, int r, int n, float L, int& lo, int& hi,
+                                         float& sh) {
+  if (n < 2 * r + 1) {
+    lo = 0;
+    hi = n - 1;
+    sh = 0.0f;
+    return k == 1;
+  }
+  const int a = c - r, b = c + r;
+  if (k == 0) {
+    lo = a + n;
+    hi = n - 1;
+    sh = -L;
+    return a < 0;
+  }
+  if (k == 2) {
+    lo = 0;
+    hi = b - n;
+    sh = L;
+    return b >= n;
+  }
+  lo = max(a, 0);
+  hi = min(b, n - 1);
+  sh = 0.0f;
+  return true;
+}
+
+// ------------------------------------------------------------------------------------
+// k_force: the fused force + integrate + rebin-key pass (and fused edge candidates).
+
+
+template <bool EDGES>
+__global__ void __launch_bounds__(256) k_force(ForceArgs a) {
+  __shared__ PairConst tab[4][kMaxTypes * kMaxTypes];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gw = blockIdx.x * (blockDim.x >> 6) + wv;
+  const bool wave_ok = gw < a.nwaves;
+  const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[gw] : 0);
+  const SimDesc sd = a.sims[s];
+  if (wave_ok)
+    for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = a.pairs[sd.tab + t];
+  __syncthreads();
+  if (!wave_ok) return;
+  const int p = gw * kWave + lane;
+  if (p - sd.p0 >= sd.n) return;
+
+  const float4 pi = a.pos[p];
+  const float4 vi = a.vel[p];
+  const uint32_t idi = a.id[p];
+  const uint32_t ti = __float_as_uint(pi.w);
+  const PairConst* row = &tab[wv][ti * sd.T];
+  const int cx = bin_axis(pi.x, sd.ihx, sd.nx);
+  const int cy = bin_axis(pi.y, sd.ihy, sd.ny);
+  const int cz = bin_axis(pi.z, sd.ihz, sd.nz);
+
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  int cnt = 0;
+  const bool do_edges = EDGES && a.edges && sd.edge_on;
+  EdgeAcc ea;
+  ea.q = 0;
+  ea.maxid = 0;
+  ea.maxk = 0;
+  const int r = sd.reach;
+  const bool gen = sd.generic != 0;
+  const int ylo = (sd.ny < 2 * r + 1) ? 0 : cy - r;
+  const int yhi = (sd.ny < 2 * r + 1) ? sd.ny - 1 : cy + r;
+  const int zlo = (sd.nz < 2 * r + 1) ? 0 : cz - r;
+  const int zhi = (sd.nz < 2 * r + 1) ? sd.nz - 1 : cz + r;
+
+  for (int zz = zlo; zz <= zhi; ++zz) {
+    int z = zz;
+    float sz = 0.0f;
+    if (z < 0) { z += sd.nz; sz = -sd.Lz; }
+    else if (z >= sd.nz) { z -= sd.nz; sz = sd.Lz; }
+    for (int yy = ylo; yy <= yhi; ++yy) {
+      int y = yy;
+      float sy = 0.0f;
+      if (y < 0) { y += sd.ny; sy = -sd.Ly; }
+      else if (y >= sd.ny) { y -= sd.ny; sy = sd.Ly; }
+      const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+      for (int run = 0; run < 3; ++run) {
+        int xlo, xhi;
+        float sx;
+        if (!axis_run(run, cx, r, sd.nx, sd.Lx, xlo, xhi, sx)) continue;
+        const int q0 = (int)a.start[rowc + xlo];
+        const int q1 = (int)a.start[rowc + xhi + 1];
+        const bool nowrap = (sx == 0.0f) && (sy == 0.0f) && (sz == 0.0f);
+        for (int q = q0; q < q1; ++q) {
+          const float4 pj = a.pos[q];
+          float dx, dy, dz;
+          bool raw;
+          if (!gen) {
+            dx = (pj.x - pi.x) + sx;
+            dy = (pj.y - pi.y) + sy;
+            dz = (pj.z - pi.z) + sz;
+            raw = nowrap;
+          } else {
+            const float ux = pj.x - pi.x, uy = pj.y - pi.y, uz = pj.z - pi.z;
+            dx = min_image(ux, sd.Lx);
+            dy = min_image(uy, sd.Ly);
+            dz = min_image(uz, sd.Lz);
+            raw = (dx == ux) && (dy == uy) && (dz == uz);
+          }
+          const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+          const uint32_t tj = __float_as_uint(pj.w);
+          const float sp = s2 + 1e-4f;
+          const PairConst pc = row[tj];
+          if (sp < pc.thr && q != p) {
+            cnt++;
+            const float inv = __builtin_amdgcn_rsqf(sp);
+            const float dist = sp * inv;
+            const float rr = dist * pc.inv_eff;
+            const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
+            const float net = (sd.rep * ex - sd.att * rr) * pc.fv;
+            const float w = net * inv;
+            fx += dx * w;
+            fy += dy * w;
+            fz += dz * w;
+          }
+          if (EDGES) {
+            if (do_edges && raw && tj == ti && s2 < sd.edge_d2 && q != p) {
+              const uint32_t jid = a.id[q];
+              if (jid > idi) edge_push(ea, s2, jid, a.edge_cap, a.ecand_d2, a.ecand_id, a.P, p);
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // ParticleSimulation.cu:136-161
+  const int prev = __float_as_int(vi.w);
+  const float avg = (float)(cnt + prev) * 0.5f;
+  const float dens = fminf(avg / sd.max_exp, 1.0f);
+  const float fac = 1.0f - sd.one_minus_balance * dens;
+  const float m = sd.fm * fac;
+  fx *= m;
+  fy *= m;
+  fz *= m;
+  const float vx = vi.x * sd.friction + fx * sd.dt;
+  const float vy = vi.y * sd.friction + fy * sd.dt;
+  const float vz = vi.z * sd.friction + fz * sd.dt;
+  const float x = wrap_pos(pi.x + vx * sd.dt, sd.Lx);
+  const float y = wrap_pos(pi.y + vy * sd.dt, sd.Ly);
+  const float z = wrap_pos(pi.z + vz * sd.dt, sd.Lz);
+  const float fin = fabsf(x) + fabsf(y) + fabsf(z) + fabsf(vx) + fabsf(vy) + fabsf(vz);
+  if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
+
+  a.tpos[p] = make_float4(x, y, z, pi.w);
+  a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
+  a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  const uint32_t c = (uint32_t)(sd.c0 + bin_axis(x, sd.ihx, sd.nx) +
+                                sd.nx * (bin_axis(y, sd.ihy, sd.ny) +
+                                         sd.ny * bin_axis(z, sd.ihz, sd.nz)));
+  a.key[p] = c;
+  a.arrival[p] = atomicAdd(&a.hist[c], 1u);
+
+  if (EDGES) {
+    if (do_edges) {
+      const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p);
+      a.ecount[idi] = (uint32_t)e;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// k_edges: standalone proximity-graph candidates on the current sorted state, with a
+// non-periodic stencil of per-axis reach (the edge distance may exceed the cell size).
+
+
+__global__ void __launch_bounds__(256) k_edges(EdgeArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (gw >= a.nwaves) return;
+  const SimDesc sd = a.sims[a.wave_sim[gw]];
+  const int p = gw * kWave + lane;
+  if (p - sd.p0 >= sd.n) return;
+  const float4 pi = a.pos[p];
+  const uint32_t idi = a.id[p];
+  const uint32_t ti = __float_as_uint(pi.w);
+  const int cx = bin_axis(pi.x, sd.ihx, sd.nx);
+  const int cy = bin_axis(pi.y, sd.ihy, sd.ny);
+  const int cz = bin_axis(pi.z, sd.ihz, sd.nz);
+  const int xlo = max(cx - sd.edge_rx, 0), xhi = min(cx + sd.edge_rx, sd.nx - 1);
+  const int ylo = max(cy - sd.edge_ry, 0), yhi = min(cy + sd.edge_ry, sd.ny - 1);
+  const int zlo = max(cz - sd.edge_rz, 0), zhi = min(cz + sd.edge_rz, sd.nz - 1);
+  EdgeAcc ea;
+  ea.q = 0;
+  ea.maxid = 0;
+  ea.maxk = 0;
+  for (int z = zlo; z <= zhi; ++z)
+    for (int y = ylo; y <= yhi; ++y) {
+      const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+      const int q0 = (int)a.start[rowc + xlo];
+      const int q1 = (int)a.start[rowc + xhi + 1];
+      for (int q = q0; q < q1; ++q) {
+        const float4 pj = a.pos[q];
+        if (__float_as_uint(pj.w) != ti) continue;       // :219
+        const float dx = pj.x - pi.x;                     // :222-225, non-periodic
+        const float dy = pj.y - pi.y;
+        const float dz = pj.z - pi.z;
+        const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (!(s2 < sd.edge_d2)) continue;                 // :227
+        const uint32_t jid = a.id[q];
+        if (jid <= idi) continue;                         // j > i, :215
+        edge_push(ea, s2, jid, a.edge_cap, a.ecand_d2, a.ecand_id, a.P, p);
+      }
+    }
+  const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p);
+  a.ecount[idi] = (uint32_t)e;
+}
+
+// ------------------------------------------------------------------------------------
+// k_scan: single-pass decoupled look-back exclusive scan over up to two arrays.
+
+__device__ __forceinline__ unsigned long long pack_flag(uint32_t epoch, uint32_t status,
+                                                        uint32_t value) {
+  return ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34) |
+         ((unsigned long long)status << 32) | (unsigned long long)value;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan(ScanPlan plan) {
+  __shared__ uint32_t smem[kScanTile + kScanTile / 32];
+  __shared__ uint32_t wsum[kScanThreads / kWave];
+  __shared__ uint32_t s_tile, s_epoch, s_prefix;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wv = tid >> 6;
+  if (tid == 0) {
+    const unsigned long long t = atomicAdd(plan.ticket, 1ull);
+    s_epoch = (uint32_t)((t / (unsigned long long)plan.tiles_total) & 0x3FFFFFFFu);
+    s_tile = (uint32_t)(t % (unsigned long long)plan.tiles_total);
+  }
+  __syncthreads();
+  const uint32_t g = s_tile;
+  const uint32_t epoch = s_epoch;
+  const int seg = (plan.nseg > 1 && (int)g >= plan.seg[0].tiles) ? 1 : 0;
+  const int tile = (int)g - (seg ? plan.seg[0].tiles : 0);
+  const uint32_t* in = plan.seg[seg].in;
+  uint32_t* out = plan.seg[seg].out;
+  const int n = plan.seg[seg].n;
+  const long base = (long)tile * kScanTile;
+
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const int e = i * kScanThreads + tid;
+    const long gi = base + e;
+    smem[e + (e >> 5)] = (gi < n) ? in[gi] : 0u;
+  }
+  __syncthreads();
+  uint32_t items[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int e = tid * kScanItems + k;
+    items[k] = smem[e + (e >> 5)];
+    sum += items[k];
+  }
+  const uint32_t incl = wave_inclusive_scan(sum, lane);
+  if (lane == kWave - 1) wsum[wv] = incl;
+  __syncthreads();
+  uint32_t wpre = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / kWave; ++w) {
+    const uint32_t v = wsum[w];
+    if (w < wv) wpre += v;
+    total += v;
+  }
+  const uint32_t texcl = wpre + incl - sum;
+
+  if (tid == 0) {
+    uint32_t prefix = 0;
+    if (tile == 0) {
+      __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 2u, total), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 1u, total), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      // Look back over this segment's earlier tiles.  Every earlier tile holds an earlier
+      // ticket, so its workgroup is already running and publishes without waiting.
+      int j = (int)g - 1;
+      while (true) {
+        const unsigned long long w =
+            __hip_atomic_load(&plan.flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t we = (uint32_t)(w >> 34);
+        const uint32_t st = (uint32_t)(w >> 32) & 3u;
+        if (we != epoch || st == 0u) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        prefix += (uint32_t)w;
+        if (st == 2u) break;
+        --j;
+      }
+      __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 2u, prefix + total), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_prefix = prefix;
+  }
+  __syncthreads();
+  uint32_t run = s_prefix + texcl;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int e = tid * kScanItems + k;
+    smem[e + (e >> 5)] = run;
+    run += items[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const int e = i * kScanThreads + tid;
+    const long gi = base + e;
+    if (gi < n) out[gi] = smem[e + (e >> 5)];
+  }
+  if (tid == 0 && tile == plan.seg[seg].tiles - 1) out[n] = s_prefix + total;
+}
+
+// ------------------------------------------------------------------------------------
+// k_fill / k_rank_scatter: the two halves of the stable counting sort.
+
+
+__global__ void __launch_bounds__(256) k_fill(SortArgs a) {
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (gw >= a.nwaves) return;
+  const SimDesc sd = a.sims[a.wave_sim[gw]];
+  const int p = gw * kWave + (threadIdx.x & (kWave - 1));
+  if (p - sd.p0 >= sd.n) return;
+  const uint32_t c = a.key[p];
+  const uint32_t me = a.src_id[p];
+  a.tmp_id[a.start[c] + a.arrival[p]] = me;
+  a.hist[c] = 0u;  // every writer stores 0; pad cells are never keys and keep their size
+  if (a.edges) {
+    const uint32_t off = a.eoff[me];
+    const int e = (int)(a.eoff[me + 1] - off);
+    for (int k = 0; k < e; ++k)
+      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_rank_scatter(SortArgs a) {
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (gw >= a.nwaves) return;
+  const SimDesc sd = a.sims[a.wave_sim[gw]];
+  const int p = gw * kWave + (threadIdx.x & (kWave - 1));
+  if (p - sd.p0 >= sd.n) return;
+  const uint32_t c = a.key[p];
+  const uint32_t me = a.src_id[p];
+  const uint32_t s0 = a.start[c], s1 = a.start[c + 1];
+  uint32_t rank = 0;
+  for (uint32_t q = s0; q < s1; ++q) rank += (a.tmp_id[q] < me) ? 1u : 0u;
+  const uint32_t dst = s0 + rank;
+  a.dst_pos[dst] = a.src_pos[p];
+  a.dst_vel[dst] = a.src_vel[p];
+  a.dst_id[dst] = me;
+}
+
+// Canonical edge list of the standalone edge pass: owner index ascending (offsets from
+// the scan of per-owner counts), then distance rank.
+__global__ void __launch_bounds__(256) k_edge_write(SortArgs a) {
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (gw >= a.nwaves) return;
+  const SimDesc sd = a.sims[a.wave_sim[gw]];
+  const int p = gw * kWave + (threadIdx.x & (kWave - 1));
+  if (p - sd.p0 >= sd.n) return;
+  const uint32_t me = a.src_id[p];
+  const uint32_t off = a.eoff[me];
+  const int e = (int)(a.eoff[me + 1] - off);
+  for (int k = 0; k < e; ++k)
+    a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
+}
+
+// ------------------------------------------------------------------------------------
+// binning of a given order (load, grid change)
+
+
+__global__ void __launch_bounds__(256) k_bin(BinArgs a) {
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (gw >= a.nwaves) return;
+  const SimDesc sd = a.sims[a.wave_sim[gw]];
+  const int p = gw * kWave + (threadIdx.x & (kWave - 1));
+  if (p - sd.p0 >= sd.n) return;
+  const float4 pi = a.pos[p];
+  const uint32_t c = (uint32_t)(sd.c0 + bin_axis(pi.x, sd.ihx, sd.nx) +
+                                sd.nx * (bin_axis(pi.y, sd.ihy, sd.ny) +
+                                         sd.ny * bin_axis(pi.z, sd.ihz, sd.nz)));
+  a.key[p] = c;
+  a.arrival[p] = atomicAdd(&a.hist[c], 1u);
+}
+
+// ------------------------------------------------------------------------------------
+// state load / unload in the reference's 44-byte AoS layout (index order)
+
+
+
+// Index-order AoS -> slot-order temporaries of one simulation (slot = p0 + index).
+__global__ void k_load_aos(IoArgs a) {
+  const SimDesc sd = a.sims[a.sim];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= sd.n) return;
+  const AosParticle r = a.aos[i];
+  const int p = sd.p0 + i;
+  const int prev = a.counts_in ? a.counts_in[i] : 0;
+  a.tpos[p] = make_float4(r.px, r.py, r.pz, __uint_as_float(r.ptype));
+  a.tvel[p] = make_float4(r.vx, r.vy, r.vz, __int_as_float(prev));
+  a.tid[p] = (uint32_t)p;
+  a.acc[p] = make_float4(r.ax, r.ay, r.az, 0.0f);
+  a.pad[p] = r.pad;
+}
+
+// Sorted state -> index-order AoS of one simulation.
+__global__ void k_unload_aos(IoArgs a) {
+  const SimDesc sd = a.sims[a.sim];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int p = sd.p0 + k;
+  const float4 ps = a.pos[p];
+  const float4 vs = a.vel[p];
+  const uint32_t g = a.id[p];
+  const float4 ac = a.acc[g];
+  AosParticle r;
+  r.px = ps.x; r.py = ps.y; r.pz = ps.z;
+  r.vx = vs.x; r.vy = vs.y; r.vz = vs.z;
+  r.ax = ac.x; r.ay = ac.y; r.az = ac.z;
+  r.ptype = __float_as_uint(ps.w);
+  r.pad = a.pad[g];
+  a.aos[g - sd.p0] = r;
+}
+
+__global__ void k_get_counts(IoArgs a) {
+  const SimDesc sd = a.sims[a.sim];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int p = sd.p0 + k;
+  a.counts_out[a.id[p] - sd.p0] = __float_as_int(a.vel[p].w);
+}
+
+__global__ void k_set_counts(IoArgs a) {
+  const SimDesc sd = a.sims[a.sim];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int p = sd.p0 + k;
+  float4 v = a.vel[p];
+  v.w = __int_as_float(a.counts_in[a.id[p] - sd.p0]);
+  a.tvel[p] = v;  // the caller passes the current velocity buffer as tvel
+}
+
+__global__ void k_sort_order(IoArgs a) {
+  const SimDesc sd = a.sims[a.sim];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int p = sd.p0 + k;
+  const float4 ps = a.pos[p];
+  a.cell_out[k] = (uint32_t)(bin_axis(ps.x, sd.ihx, sd.nx) +
+                             sd.nx * (bin_axis(ps.y, sd.ihy, sd.ny) +
+                                      sd.ny * bin_axis(ps.z, sd.ihz, sd.nz)));
+  a.index_out[k] = a.id[p] - sd.p0;
+}
+
+// ------------------------------------------------------------------------------------
+// seeded initial state (replacement for ParticleSimulation.cu:21-66)
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ float u24(unsigned long long seed, unsigned long long stream,
+                                     unsigned long long idx) {
+  const unsigned long long key = seed * 0x632BE59BD9B4E019ull + stream * 0x85EBCA77C2B2AE63ull;
+  const unsigned long long base = mix64(key + 0x9E3779B97F4A7C15ull);
+  const unsigned long long z = mix64(base + (idx + 1ull) * 0x9E3779B97F4A7C15ull);
+  return (float)(uint32_t)(z >> 40) * 5.9604644775390625e-8f;  // 2^-24
+}
+
+
+__global__ void k_init(InitArgs a) {
+  const SimDesc sd = a.sims[a.sim];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= sd.n) return;
+  const float sw = fminf(a.spawn, a.cw), sh = fminf(a.spawn, a.ch), sdp = fminf(a.spawn, a.cd);
+  const float ox = (a.cw - sw) * 0.5f, oy = (a.ch - sh) * 0.5f, oz = (a.cd - sdp) * 0.5f;
+  // stream 1: positions (3 per particle), stream 2: type — as cellflow_amd.configs
+  const unsigned long long i3 = 3ull * (unsigned long long)i;
+  const float x = ox + u24(a.seed, 1, i3) * sw;
+  const float y = oy + u24(a.seed, 1, i3 + 1) * sh;
+  const float z = oz + u24(a.seed, 1, i3 + 2) * sdp;
+  uint32_t t = (uint32_t)(u24(a.seed, 2, (unsigned long long)i) * (float)sd.T);
+  if (t >= (uint32_t)sd.T) t = (uint32_t)sd.T - 1;
+  const int p = sd.p0 + i;
+  a.tpos[p] = make_float4(x, y, z, __uint_as_float(t));
+  a.tvel[p] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+  a.tid[p] = (uint32_t)p;
+  a.acc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  a.pad[p] = 0.0f;
+}
+
+// ------------------------------------------------------------------------------------
+// line vertices (ParticleSimulation.cu:253-275 layout) from the canonical edge list
+
+
+__global__ void k_slot_of(const uint32_t* __restrict__ id, uint32_t* slot_of, const SimDesc* sims,
+                          int sim) {
+  const SimDesc sd = sims[sim];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int p = sd.p0 + k;
+  slot_of[id[p] - sd.p0] = (uint32_t)p;
+}
+
+__global__ void k_line_vertices(LineArgs a) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nedges) return;
+  const int2 ij = a.edges[e];
+  const float4 pi = a.pos[a.slot_of[ij.x]];
+  const float4 pj = a.pos[a.slot_of[ij.y]];
+  const uint32_t ci = __float_as_uint(pi.w) % (uint32_t)a.T;
+  float r = 0.0f, g = 0.0f, b = 0.0f;
+  if ((int)ci < a.ncolors) {
+    r = a.colors[3 * ci];
+    g = a.colors[3 * ci + 1];
+    b = a.colors[3 * ci + 2];
+  }
+  float* o = a.out + e * 12;
+  o[0] = pi.x; o[1] = pi.y; o[2] = pi.z; o[3] = r; o[4] = g; o[5] = b;
+  o[6] = pj.x; o[7] = pj.y; o[8] = pj.z; o[9] = r; o[10] = g; o[11] = b;
+}
+
+// ------------------------------------------------------------------------------------
+// host-side launchers
+
+static inline int grid_for(int nwaves, int block) { return (nwaves * kWave + block - 1) / block; }
+
+hipError_t launch_force(const ForceArgs& a, bool edges, int block, hipStream_t st) {
+  if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+  const int g = grid_for(a.nwaves, block);
+  if (edges)
+    hipLaunchKernelGGL(k_force<true>, dim3(g), dim3(block), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_force<false>, dim3(g), dim3(block), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_edges(const EdgeArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_edges, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const ScanPlan& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan, dim3(p.tiles_total), dim3(kScanThreads), 0, st, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(const SortArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rank_scatter(const SortArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_rank_scatter, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_write(const SortArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_edge_write, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin(const BinArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_io(int which, const IoArgs& a, int n, hipStream_t st) {
+  const int g = (n + 255) / 256;
+  if (g == 0) return hipSuccess;
+  switch (which) {
+    case 0: hipLaunchKernelGGL(k_load_aos, dim3(g), dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL(k_unload_aos, dim3(g), dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_get_counts, dim3(g), dim3(256), 0, st, a); break;
+    case 3: hipLaunchKernelGGL(k_set_counts, dim3(g), dim3(256), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_sort_order, dim3(g), dim3(256), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_init(const InitArgs& a, int n, hipStream_t st) {
+  const int g = (n + 255) / 256;
+  if (g == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_init, dim3(g), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_of(const uint32_t* id, uint32_t* slot_of, const SimDesc* sims, int sim, int n,
+                          hipStream_t st) {
+  const int g = (n + 255) / 256;
+  if (g == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_slot_of, dim3(g), dim3(256), 0, st, id, slot_of, sims, sim);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_vertices(const LineArgs& a, hipStream_t st) {
+  const long g = (a.nedges + 255) / 256;
+  if (g == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_line_vertices, dim3((unsigned)g), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace cf

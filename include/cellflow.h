@@ -1,0 +1,222 @@
+/* cellflow.h — C ABI of the MI355X particle-life step engine.
+ *
+ * This is the drop-in boundary for the reference's simulation-step path, the C++ class
+ * `ParticleSimulation` (cuda-native/include/ParticleSimulation.cuh:10-103, implemented in
+ * cuda-native/src/ParticleSimulation.cu:426-754).  Each entry point below names the
+ * reference member it replaces.  Plain C types and host pointers only; every call returns
+ * an int status (CF_OK on success) and never exits the process (the reference's
+ * CUDA_CHECK calls exit(1), ParticleSimulation.cu:10-18).  A handle is not thread-safe;
+ * separate handles are independent.
+ *
+ * Semantics that differ from the reference on purpose:
+ *   - the step is a Jacobi update (every particle reads the pre-step state); the
+ *     reference updates `particles[idx]` in place while other threads read it
+ *     (ParticleSimulation.cu:89 vs :164), which makes its output schedule-dependent;
+ *   - proximity-graph edges are returned as (i, j) index pairs in canonical order
+ *     (owner i ascending, then distance rank) instead of the atomicAdd order of
+ *     ParticleSimulation.cu:250;  cf_get_line_vertices expands them to the reference's
+ *     6-float line vertices (:253-275);
+ *   - max_connections in (16, 32] uses a candidate list of 2*max_connections entries,
+ *     where the reference overflows its 32-entry local array (:210-215).
+ */
+#ifndef CELLFLOW_H
+#define CELLFLOW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CF_OK 0
+#define CF_ERR_ARG 1        /* invalid argument (sizes, types, null pointers) */
+#define CF_ERR_DEVICE 2     /* no usable GPU, or a HIP runtime error */
+#define CF_ERR_CAPACITY 3   /* output buffer too small; the needed size is returned */
+#define CF_ERR_NONFINITE 4  /* a position or velocity became NaN/Inf during the step */
+#define CF_ERR_HANDLE 5     /* handle is null, destroyed, or of the wrong kind */
+#define CF_ERR_STATE 6      /* call not valid in the current state (e.g. no edges built) */
+
+#define CF_MAX_TYPES 10     /* MAX_PARTICLE_TYPES, cuda-native/include/SimulationParams.h:64 */
+#define CF_MAX_CONNECTIONS 32
+
+/* struct Particle, SimulationParams.h:6-12: 44 bytes, float3 members 4-byte aligned. */
+typedef struct cf_particle {
+  float pos[3];
+  float vel[3];
+  float acc[3];
+  uint32_t ptype;
+  float pad;
+} cf_particle;
+
+/* struct ParticleColor, SimulationParams.h:59-61. */
+typedef struct cf_color {
+  float r, g, b;
+} cf_color;
+
+/* The fields of SimulationParams (SimulationParams.h:14-57) that the step reads
+ * (simulateParticlesKernel, ParticleSimulation.cu:69-166).  Render-only fields are not
+ * part of the step and are omitted. */
+typedef struct cf_params {
+  float radius;                 /* SimulationParams.h:15 */
+  float delta_t;                /* :16 */
+  float friction;               /* :17 */
+  float repulsion;              /* :18 */
+  float attraction;             /* :19 */
+  float k;                      /* :20 */
+  float balance;                /* :21 */
+  float canvas_width;           /* :22 */
+  float canvas_height;          /* :23 */
+  float canvas_depth;           /* :24 */
+  int num_particle_types;       /* :26 */
+  float ratio_with_lfo;         /* :27 */
+  float force_multiplier;       /* :28 */
+  int max_expected_neighbors;   /* :29 */
+} cf_params;
+
+/* Proximity-graph settings (CellFlowWidget.cpp:569-575 passes these per frame). */
+typedef struct cf_edge_opts {
+  float distance;               /* proximityDistance */
+  int max_connections;          /* maxConnectionsPerParticle, 1..32 */
+} cf_edge_opts;
+
+/* Launch/grid choices made by the autotuner (no reference counterpart). */
+typedef struct cf_tuning {
+  int cell_divisor;             /* 1: cell >= max cutoff, 27-cell stencil; 2: half cells, 125 */
+  int block_size;               /* threads per workgroup of the force kernel: 64..1024 */
+  int steps_per_graph;          /* steps captured per HIP graph replay (>= 1) */
+} cf_tuning;
+
+typedef struct cf_sim cf_sim;
+typedef struct cf_ensemble cf_ensemble;
+
+/* ---- lifetime -------------------------------------------------------------------- */
+
+/* ParticleSimulation(int particleCount), ParticleSimulation.cuh:12 / .cu:427-435.
+ * Allocates device buffers on the current HIP device.  The state starts at zero with
+ * `num_types` types; load it with cf_set_particles or cf_init_particles (the reference
+ * seeds it from time(nullptr), .cu:480). */
+int cf_create(int particle_count, int num_types, cf_sim** out);
+/* ~ParticleSimulation, .cu:447-449. */
+int cf_destroy(cf_sim* sim);
+/* setParticleCount, .cu:564-572: reallocates; neighbour counts return to zero. */
+int cf_set_particle_count(cf_sim* sim, int particle_count);
+/* setNumParticleTypes, .cu:574-579 (the caller reloads table, radii and state). */
+int cf_set_num_particle_types(cf_sim* sim, int num_types);
+int cf_get_particle_count(cf_sim* sim, int* out);      /* getParticleCount, .cuh:40 */
+int cf_get_num_particle_types(cf_sim* sim, int* out); /* getNumParticleTypes, .cuh:43 */
+
+/* ---- particle state ---------------------------------------------------------------- */
+
+/* No reference counterpart (state only comes from initializeParticlesKernel there);
+ * needed to drive the engine from a given state. Resets neighbour counts to zero. */
+int cf_set_particles(cf_sim* sim, const cf_particle* particles, int count);
+/* getParticleData, .cu:558-562: the full 44-byte records in index order. */
+int cf_get_particles(cf_sim* sim, cf_particle* particles, int count);
+/* Structure-of-arrays forms: pos/vel are count x 3 floats, type is count uint32. */
+int cf_set_state_soa(cf_sim* sim, const float* pos, const float* vel, const uint32_t* type,
+                     int count);
+int cf_get_state_soa(cf_sim* sim, float* pos, float* vel, uint32_t* type, int count);
+/* The neighbour-count ping-pong buffers d_neighborCountsA/B (.cu:456-462, :544-555):
+ * the counts the next step reads as `prevNeighborCount` (.cu:136). */
+int cf_set_neighbor_counts(cf_sim* sim, const int32_t* counts, int count);
+int cf_get_neighbor_counts(cf_sim* sim, int32_t* counts, int count);
+/* Seeded replacement for initCurandKernel + initializeParticlesKernel (.cu:21-66):
+ * positions uniform in the central min(spawn_region, canvas) cube, zero velocity and
+ * acceleration, type uniform in [0, num_types).  Counter-based (SplitMix64), so the same
+ * seed gives the same state on every device. */
+int cf_init_particles(cf_sim* sim, uint64_t seed, const float canvas[3], float spawn_region);
+
+/* ---- force table and per-type radius modifiers -------------------------------------- */
+
+/* d_forceTable upload (.cu:529-530) with a caller-given final num_types^2 table
+ * (row = own type, column = other type); values are used as given (no clamp). */
+int cf_set_force_table(cf_sim* sim, const float* table, int num_types);
+int cf_get_force_table(cf_sim* sim, float* table, int count);
+/* getRawForceTableValues (.cuh:47) is a writable pointer in the reference; here it is a
+ * set/get pair on the host-side raw table. */
+int cf_set_raw_force_table(cf_sim* sim, const float* raw, int count);
+int cf_get_raw_force_table(cf_sim* sim, float* raw, int count);
+/* updateForceTable(range, bias, offset), .cu:521-531: table = clamp(tanh(raw*offset)*range
+ * + bias, -1, 1) from the raw table. */
+int cf_update_force_table(cf_sim* sim, float range, float bias, float offset);
+/* radioByType: initializeRadioByType upload (.cu:533-539) with given values. */
+int cf_set_radio_by_type(cf_sim* sim, const float* values, int count);
+/* setRadioByTypeValue, .cu:615-622: out-of-range indices are ignored (returns CF_OK). */
+int cf_set_radio_by_type_value(cf_sim* sim, int index, float value);
+/* getRadioByType, .cu:607-613. */
+int cf_get_radio_by_type(cf_sim* sim, float* values, int count);
+/* rotateRadioByType, .cu:594-605. */
+int cf_rotate_radio_by_type(cf_sim* sim);
+
+/* ---- the step and the proximity graph ------------------------------------------------ */
+
+/* `nsteps` calls of simulate(params) (.cu:541-556).  When `edges` is non-null the
+ * proximity graph of every post-step state is built too, as CellFlowWidget::paintGL does
+ * after each step (CellFlowWidget.cpp:551-575); the graph of the last state is kept for
+ * cf_get_edges.  Synchronous like the reference (it waits for the GPU); returns
+ * CF_ERR_NONFINITE if any position or velocity became NaN/Inf. */
+int cf_simulate(cf_sim* sim, const cf_params* params, int nsteps, const cf_edge_opts* edges);
+/* generateProximityGraph, .cu:624-687, on the current state (the GL VBO is replaced by
+ * an internal buffer read with cf_get_edges / cf_get_line_vertices). */
+int cf_build_edges(cf_sim* sim, float distance, int max_connections);
+/* The last built edge list: `pairs` holds capacity (i, j) int32 pairs.  Returns
+ * CF_ERR_CAPACITY with *count set to the needed number when capacity is too small. */
+int cf_get_edges(cf_sim* sim, int32_t* pairs, long long capacity, long long* count);
+/* The reference's line-vertex layout (.cu:253-275): per edge two vertices of
+ * (x, y, z, r, g, b), colour = colors[ptype_i % num_types]; *vertex_count = 2 * edges
+ * (the value generateProximityGraph returns through outVertexCount). */
+int cf_get_line_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float* out,
+                         long long capacity_floats, long long* vertex_count);
+
+/* ---- introspection (parity checks) -------------------------------------------------- */
+
+/* Grid of the current binning: cells per axis and the reach of the force stencil. */
+int cf_get_grid(cf_sim* sim, int dims[3], int* reach);
+/* The canonical (cell, index) order of the current binning: for slot k, the particle
+ * index and its linear cell id (x fastest). */
+int cf_get_sort_order(cf_sim* sim, uint32_t* cell_of_slot, uint32_t* index_of_slot, int count);
+/* Per-kernel average durations (ms) over `nsteps` steps timed with HIP events on the
+ * engine's stream: names is a comma-separated list written into names_buf. */
+int cf_profile_steps(cf_sim* sim, const cf_params* params, int nsteps, const cf_edge_opts* edges,
+                     double* ms_out, int max_kernels, int* nkernels, char* names_buf,
+                     int names_cap);
+
+/* ---- tuning ------------------------------------------------------------------------- */
+
+int cf_set_tuning(cf_sim* sim, const cf_tuning* tuning);
+int cf_get_tuning(cf_sim* sim, cf_tuning* tuning);
+
+/* ---- host-only helpers (no GPU needed) ---------------------------------------------- */
+
+/* The grid rule: cells per axis for a parameter set, type radii and particle count. */
+int cf_grid_dims(const cf_params* params, const float* radio_by_type, int particle_count,
+                 int cell_divisor, int dims[3]);
+/* Smallest float t with sqrtf(t) >= eff_radius: the engine tests s + 1e-4f < t, which is
+ * the reference's sqrtf(s + 0.0001f) < effectiveRadius (.cu:105, :112) exactly. */
+float cf_cutoff_threshold(float eff_radius);
+/* The updateForceTable transform (.cu:521-531) on host arrays. */
+void cf_force_table_transform(const float* raw, int count, float range, float bias,
+                              float offset, float* out);
+const char* cf_last_error_string(void);
+int cf_version(void);
+int cf_device_count(int* count);
+
+/* ---- batched ensemble (no reference counterpart) ------------------------------------ */
+
+/* nsims independent simulations stepped together by the same kernels as cf_simulate.
+ * Each has its own particle count, type count, force table, radii and parameters. */
+int cf_ensemble_create(int nsims, const int* particle_counts, const int* num_types,
+                       cf_ensemble** out);
+int cf_ensemble_destroy(cf_ensemble* ens);
+int cf_ensemble_set_sim(cf_ensemble* ens, int sim, const cf_params* params, const float* table,
+                        const float* radio_by_type, const cf_particle* particles, int count);
+int cf_ensemble_simulate(cf_ensemble* ens, int nsteps);
+int cf_ensemble_get_particles(cf_ensemble* ens, int sim, cf_particle* particles, int count);
+int cf_ensemble_get_neighbor_counts(cf_ensemble* ens, int sim, int32_t* counts, int count);
+int cf_ensemble_set_tuning(cf_ensemble* ens, const cf_tuning* tuning);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CELLFLOW_H */

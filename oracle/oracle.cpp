@@ -1,0 +1,407 @@
+// CellFlow step oracle — TEST INFRASTRUCTURE ONLY.
+//
+// A scalar CPU restatement of the reference particle-life step and proximity-graph
+// builder.  It exists to check the HIP engine; nothing in the product links it.
+// Only the test suite, the smoke check and the benchmark's CPU-baseline leg load it.
+//
+// Arithmetic contract (shared with the HIP kernels, fixed by the engine's spec):
+//   * discrete decisions (cutoff predicate, neighbour count, edge predicate, cell
+//     binning) are evaluated in FP32 with the op order written below, compiled with
+//     -ffp-contract=off and explicit fmaf, so they are bit-identical to the GPU's;
+//   * everything continuous (forces, integration, wrap) is evaluated in FP64.
+//
+// Functions and the reference lines they restate (paths relative to the reference root):
+//   orc_pair_in_range   cuda-native/src/ParticleSimulation.cu:92-112  (min-image, softened
+//                       distance, per-type effective radius, strict cutoff)
+//   orc_step            ParticleSimulation.cu:69-166 with Jacobi semantics (every read from
+//                       the pre-step state; the reference updates in place and races)
+//   orc_edges           ParticleSimulation.cu:187-246 (same type, j>i, non-periodic, first
+//                       2*maxC by index, stable sort by distance, keep maxC); output is the
+//                       (owner, rank) order instead of the reference's atomicAdd order
+//   orc_force_table     ParticleSimulation.cu:521-531 (updateForceTable transform)
+//   orc_bin / orc_order the engine's own uniform-grid rule (no reference counterpart: the
+//                       reference enumerates all pairs, :86)
+//   orc_rollout         repeated orc_step with the neighbour-count ping-pong of :541-556
+//   orc_hist / orc_loss the differentiable-rollout loss (engine extension, no reference code)
+//
+// Pinning: every function above is checked in tests/test_oracle.py against closed forms
+// (two-particle force, isolated particle, balance=1, zero table), invariants (pair
+// symmetry, permutation and periodic-shift equivariance), brute force on tiny inputs
+// (all-pairs vs grid enumeration, edge truncation cases built by hand) and the four
+// reference preset files.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+struct orc_params {
+  float radius, dt, friction, repulsion, attraction, k, balance;
+  float L[3];
+  int T;
+  float ratio;
+  float force_multiplier;
+  int max_expected_neighbors;
+};
+
+}  // extern "C"
+
+namespace {
+
+inline float min_image_f(float d, float L) {
+  // ParticleSimulation.cu:97-102: one shift each way, in this order.
+  if (d > L * 0.5f) d -= L;
+  if (d < -L * 0.5f) d += L;
+  return d;
+}
+
+inline double min_image_d(double d, double L) {
+  if (d > L * 0.5) d -= L;
+  if (d < -L * 0.5) d += L;
+  return d;
+}
+
+inline float eff_radius_f(const orc_params& p, const float* rbt, unsigned ti, unsigned tj) {
+  // ParticleSimulation.cu:108-110, FP32, no contraction.
+  float ri = p.radius * (1.0f + rbt[ti] * p.ratio);
+  float rj = p.radius * (1.0f + rbt[tj] * p.ratio);
+  return (ri + rj) * 0.5f;
+}
+
+inline double eff_radius_d(const orc_params& p, const float* rbt, unsigned ti, unsigned tj) {
+  double r = p.radius, rho = p.ratio;
+  double ri = r * (1.0 + (double)rbt[ti] * rho);
+  double rj = r * (1.0 + (double)rbt[tj] * rho);
+  return (ri + rj) * 0.5;
+}
+
+// FP32 decision for one ordered pair (i sees j); returns the squared FP32 distance too.
+inline bool pair_in_range_f(const orc_params& p, const float* rbt, const float* xi, const float* xj,
+                            unsigned ti, unsigned tj) {
+  float dx = min_image_f(xj[0] - xi[0], p.L[0]);
+  float dy = min_image_f(xj[1] - xi[1], p.L[1]);
+  float dz = min_image_f(xj[2] - xi[2], p.L[2]);
+  float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  float dist = sqrtf(s + 0.0001f);  // ParticleSimulation.cu:105
+  return dist < eff_radius_f(p, rbt, ti, tj);  // :112, strict
+}
+
+struct Grid {
+  int n[3];
+  double h[3];
+  std::vector<int> start;   // cell -> first index into order
+  std::vector<int> order;   // particle ids sorted by cell
+};
+
+// A coarse grid for the oracle's own accelerated enumeration.  Independent of the
+// engine's grid: it only has to cover every pair within `reach_dist`.
+Grid build_grid(int n, const float* pos, const float* L, double reach_dist) {
+  Grid g;
+  for (int a = 0; a < 3; ++a) {
+    int na = (int)std::floor((double)L[a] / (reach_dist * 1.01));
+    na = std::max(1, std::min(na, 512));
+    g.n[a] = na;
+    g.h[a] = (double)L[a] / na;
+  }
+  long long nc = (long long)g.n[0] * g.n[1] * g.n[2];
+  while (nc > std::max<long long>(8LL * n, 64)) {
+    int a = 0;
+    if (g.n[1] > g.n[a]) a = 1;
+    if (g.n[2] > g.n[a]) a = 2;
+    g.n[a] = std::max(1, g.n[a] - 1);
+    g.h[a] = (double)L[a] / g.n[a];
+    nc = (long long)g.n[0] * g.n[1] * g.n[2];
+  }
+  std::vector<int> cell(n);
+  std::vector<int> cnt(nc + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    int c[3];
+    for (int a = 0; a < 3; ++a) {
+      double u = (double)pos[3 * i + a] / g.h[a];
+      int ca = (u >= 0) ? (int)std::floor(u) : 0;
+      c[a] = std::min(std::max(ca, 0), g.n[a] - 1);
+    }
+    cell[i] = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+    cnt[cell[i] + 1]++;
+  }
+  for (long long c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+  g.start = cnt;
+  g.order.assign(n, 0);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < n; ++i) g.order[fill[cell[i]]++] = i;
+  return g;
+}
+
+// Visit every particle in the cells within `r` cells of particle i's cell (periodic or
+// not); each cell is visited at most once.
+template <class F>
+void for_each_near(const Grid& g, const float* pos, int i, int r, bool periodic, F&& f) {
+  int c[3];
+  for (int a = 0; a < 3; ++a) {
+    double u = (double)pos[3 * i + a] / g.h[a];
+    int ca = (u >= 0) ? (int)std::floor(u) : 0;
+    c[a] = std::min(std::max(ca, 0), g.n[a] - 1);
+  }
+  std::vector<int> offs[3];
+  for (int a = 0; a < 3; ++a) {
+    if (periodic && 2 * r + 1 >= g.n[a]) {
+      for (int v = 0; v < g.n[a]; ++v) offs[a].push_back(v);
+    } else {
+      for (int o = -r; o <= r; ++o) {
+        int v = c[a] + o;
+        if (periodic) v = ((v % g.n[a]) + g.n[a]) % g.n[a];
+        else if (v < 0 || v >= g.n[a]) continue;
+        offs[a].push_back(v);
+      }
+    }
+  }
+  for (int z : offs[2])
+    for (int y : offs[1])
+      for (int x : offs[0]) {
+        int cell = x + g.n[0] * (y + g.n[1] * z);
+        for (int k = g.start[cell]; k < g.start[cell + 1]; ++k) f(g.order[k]);
+      }
+}
+
+double max_eff_radius(const orc_params& p, const float* rbt) {
+  double m = 0;
+  for (int a = 0; a < p.T; ++a)
+    for (int b = 0; b < p.T; ++b) m = std::max(m, (double)eff_radius_f(p, rbt, a, b));
+  return m;
+}
+
+struct PairTerm {
+  double f[3];
+  double absnet;
+};
+
+// FP64 force of j on i for a pair already accepted by the FP32 predicate.
+inline PairTerm pair_force_d(const orc_params& p, const float* table, const float* rbt,
+                             const double* xi, const double* xj, unsigned ti, unsigned tj) {
+  double d[3];
+  for (int a = 0; a < 3; ++a) d[a] = min_image_d(xj[a] - xi[a], (double)p.L[a]);
+  double s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double dist = std::sqrt(s + (double)0.0001f);
+  double eff = eff_radius_d(p, rbt, ti, tj);
+  double fv = table[ti * p.T + tj];                 // :116
+  double r = dist / eff;                            // :119
+  double net = ((double)p.repulsion * std::exp(-(double)p.k * r * r) - (double)p.attraction * r) * fv;  // :120-125
+  PairTerm t;
+  for (int a = 0; a < 3; ++a) t.f[a] = d[a] / dist * net;  // :128-131
+  t.absnet = std::fabs(net);
+  return t;
+}
+
+void set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+}  // namespace
+
+extern "C" {
+
+// FP32 cutoff decision for the ordered pair (i, j), exactly as the engine evaluates it.
+int orc_pair_in_range(const orc_params* p, const float* rbt, const float* xi, const float* xj,
+                      unsigned ti, unsigned tj) {
+  return pair_in_range_f(*p, rbt, xi, xj, ti, tj) ? 1 : 0;
+}
+
+// FP32 effective radius of a type pair (ParticleSimulation.cu:108-110).
+float orc_eff_radius(const orc_params* p, const float* rbt, unsigned ti, unsigned tj) {
+  return eff_radius_f(*p, rbt, ti, tj);
+}
+
+// One Jacobi step.  pos/vel are FP64 (an FP32 state promoted exactly, or an FP64 rollout
+// state); the FP32 decisions use (float)pos.  mode 0 enumerates all j != i in ascending
+// order (ParticleSimulation.cu:86); mode 1 enumerates a covering grid (same neighbour set).
+// fabs_out[i] (optional) receives sum_j |net_ij|, the magnitude scale of the force sum.
+int orc_step(int n, const double* pos, const double* vel, const uint32_t* type, const int32_t* prev,
+             const orc_params* pp, const float* table, const float* rbt, double* pos_out,
+             double* vel_out, double* acc_out, int32_t* count_out, double* fabs_out, int mode,
+             int nthreads) {
+  const orc_params& p = *pp;
+  if (n < 0 || p.T <= 0 || p.T > 10) return 1;
+  for (int i = 0; i < n; ++i)
+    if (type[i] >= (uint32_t)p.T) return 1;
+  set_threads(nthreads);
+  std::vector<float> posf(3 * (size_t)n);
+  for (size_t k = 0; k < 3 * (size_t)n; ++k) posf[k] = (float)pos[k];
+  Grid g;
+  double reach = max_eff_radius(p, rbt);
+  const bool use_grid = (mode == 1) && reach > 0;
+  if (use_grid) g = build_grid(n, posf.data(), p.L, reach);
+
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    const unsigned ti = type[i];
+    double F[3] = {0, 0, 0};
+    double fabs_sum = 0;
+    int count = 0;
+    auto visit = [&](int j) {
+      if (j == i) return;  // :87
+      const unsigned tj = type[j];
+      if (!pair_in_range_f(p, rbt, &posf[3 * i], &posf[3 * j], ti, tj)) return;
+      count++;  // :113
+      PairTerm t = pair_force_d(p, table, rbt, &pos[3 * i], &pos[3 * j], ti, tj);
+      for (int a = 0; a < 3; ++a) F[a] += t.f[a];
+      fabs_sum += t.absnet;
+    };
+    if (use_grid) {
+      for_each_near(g, posf.data(), i, 1, true, visit);
+    } else if (reach > 0) {
+      for (int j = 0; j < n; ++j) visit(j);
+    }
+    // :136-143 adaptive factor from the current and previous neighbour counts.
+    double avg = (double)(count + prev[i]) * 0.5;
+    double dens = std::fmin(avg / (double)p.max_expected_neighbors, 1.0);  // fminf semantics (NaN -> 1)
+    double fac = 1.0 - (1.0 - (double)p.balance) * dens;
+    for (int a = 0; a < 3; ++a) F[a] *= (double)p.force_multiplier * fac;
+    for (int a = 0; a < 3; ++a) {
+      double v = vel[3 * i + a] * (double)p.friction + F[a] * (double)p.dt;  // :149-151
+      double x = pos[3 * i + a] + v * (double)p.dt;                            // :154-156
+      double L = (double)p.L[a];
+      x = std::fmod(x + L, L);                                                 // :159-161
+      vel_out[3 * i + a] = v;
+      pos_out[3 * i + a] = x;
+      if (acc_out) acc_out[3 * i + a] = F[a];                                  // :146
+    }
+    count_out[i] = count;                                                      // :165
+    if (fabs_out) fabs_out[i] = fabs_sum;
+  }
+  return 0;
+}
+
+// `steps` Jacobi steps in FP64 with FP32 decisions, carrying the count ping-pong
+// (ParticleSimulation.cu:541-556).  pos/vel/prev are updated in place.
+int orc_rollout(int n, double* pos, double* vel, const uint32_t* type, int32_t* prev,
+                const orc_params* p, const float* table, const float* rbt, int steps, int mode,
+                int nthreads) {
+  std::vector<double> pos2(3 * (size_t)n), vel2(3 * (size_t)n);
+  std::vector<int32_t> cnt(n);
+  for (int s = 0; s < steps; ++s) {
+    int rc = orc_step(n, pos, vel, type, prev, p, table, rbt, pos2.data(), vel2.data(), nullptr,
+                      cnt.data(), nullptr, mode, nthreads);
+    if (rc) return rc;
+    std::memcpy(pos, pos2.data(), sizeof(double) * 3 * n);
+    std::memcpy(vel, vel2.data(), sizeof(double) * 3 * n);
+    std::memcpy(prev, cnt.data(), sizeof(int32_t) * n);
+  }
+  return 0;
+}
+
+// Engine grid rule restated: cell = clamp(floor(x * inv_h), 0, n-1) per axis in FP32 with
+// inv_h = (float)n / L, linear id x-fastest.
+int orc_bin(int n, const float* pos, const int* dims, const float* L, uint32_t* cell_out) {
+  float inv_h[3];
+  for (int a = 0; a < 3; ++a) {
+    if (dims[a] < 1) return 1;
+    inv_h[a] = (float)dims[a] / L[a];
+  }
+  for (int i = 0; i < n; ++i) {
+    int c[3];
+    for (int a = 0; a < 3; ++a) {
+      float u = pos[3 * i + a] * inv_h[a];
+      int ca = (u >= 0.0f) ? (int)floorf(u) : 0;
+      if (!(u < (float)dims[a])) ca = (u >= 0.0f) ? dims[a] - 1 : 0;
+      c[a] = std::min(std::max(ca, 0), dims[a] - 1);
+    }
+    cell_out[i] = (uint32_t)(c[0] + dims[0] * (c[1] + dims[1] * c[2]));
+  }
+  return 0;
+}
+
+// Canonical (cell, id) order: order_out[k] = id of the k-th particle.
+int orc_order(int n, const uint32_t* cell, uint32_t* order_out) {
+  std::vector<uint32_t> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return cell[a] < cell[b]; });
+  std::memcpy(order_out, idx.data(), sizeof(uint32_t) * n);
+  return 0;
+}
+
+// Proximity graph (ParticleSimulation.cu:187-246).  Output: pairs (i, j) in canonical
+// order (owner i ascending, then distance rank).  mode 0 is the reference's own scan
+// (j = i+1.. while found < 2*maxC); mode 1 uses a non-periodic grid (same result).
+// Returns 0, or 3 when `capacity` pairs are too few (count_out then holds the need).
+int orc_edges(int n, const float* pos, const uint32_t* type, float dist, int max_conn, int mode,
+              int32_t* pairs_out, long long capacity, long long* count_out, int nthreads) {
+  set_threads(nthreads);
+  const float d2 = dist * dist;                 // ParticleSimulation.cu:660
+  const int maxc = std::min(max_conn, 32);      // :212
+  const int cap = 2 * maxc;                     // :215 (capacity of the candidate list)
+  std::vector<std::vector<std::pair<float, int>>> kept(n);
+  if (maxc <= 0 || n <= 1) {
+    *count_out = 0;
+    return 0;
+  }
+  auto d2of = [&](int i, int j) {
+    float dx = pos[3 * j] - pos[3 * i];
+    float dy = pos[3 * j + 1] - pos[3 * i + 1];
+    float dz = pos[3 * j + 2] - pos[3 * i + 2];
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  };
+  Grid g;
+  float Lb[3] = {0, 0, 0};
+  if (mode == 1) {
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) Lb[a] = std::max(Lb[a], pos[3 * i + a]);
+    for (int a = 0; a < 3; ++a) Lb[a] = std::nextafter(Lb[a], 1e30f) * 1.001f + 1e-30f;
+    g = build_grid(n, pos, Lb, std::max((double)dist, 1e-6));
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    std::vector<std::pair<float, int>> cand;  // (d2, j) in ascending j
+    if (mode == 0) {
+      for (int j = i + 1; j < n && (int)cand.size() < cap; ++j) {  // :215
+        if (type[j] != type[i]) continue;                          // :219
+        float s = d2of(i, j);
+        if (s < d2) cand.push_back({s, j});                         // :227-231
+      }
+    } else {
+      for_each_near(g, pos, i, 1, false, [&](int j) {
+        if (j <= i || type[j] != type[i]) return;
+        float s = d2of(i, j);
+        if (s < d2) cand.push_back({s, j});
+      });
+      std::sort(cand.begin(), cand.end(), [](auto& a, auto& b) { return a.second < b.second; });
+      if ((int)cand.size() > cap) cand.resize(cap);
+    }
+    // :235-243 stable insertion sort by distance (ties keep index order).
+    std::stable_sort(cand.begin(), cand.end(), [](auto& a, auto& b) { return a.first < b.first; });
+    if ((int)cand.size() > maxc) cand.resize(maxc);  // :246
+    kept[i] = std::move(cand);
+  }
+  long long total = 0;
+  for (int i = 0; i < n; ++i) total += (long long)kept[i].size();
+  *count_out = total;
+  if (total > capacity) return 3;
+  long long w = 0;
+  for (int i = 0; i < n; ++i)
+    for (auto& e : kept[i]) {
+      pairs_out[2 * w] = i;
+      pairs_out[2 * w + 1] = e.second;
+      ++w;
+    }
+  return 0;
+}
+
+// updateForceTable (ParticleSimulation.cu:521-531): clamp(tanh(raw*offset)*range+bias, -1, 1).
+// The operands are float, so the float overloads of tanh/fmax/fmin apply; all in FP32.
+void orc_force_table(const float* raw, int count, float range, float bias, float offset, float* out) {
+  for (int i = 0; i < count; ++i) {
+    float transformed = tanhf(raw[i] * offset) * range + bias;
+    out[i] = fmaxf(-1.0f, fminf(1.0f, transformed));
+  }
+}
+
+}  // extern "C"
