@@ -1,0 +1,151 @@
+"""Benchmark: particle-steps/s of the plain step at 100k particles with the proximity graph on.
+
+Workload (cellflow_amd/configs/c2_100k_edges.json): 100,000 particles, 8 types, periodic
+cube of side 2.0, force matrix N(0.1, 0.6), radius 0.04, ratio 0.3, density factor on,
+edges within 0.02 capped at 16 per particle.  Inputs are synthetic (seeded SplitMix64)
+and resident on the GPU before the timed region.  One timed step = binning + force +
+integrate + stable re-sort + the proximity graph of the post-step state.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (k_force), timed live with
+HIP events on the engine's stream; `cpu_baseline` times the oracle (FP64 scalar C++
+restatement, OpenMP) on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+STEP_BYTES = 60.0         # compulsory state traffic per particle-step
+EDGE_BYTES = 20.0         # + per particle for the edge pass
+EDGE_PAIR_BYTES = 8.0     # + per emitted edge (an int32 pair)
+METRIC = "particle-steps/s @100k; batched-ensemble & adjoint speedup vs plain step"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(path):
+    """HBM bytes per k_force launch from the committed rocprofv3 PMC summary, if present."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(cfg, budget_s=12.0):
+    import numpy as np
+
+    import oracle
+    from cellflow_amd import configs
+    st = configs.make_state(cfg)
+    phys = configs.physics(cfg)
+    e = cfg["edges"]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pos = st["pos"].astype(np.float64)
+    vel = st["vel"].astype(np.float64)
+    prev = st["prev"].copy()
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        out = oracle.step(pos, vel, st["type"], prev, phys, st["table"], st["rbt"], mode=1,
+                          nthreads=threads)
+        pos, vel, prev = out["pos"], out["vel"], out["count"]
+        oracle.edges(pos.astype(np.float32), st["type"], e["distance"], e["max_connections"],
+                     mode=1, nthreads=threads)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    return {"value": cfg["n"] * steps / el, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} steps of the 100k workload (grid enumeration, FP64, edges on), "
+                      f"{el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-steps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this engine "
+              "runs one simulation per GPU; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cellflow_amd import ParticleSimulation, SimulationParams, configs, particles_from_state
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    cfg = configs.load("c2_100k_edges")
+    st = configs.make_state(cfg)
+    phys = configs.physics(cfg)
+    params = SimulationParams.from_physics(phys)
+    e = cfg["edges"]
+    edges = (e["distance"], e["max_connections"])
+    n = cfg["n"]
+
+    sim = ParticleSimulation(n, cfg["num_types"])
+    sim.setForceTable(st["table"])
+    sim.setRadioByType(st["rbt"])
+    sim.setParticles(particles_from_state(st["pos"], st["vel"], st["type"]))
+    log(f"[bench] warmup {args.warmup} steps")
+    sim.simulate(params, args.warmup, edges=edges)
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    sim.simulate(params, args.steps, edges=edges)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nedges = len(sim.getEdges())
+    rate = n * args.steps / dt
+
+    log(f"[bench] {rate:.4g} particle-steps/s; per-kernel timing over {args.profile_steps} steps")
+    kt = sim.profileSteps(params, args.profile_steps, edges=edges)
+    dominant = max(kt, key=kt.get)
+    force_ms = kt["force"]
+    alg_bytes = n * (STEP_BYTES + EDGE_BYTES) + EDGE_PAIR_BYTES * nedges
+    achieved = alg_bytes / (force_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_force", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": read_traffic(os.path.join(ROOT, "profiles", "k_force_c2_traffic.txt")),
+            "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": force_ms,
+            "dominant_kernel": dominant,
+            "kernel_ms": {k: round(v, 5) for k, v in kt.items()}}
+
+    out = {"metric": METRIC, "value": rate, "unit": "particle-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": cfg["name"], "particles": n, "types": cfg["num_types"],
+                      "box": cfg["canvas"][0], "radius": cfg["radius"], "ratio": cfg["ratio"],
+                      "edges": e, "edges_last_step": nedges, "grid": sim.getGrid()[0]},
+           "roofline": roof}
+    sim.close()
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

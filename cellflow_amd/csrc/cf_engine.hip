@@ -166,7 +166,7 @@ struct Engine {
   long long edges_count = -1;  // -1: no edge list built
   int edges_maxc = 0;
   uint32_t* slot_of = nullptr;
-  cf_tuning tun{1, 256, 32};
+  cf_tuning tun{1, 0, 16, 256, 32};
   uint32_t* hsmall = nullptr;  // pinned readback words
 
   ~Engine() { release(); }
@@ -300,15 +300,29 @@ struct Engine {
       d.rep = p.repulsion;
       d.att = p.attraction;
       d.kexp2 = -p.k * 1.44269504088896340736f;
-      for (int a = 0; a < h.T; ++a)
+      d.hx = d.Lx / (float)d.nx;
+      d.hy = d.Ly / (float)d.ny;
+      d.hz = d.Lz / (float)d.nz;
+      d.edge_inside = 0;
+      const float ed2 = er.distance * er.distance;  // ParticleSimulation.cu:660
+      for (int a = 0; a < h.T; ++a) {
+        // Pruning reach of owner type a: every accepted pair has s + 1e-4f < thr, so its
+        // squared distance is below thr - 1e-4 (FP32 rounding absorbed by the margin).
+        double reach2 = 0.0;
         for (int bb = 0; bb < h.T; ++bb) {
           PairConst& pc = hpairs[d.tab + a * h.T + bb];
           const float e = h.have_prm ? eff_radius(p, h.rbt.data(), a, bb) : 0.0f;
           pc.thr = cutoff_threshold(e);
           pc.fv = h.table[a * h.T + bb];
           pc.inv_eff = (e > 0.0f) ? 1.0f / e : 0.0f;
-          pc.pad = 0.0f;
+          reach2 = std::max(reach2, (double)pc.thr - 1e-4);
         }
+        if (er.on) reach2 = std::max(reach2, (double)ed2);
+        const float cut2 = (float)(reach2 * (1.0 + 1e-3) + 1e-12);
+        for (int bb = 0; bb < h.T; ++bb) hpairs[d.tab + a * h.T + bb].pad = cut2;
+        if (er.on && ed2 + 1e-4f < hpairs[d.tab + a * h.T + a].thr) d.edge_inside |= (1 << a);
+      }
+      d.ident = h.have_prm ? 0 : 1;
       d.edge_on = er.on ? 1 : 0;
       d.edge_d2 = er.distance * er.distance;
       d.edge_maxc = er.maxc;
@@ -546,7 +560,8 @@ struct Engine {
   int step(bool fused_edges, int cap, hipEvent_t* ev = nullptr) {
     const int B = 1 - cur;
     ForceArgs fa = force_args(fused_edges, cap);
-    HIPR(launch_force(fa, fused_edges, tun.block_size, st));
+    HIPR(launch_force(fa, fused_edges, tun.block_size, tun.force_variant,
+                      std::max(1, std::min(2, tun.cell_divisor)), tun.lanes_per_particle, st));
     if (ev) HIPR(hipEventRecord(ev[0], st));
     HIPR(launch_scan(plan(fused_edges ? 1 : 0, b.start[B]), st));
     if (ev) HIPR(hipEventRecord(ev[1], st));
@@ -925,7 +940,7 @@ int edge_req(const cf_edge_opts* e, cf::EdgeReq* r) {
 int make_engine(std::unique_ptr<cf::Engine>& out, int kind, int nsims, const int* counts,
                 const int* types) {
   for (int s = 0; s < nsims; ++s) {
-    if (counts[s] < 1 || counts[s] > (1 << 28))
+    if (counts[s] < 1 || counts[s] > (1 << 26))
       return cf::fail(CF_ERR_ARG, "particle count %d out of range", counts[s]);
     if (types[s] < 1 || types[s] > CF_MAX_TYPES)
       return cf::fail(CF_ERR_ARG, "num_types %d out of range 1..%d", types[s], CF_MAX_TYPES);
@@ -1271,6 +1286,11 @@ static int check_tuning(const cf_tuning* t) {
     return cf::fail(CF_ERR_ARG, "cell_divisor must be 1 or 2");
   if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
     return cf::fail(CF_ERR_ARG, "block_size must be 64, 128 or 256");
+  if (t->force_variant < 0 || t->force_variant > 2)
+    return cf::fail(CF_ERR_ARG, "force_variant must be 0, 1 or 2");
+  if (t->lanes_per_particle != 4 && t->lanes_per_particle != 8 && t->lanes_per_particle != 16 &&
+      t->lanes_per_particle != 32)
+    return cf::fail(CF_ERR_ARG, "lanes_per_particle must be 4, 8, 16 or 32");
   if (t->steps_per_graph < 1) return cf::fail(CF_ERR_ARG, "steps_per_graph must be >= 1");
   return CF_OK;
 }

@@ -20,7 +20,7 @@ struct PairConst {
   float thr;       // smallest float t with sqrtf(t) >= effR (cutoff test: s + 1e-4f < thr)
   float fv;        // force table value (ParticleSimulation.cu:116)
   float inv_eff;   // 1 / effR (continuous part only)
-  float pad;
+  float pad;       // row-wide: largest squared reach of the owner type (pruning radius)
 };
 
 // Device descriptor of one simulation.  Every wave covers 64 slots of one simulation, so
@@ -36,10 +36,13 @@ struct SimDesc {
   int reach;        // force stencil reach in cells (1 or 2)
   int tab;          // offset of this simulation's PairConst table
   int edge_on;      // edges computed inside the force pass
+  int ident;        // no parameters yet: one cell, slots in index order, nothing to sort
   float Lx, Ly, Lz;
   float ihx, ihy, ihz;      // n / L in FP32 (binning)
   float dt, friction, fm, one_minus_balance, max_exp;
   float rep, att, kexp2;    // repulsion, attraction, -k*log2(e)
+  float hx, hy, hz;          // cell sizes L / n in FP32 (stencil pruning)
+  int edge_inside;          // bit t: for owner type t the edge test implies the cutoff test
   float edge_d2;            // proximityDistance^2 in FP32 (ParticleSimulation.cu:660)
   int edge_maxc;            // min(maxConn, 32)
   int edge_rx, edge_ry, edge_rz;  // non-periodic edge stencil reach (standalone edge pass)
@@ -217,7 +220,8 @@ struct LineArgs {
 };
 
 // host-side launchers (cf_kernels.hip)
-hipError_t launch_force(const ForceArgs& a, bool edges, int block, hipStream_t st);
+hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
+                        int lanes, hipStream_t st);
 hipError_t launch_edges(const EdgeArgs& a, hipStream_t st);
 hipError_t launch_edge_write(const SortArgs& a, hipStream_t st);
 hipError_t launch_scan(const ScanPlan& p, hipStream_t st);

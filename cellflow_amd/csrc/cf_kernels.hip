@@ -295,6 +295,664 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs a) {
 }
 
 // ------------------------------------------------------------------------------------
+// k_force_lds: the same pass with the stencil staged in LDS.  One wave per workgroup.
+//
+// The wave's 64 owners are consecutive slots, so their cells are consecutive in (z, y, x)
+// order and, for each stencil row offset (dy, dz), their runs of neighbour slots fall in
+// consecutive neighbour rows.  Per row offset the wave reduces its lanes' runs to one
+// slot interval [A, B) (one interval per y/z wrap pattern present), copies the intervals
+// into a 1024-candidate LDS tile with direct global->LDS loads (all in flight at once,
+// one wait), and each lane then walks only its own runs inside the tile.  Intervals
+// longer than the tile are split over several batches.
+
+constexpr int kLdsCand = 1024;   // float4 candidates per LDS batch (16 KB)
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, kWave));
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, kWave));
+  return v;
+}
+
+// Neighbour row of stencil row index k for a lane in cell (cy, cz): absolute rows when the
+// axis has fewer than 2r+1 cells (minimum image per pair), else offset rows with the
+// periodic image shift.  wrap = 3*ywrap + zwrap (0 none, 1 below, 2 above).
+__device__ __forceinline__ void stencil_row(const SimDesc& sd, int k, int nyo, int cy, int cz,
+                                            int& rowc, float& sy, float& sz, int& wrap) {
+  const int r = sd.reach;
+  const int iy = k % nyo, iz = k / nyo;
+  int y, z, yw = 0, zw = 0;
+  sy = 0.0f;
+  sz = 0.0f;
+  if (sd.ny < 2 * r + 1) {
+    y = iy;
+  } else {
+    y = cy + iy - r;
+    if (y < 0) { y += sd.ny; sy = -sd.Ly; yw = 1; }
+    else if (y >= sd.ny) { y -= sd.ny; sy = sd.Ly; yw = 2; }
+  }
+  if (sd.nz < 2 * r + 1) {
+    z = iz;
+  } else {
+    z = cz + iz - r;
+    if (z < 0) { z += sd.nz; sz = -sd.Lz; zw = 1; }
+    else if (z >= sd.nz) { z -= sd.nz; sz = sd.Lz; zw = 2; }
+  }
+  rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+  wrap = 3 * yw + zw;
+}
+
+// x sub-runs of a lane: [xa0, xb0] unshifted and [xa1, xb1] with shift sx1 (empty if
+// xa1 > xb1).
+__device__ __forceinline__ void stencil_x(const SimDesc& sd, int cx, int& xa0, int& xb0, int& xa1,
+                                          int& xb1, float& sx1) {
+  const int r = sd.reach;
+  sx1 = 0.0f;
+  xa1 = 1;
+  xb1 = 0;
+  if (sd.nx < 2 * r + 1) {
+    xa0 = 0;
+    xb0 = sd.nx - 1;
+    return;
+  }
+  const int lo = cx - r, hi = cx + r;
+  if (lo < 0) {
+    xa0 = 0; xb0 = hi; xa1 = lo + sd.nx; xb1 = sd.nx - 1; sx1 = -sd.Lx;
+  } else if (hi >= sd.nx) {
+    xa0 = lo; xb0 = sd.nx - 1; xa1 = 0; xb1 = hi - sd.nx; sx1 = sd.Lx;
+  } else {
+    xa0 = lo; xb0 = hi;
+  }
+}
+
+struct PairAcc {
+  float fx, fy, fz;
+  int cnt;
+  EdgeAcc ea;
+};
+
+template <bool EDGES>
+__device__ __forceinline__ void visit_tile(PairAcc& acc, const ForceArgs& a, const SimDesc& sd,
+                                           const float4* __restrict__ tile, const PairConst* row,
+                                           const float4& pi, uint32_t ti, uint32_t idi, int p,
+                                           uint32_t lo, uint32_t hi, uint32_t base, int off,
+                                           float sx, float sy, float sz, bool gen, bool do_edges) {
+  const bool nowrap = (sx == 0.0f) && (sy == 0.0f) && (sz == 0.0f);
+  for (uint32_t q = lo; q < hi; ++q) {
+    const float4 pj = tile[off + (int)(q - base)];
+    float dx, dy, dz;
+    bool raw;
+    if (!gen) {
+      dx = (pj.x - pi.x) + sx;
+      dy = (pj.y - pi.y) + sy;
+      dz = (pj.z - pi.z) + sz;
+      raw = nowrap;
+    } else {
+      const float ux = pj.x - pi.x, uy = pj.y - pi.y, uz = pj.z - pi.z;
+      dx = min_image(ux, sd.Lx);
+      dy = min_image(uy, sd.Ly);
+      dz = min_image(uz, sd.Lz);
+      raw = (dx == ux) && (dy == uy) && (dz == uz);
+    }
+    const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const uint32_t tj = __float_as_uint(pj.w);
+    const float sp = s2 + 1e-4f;
+    const PairConst pc = row[tj];
+    if (sp < pc.thr && (int)q != p) {
+      acc.cnt++;
+      const float inv = __builtin_amdgcn_rsqf(sp);
+      const float dist = sp * inv;
+      const float rr = dist * pc.inv_eff;
+      const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
+      const float net = (sd.rep * ex - sd.att * rr) * pc.fv;
+      const float w = net * inv;
+      acc.fx += dx * w;
+      acc.fy += dy * w;
+      acc.fz += dz * w;
+    }
+    if (EDGES) {
+      if (do_edges && raw && tj == ti && s2 < sd.edge_d2 && (int)q != p) {
+        const uint32_t jid = a.id[q];
+        if (jid > idi) edge_push(acc.ea, s2, jid, a.edge_cap, a.ecand_d2, a.ecand_id, a.P, p);
+      }
+    }
+  }
+}
+
+template <bool EDGES, int R>
+__global__ void __launch_bounds__(64) k_force_lds(ForceArgs a) {
+  constexpr int kRows = (2 * R + 1) * (2 * R + 1);
+  __shared__ float4 tile[kLdsCand];
+  __shared__ uint4 lruns[kRows][kWave];
+  __shared__ PairConst tab[kMaxTypes * kMaxTypes];
+  const int lane = threadIdx.x;
+  const int gw = blockIdx.x;
+  if (gw >= a.nwaves) return;  // whole workgroup
+  const int s = __builtin_amdgcn_readfirstlane((int)a.wave_sim[gw]);
+  const SimDesc sd = a.sims[s];
+  for (int t = lane; t < sd.T * sd.T; t += kWave) tab[t] = a.pairs[sd.tab + t];
+  const int p = gw * kWave + lane;
+  const bool active = (p - sd.p0) < sd.n;
+  float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = pi;
+  uint32_t idi = 0;
+  if (active) {
+    pi = a.pos[p];
+    vi = a.vel[p];
+    idi = a.id[p];
+  }
+  const uint32_t ti = active ? __float_as_uint(pi.w) : 0u;
+  const int cx = bin_axis(pi.x, sd.ihx, sd.nx);
+  const int cy = bin_axis(pi.y, sd.ihy, sd.ny);
+  const int cz = bin_axis(pi.z, sd.ihz, sd.nz);
+  const int nyo = (sd.ny < 2 * R + 1) ? sd.ny : 2 * R + 1;
+  const int nzo = (sd.nz < 2 * R + 1) ? sd.nz : 2 * R + 1;
+  const int nrows = nyo * nzo;
+  int xa0, xb0, xa1, xb1;
+  float sx1;
+  stencil_x(sd, cx, xa0, xb0, xa1, xb1, sx1);
+  const bool has1 = xa1 <= xb1;
+
+  // phase A: this lane's slot runs for every stencil row (independent loads)
+#pragma unroll
+  for (int k = 0; k < kRows; ++k) {
+    if (k < nrows) {
+      int rowc, wrap;
+      float sy, sz;
+      stencil_row(sd, k, nyo, cy, cz, rowc, sy, sz, wrap);
+      uint4 rr = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);
+      if (active) {
+        rr.x = a.start[rowc + xa0];
+        rr.y = a.start[rowc + xb0 + 1];
+        if (has1) {
+          rr.z = a.start[rowc + xa1];
+          rr.w = a.start[rowc + xb1 + 1];
+        }
+      }
+      lruns[k][lane] = rr;
+    }
+  }
+  __syncthreads();
+
+  const PairConst* row = &tab[ti * sd.T];
+  const bool gen = sd.generic != 0;
+  PairAcc acc;
+  acc.fx = acc.fy = acc.fz = 0.0f;
+  acc.cnt = 0;
+  acc.ea.q = 0;
+  acc.ea.maxid = 0;
+  acc.ea.maxk = 0;
+  const bool do_edges = EDGES && a.edges && sd.edge_on;
+
+  // phase B: batches of staged intervals; lane j of the wave describes piece j
+  int used = 0, npieces = 0;
+  int my_k = 0, my_g = 0, my_off = 0;
+  uint32_t my_a = 0, my_b = 0;
+  auto flush = [&]() {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int j = 0; j < npieces; ++j) {
+      const int k = __builtin_amdgcn_readlane(my_k, j);
+      const int g = __builtin_amdgcn_readlane(my_g, j);
+      const uint32_t pa = (uint32_t)__builtin_amdgcn_readlane((int)my_a, j);
+      const uint32_t pb = (uint32_t)__builtin_amdgcn_readlane((int)my_b, j);
+      const int off = __builtin_amdgcn_readlane(my_off, j);
+      int rowc, wrap;
+      float sy, sz;
+      stencil_row(sd, k, nyo, cy, cz, rowc, sy, sz, wrap);
+      if (active && wrap == g) {
+        const uint4 rr = lruns[k][lane];
+        visit_tile<EDGES>(acc, a, sd, tile, row, pi, ti, idi, p, max(rr.x, pa), min(rr.y, pb), pa,
+                          off, 0.0f, sy, sz, gen, do_edges);
+        visit_tile<EDGES>(acc, a, sd, tile, row, pi, ti, idi, p, max(rr.z, pa), min(rr.w, pb), pa,
+                          off, sx1, sy, sz, gen, do_edges);
+      }
+    }
+    __syncthreads();
+    used = 0;
+    npieces = 0;
+  };
+
+  for (int k = 0; k < nrows; ++k) {
+    int rowc, gk;
+    float sy, sz;
+    stencil_row(sd, k, nyo, cy, cz, rowc, sy, sz, gk);
+    const uint4 rr = lruns[k][lane];
+    unsigned long long pend = __ballot(active);
+    while (pend) {
+      const int lead = __ffsll((long long)pend) - 1;
+      const int g = __builtin_amdgcn_readlane(gk, lead);
+      const bool mine = active && gk == g;
+      pend &= ~__ballot(mine);
+      uint32_t A = wave_min_u32(mine ? min(rr.x, rr.z) : 0xFFFFFFFFu);
+      const uint32_t B = wave_max_u32(mine ? max(rr.y, rr.w) : 0u);
+      A = __builtin_amdgcn_readfirstlane(A);
+      while (A < B) {
+        if (used == kLdsCand || npieces == kWave) flush();
+        const int take = min((int)(B - A), kLdsCand - used);
+        for (int j0 = 0; j0 < take; j0 += kWave) {
+          if (j0 + lane < take)
+            __builtin_amdgcn_global_load_lds((const void*)(a.pos + A + j0 + lane),
+                                             (__attribute__((address_space(3))) void*)&tile[used + j0],
+                                             16, 0, 0);
+        }
+        if (lane == npieces) {
+          my_k = k;
+          my_g = g;
+          my_a = A;
+          my_b = A + (uint32_t)take;
+          my_off = used;
+        }
+        npieces++;
+        used += take;
+        A += (uint32_t)take;
+      }
+    }
+  }
+  if (npieces) flush();
+
+  if (!active) return;
+  // ParticleSimulation.cu:136-161
+  const int prev = __float_as_int(vi.w);
+  const float avg = (float)(acc.cnt + prev) * 0.5f;
+  const float dens = fminf(avg / sd.max_exp, 1.0f);
+  const float fac = 1.0f - sd.one_minus_balance * dens;
+  const float m = sd.fm * fac;
+  const float fx = acc.fx * m, fy = acc.fy * m, fz = acc.fz * m;
+  const float vx = vi.x * sd.friction + fx * sd.dt;
+  const float vy = vi.y * sd.friction + fy * sd.dt;
+  const float vz = vi.z * sd.friction + fz * sd.dt;
+  const float x = wrap_pos(pi.x + vx * sd.dt, sd.Lx);
+  const float y = wrap_pos(pi.y + vy * sd.dt, sd.Ly);
+  const float z = wrap_pos(pi.z + vz * sd.dt, sd.Lz);
+  const float fin = fabsf(x) + fabsf(y) + fabsf(z) + fabsf(vx) + fabsf(vy) + fabsf(vz);
+  if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
+  a.tpos[p] = make_float4(x, y, z, pi.w);
+  a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(acc.cnt));
+  a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  const uint32_t c = (uint32_t)(sd.c0 + bin_axis(x, sd.ihx, sd.nx) +
+                                sd.nx * (bin_axis(y, sd.ihy, sd.ny) +
+                                         sd.ny * bin_axis(z, sd.ihz, sd.nz)));
+  a.key[p] = c;
+  a.arrival[p] = atomicAdd(&a.hist[c], 1u);
+  if (EDGES) {
+    if (do_edges) {
+      const int e = edge_finish(acc.ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id,
+                                a.estage, a.P, p);
+      a.ecount[idi] = (uint32_t)e;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// k_force_grp: G lanes per particle.
+//
+// Setup: the particle's lanes build its run table in LDS: one or two slot runs per
+// stencil row (the unshifted x-range and the periodic-wrap x-range, with the image shift
+// packed into the run's length word).  Rows and x-cells that cannot hold a partner are
+// left out: a cell is kept only if the particle's distance to it, less a margin of 1e-3
+// cells, is below the largest cutoff of the particle's type (with a relative margin of
+// 1e-3), so no pair that the FP32 predicate accepts is ever dropped.
+// Main loop: the G lanes split the flattened run list cyclically (lane g takes candidates
+// g, g+G, ...), so each lane walks about 1/G of the candidates and neighbouring lanes
+// read neighbouring slots.  Partial forces and counts are reduced across the group; the
+// particle itself is visited once (its own cell is always kept) and removed from the
+// count afterwards instead of being tested per candidate.
+// Edges: candidates of all G lanes are appended to a per-particle LDS list; the group
+// selects the 2*maxc smallest indices and ranks them by (distance, index).  A particle
+// with more than kEdgeList candidates falls back to the exact streaming selection.
+
+constexpr int kEdgeList = 64;
+constexpr uint32_t kLenMask = (1u << 26) - 1u;
+
+template <int G>
+__device__ __forceinline__ float group_sum_f(float v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ int group_sum_i(int v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// gap (in cells) from fractional position f in [0, 1) to the cell at offset d, minus the
+// safety margin
+__device__ __forceinline__ float cell_gap(int d, float f) {
+  const float g = (d > 0) ? ((float)d - f) : ((d < 0) ? ((float)(-d - 1) + f) : 0.0f);
+  return fmaxf(g - 1e-3f, 0.0f);
+}
+
+__device__ __forceinline__ float shift_of(uint32_t c, float L) {
+  return (c == 1u) ? -L : ((c == 2u) ? L : 0.0f);
+}
+
+template <bool EDGES>
+struct GrpAcc {
+  float fx, fy, fz;
+  int cnt;
+};
+
+template <bool EDGES, int R, int G>
+__global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
+  constexpr int kRows = (2 * R + 1) * (2 * R + 1);
+  constexpr int kRuns = 2 * kRows;
+  constexpr int kOwn = kWave / G;  // particles per wave
+  __shared__ PairConst tab[4][kMaxTypes * kMaxTypes];
+  __shared__ uint2 runs[4][kOwn][kRuns];  // (start slot, length | shift code << 26)
+  __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
+  __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
+  __shared__ uint32_t el_n[4][kOwn];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gw = blockIdx.x * (blockDim.x >> 6) + wv;   // wave index in units of kOwn slots
+  const long base = (long)gw * kOwn;
+  const bool wave_ok = base < (long)a.nwaves * kWave;
+  const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
+  const SimDesc sd = a.sims[s];
+  if (wave_ok)
+    for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = a.pairs[sd.tab + t];
+  const int o = lane / G, g = lane % G;
+  const int p = (int)base + o;
+  const bool active = wave_ok && (p - sd.p0) < sd.n;
+  float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = pi;
+  uint32_t idi = 0;
+  if (active) {
+    pi = a.pos[p];
+    vi = a.vel[p];
+    idi = a.id[p];
+  }
+  const uint32_t ti = active ? __float_as_uint(pi.w) : 0u;
+  const int cx = bin_axis(pi.x, sd.ihx, sd.nx);
+  const int cy = bin_axis(pi.y, sd.ihy, sd.ny);
+  const int cz = bin_axis(pi.z, sd.ihz, sd.nz);
+  const bool gen = sd.generic != 0;
+  const int nyo = gen && sd.ny < 2 * R + 1 ? sd.ny : 2 * R + 1;
+  const int nzo = gen && sd.nz < 2 * R + 1 ? sd.nz : 2 * R + 1;
+  const int nrows = nyo * nzo;
+  if (EDGES && g == 0) el_n[wv][o] = 0u;
+  __syncthreads();  // pair table ready
+
+  const PairConst* row = &tab[wv][ti * sd.T];
+  // ---- run table ------------------------------------------------------------------
+  {
+    const float cut2 = row[0].pad;  // largest squared reach of this type, with margins
+    const float fxr = fminf(fmaxf(pi.x * sd.ihx - (float)cx, 0.0f), 1.0f);
+    const float fyr = fminf(fmaxf(pi.y * sd.ihy - (float)cy, 0.0f), 1.0f);
+    const float fzr = fminf(fmaxf(pi.z * sd.ihz - (float)cz, 0.0f), 1.0f);
+    for (int r = g; r < kRows; r += G) {
+      uint2 e0 = make_uint2(0u, 0u), e1 = make_uint2(0u, 0u);
+      if (active && r < nrows) {
+        const int iy = r % nyo, iz = r / nyo;
+        int y, z, dy = 0, dz = 0;
+        uint32_t cyc = 0u, czc = 0u;
+        if (gen && sd.ny < 2 * R + 1) {
+          y = iy;
+        } else {
+          dy = iy - R;
+          y = cy + dy;
+          if (y < 0) { y += sd.ny; cyc = 1u; }
+          else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
+        }
+        if (gen && sd.nz < 2 * R + 1) {
+          z = iz;
+        } else {
+          dz = iz - R;
+          z = cz + dz;
+          if (z < 0) { z += sd.nz; czc = 1u; }
+          else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
+        }
+        float gyz = 0.0f;
+        if (!gen) {
+          const float gy = cell_gap(dy, fyr) * sd.hy, gz = cell_gap(dz, fzr) * sd.hz;
+          gyz = fmaf(gy, gy, gz * gz);
+        }
+        if (gen || gyz < cut2) {
+          const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+          int xlo, xhi;
+          if (gen && sd.nx < 2 * R + 1) {
+            xlo = 0;
+            xhi = sd.nx - 1;
+          } else {
+            xlo = cx - R;
+            xhi = cx + R;
+            if (!gen) {
+              while (xlo < cx) {
+                const float gx = cell_gap(xlo - cx, fxr) * sd.hx;
+                if (fmaf(gx, gx, gyz) < cut2) break;
+                ++xlo;
+              }
+              while (xhi > cx) {
+                const float gx = cell_gap(xhi - cx, fxr) * sd.hx;
+                if (fmaf(gx, gx, gyz) < cut2) break;
+                --xhi;
+              }
+            }
+          }
+          const uint32_t yzc = (cyc << 2) | (czc << 4);
+          int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
+          uint32_t xc1 = 0u;
+          if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
+          else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
+          const uint32_t s0 = a.start[rowc + a0];
+          e0 = make_uint2(s0, (a.start[rowc + b0 + 1] - s0) | (yzc << 26));
+          if (a1 <= b1) {
+            const uint32_t s1 = a.start[rowc + a1];
+            e1 = make_uint2(s1, (a.start[rowc + b1 + 1] - s1) | ((yzc | xc1) << 26));
+          }
+        }
+      }
+      runs[wv][o][2 * r] = e0;
+      runs[wv][o][2 * r + 1] = e1;
+    }
+  }
+  __syncthreads();
+
+  // ---- candidates -----------------------------------------------------------------
+  const bool do_edges = EDGES && a.edges && sd.edge_on && active;
+  const bool ein = do_edges && ((sd.edge_inside >> ti) & 1);   // edge test implies in-range
+  const bool eout = do_edges && !ein;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  int cnt = 0;
+  uint32_t cum = 0;
+  const int nruns = active ? 2 * nrows : 0;
+  for (int run = 0; run < nruns; ++run) {
+    const uint2 e = runs[wv][o][run];
+    const uint32_t len = e.y & kLenMask;
+    if (len == 0u) continue;
+    const uint32_t code = e.y >> 26;
+    const uint32_t q0 = e.x;
+    // this lane's first index inside the run: flattened index == g (mod G)
+    uint32_t t = (uint32_t)((g - (int)(cum & (G - 1))) & (G - 1));
+    cum += len;
+    if (code == 0u && !gen) {
+      // unshifted run: raw (non-periodic) differences
+      for (; t < len; t += G) {
+        const uint32_t q = q0 + t;
+        const float4 pj = a.pos[q];
+        const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const uint32_t tj = __float_as_uint(pj.w);
+        const PairConst pc = row[tj];
+        const float sp = s2 + 1e-4f;
+        const bool in = sp < pc.thr;
+        if (in) {
+          cnt++;
+          const float inv = __builtin_amdgcn_rsqf(sp);
+          const float rr = (sp * inv) * pc.inv_eff;
+          const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
+          const float w = (sd.rep * ex - sd.att * rr) * pc.fv * inv;
+          fx += dx * w;
+          fy += dy * w;
+          fz += dz * w;
+        }
+        if (EDGES) {
+          if ((in ? ein : eout) && tj == ti && s2 < sd.edge_d2) {
+            const uint32_t jid = a.id[q];
+            if (jid > idi) {
+              const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
+              if (kk < (uint32_t)kEdgeList) {
+                el_d[wv][o][kk] = make_float2(s2, 0.0f);
+                el_i[wv][o][kk] = jid;
+              }
+            }
+          }
+        }
+      }
+    } else {
+      // image-shifted run, or per-pair minimum image on a grid too small for shifts
+      const float sx = shift_of(code & 3u, sd.Lx), sy = shift_of((code >> 2) & 3u, sd.Ly),
+                  sz = shift_of((code >> 4) & 3u, sd.Lz);
+      for (; t < len; t += G) {
+        const uint32_t q = q0 + t;
+        const float4 pj = a.pos[q];
+        float dx, dy, dz;
+        bool raw;
+        if (!gen) {
+          dx = (pj.x - pi.x) + sx;
+          dy = (pj.y - pi.y) + sy;
+          dz = (pj.z - pi.z) + sz;
+          raw = false;
+        } else {
+          const float ux = pj.x - pi.x, uy = pj.y - pi.y, uz = pj.z - pi.z;
+          dx = min_image(ux, sd.Lx);
+          dy = min_image(uy, sd.Ly);
+          dz = min_image(uz, sd.Lz);
+          raw = (dx == ux) && (dy == uy) && (dz == uz);
+        }
+        const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const uint32_t tj = __float_as_uint(pj.w);
+        const PairConst pc = row[tj];
+        const float sp = s2 + 1e-4f;
+        if (sp < pc.thr) {
+          cnt++;
+          const float inv = __builtin_amdgcn_rsqf(sp);
+          const float rr = (sp * inv) * pc.inv_eff;
+          const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
+          const float w = (sd.rep * ex - sd.att * rr) * pc.fv * inv;
+          fx += dx * w;
+          fy += dy * w;
+          fz += dz * w;
+        }
+        if (EDGES) {
+          if (do_edges && raw && tj == ti && s2 < sd.edge_d2) {
+            const uint32_t jid = a.id[q];
+            if (jid > idi) {
+              const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
+              if (kk < (uint32_t)kEdgeList) {
+                el_d[wv][o][kk] = make_float2(s2, 0.0f);
+                el_i[wv][o][kk] = jid;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  fx = group_sum_f<G>(fx);
+  fy = group_sum_f<G>(fy);
+  fz = group_sum_f<G>(fz);
+  cnt = group_sum_i<G>(cnt);
+  if (active && 1e-4f < row[ti].thr) cnt -= 1;  // the particle itself
+
+  if (EDGES) {
+    // barriers stay outside lane-dependent branches
+    __syncthreads();
+    const uint32_t n = do_edges ? el_n[wv][o] : 0u;
+    const bool fits = n <= (uint32_t)kEdgeList;
+    const int cap = a.edge_cap, maxc = sd.edge_maxc;
+    // membership in S: fewer than cap smaller indices among the candidates
+    if (fits)
+      for (uint32_t k = g; k < n; k += G) {
+        const uint32_t ik = el_i[wv][o][k];
+        uint32_t below = 0;
+        for (uint32_t k2 = 0; k2 < n; ++k2) below += (el_i[wv][o][k2] < ik) ? 1u : 0u;
+        el_d[wv][o][k].y = (below < (uint32_t)cap) ? 1.0f : 0.0f;
+      }
+    __syncthreads();
+    // rank of each member of S by (distance, index); keep the first maxc
+    if (fits)
+      for (uint32_t k = g; k < n; k += G) {
+        const float2 dk = el_d[wv][o][k];
+        if (dk.y == 0.0f) continue;
+        const uint32_t ik = el_i[wv][o][k];
+        uint32_t rank = 0;
+        for (uint32_t k2 = 0; k2 < n; ++k2) {
+          const float2 d2 = el_d[wv][o][k2];
+          const uint32_t i2 = el_i[wv][o][k2];
+          rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
+        }
+        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + p] = ik;
+      }
+    if (do_edges && g == 0) {
+      if (fits) {
+        a.ecount[idi] = (uint32_t)min(min((int)n, cap), maxc);
+      } else {
+        // more candidates than the LDS list holds: exact streaming selection by one lane
+        EdgeAcc ea;
+        ea.q = 0;
+        ea.maxid = 0;
+        ea.maxk = 0;
+        for (int run = 0; run < 2 * nrows; ++run) {
+          const uint2 e = runs[wv][o][run];
+          const uint32_t len = e.y & kLenMask, code = e.y >> 26;
+          if (!gen && code != 0u) continue;  // image-shifted runs hold no edge partner
+          for (uint32_t t = 0; t < len; ++t) {
+            const uint32_t q = e.x + t;
+            const float4 pj = a.pos[q];
+            if (__float_as_uint(pj.w) != ti) continue;
+            const float ux = pj.x - pi.x, uy = pj.y - pi.y, uz = pj.z - pi.z;
+            if (gen && !(min_image(ux, sd.Lx) == ux && min_image(uy, sd.Ly) == uy &&
+                         min_image(uz, sd.Lz) == uz))
+              continue;
+            const float s2 = fmaf(uz, uz, fmaf(uy, uy, ux * ux));
+            if (!(s2 < sd.edge_d2)) continue;
+            const uint32_t jid = a.id[q];
+            if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
+          }
+        }
+        a.ecount[idi] =
+            (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p);
+      }
+    }
+  }
+
+  if (!active || g != 0) return;
+  // ParticleSimulation.cu:136-161
+  const int prev = __float_as_int(vi.w);
+  const float avg = (float)(cnt + prev) * 0.5f;
+  const float dens = fminf(avg / sd.max_exp, 1.0f);
+  const float fac = 1.0f - sd.one_minus_balance * dens;
+  const float m = sd.fm * fac;
+  fx *= m;
+  fy *= m;
+  fz *= m;
+  const float vx = vi.x * sd.friction + fx * sd.dt;
+  const float vy = vi.y * sd.friction + fy * sd.dt;
+  const float vz = vi.z * sd.friction + fz * sd.dt;
+  const float x = wrap_pos(pi.x + vx * sd.dt, sd.Lx);
+  const float y = wrap_pos(pi.y + vy * sd.dt, sd.Ly);
+  const float z = wrap_pos(pi.z + vz * sd.dt, sd.Lz);
+  const float fin = fabsf(x) + fabsf(y) + fabsf(z) + fabsf(vx) + fabsf(vy) + fabsf(vz);
+  if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
+  a.tpos[p] = make_float4(x, y, z, pi.w);
+  a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
+  a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  const uint32_t c = (uint32_t)(sd.c0 + bin_axis(x, sd.ihx, sd.nx) +
+                                sd.nx * (bin_axis(y, sd.ihy, sd.ny) +
+                                         sd.ny * bin_axis(z, sd.ihz, sd.nz)));
+  a.key[p] = c;
+  a.arrival[p] = atomicAdd(&a.hist[c], 1u);
+}
+
+// ------------------------------------------------------------------------------------
 // k_edges: standalone proximity-graph candidates on the current sorted state, with a
 // non-periodic stencil of per-axis reach (the edge distance may exceed the cell size).
 
@@ -399,34 +1057,50 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(ScanPlan plan) {
   }
   const uint32_t texcl = wpre + incl - sum;
 
-  if (tid == 0) {
+  if (wv == 0) {
     uint32_t prefix = 0;
     if (tile == 0) {
-      __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 2u, total), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0)
+        __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 2u, total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 1u, total), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      // Look back over this segment's earlier tiles.  Every earlier tile holds an earlier
-      // ticket, so its workgroup is already running and publishes without waiting.
-      int j = (int)g - 1;
+      if (lane == 0)
+        __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 1u, total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      // Wave-parallel look-back: lane l reads the flag of tile g-1-l.  Every earlier tile
+      // holds an earlier ticket, so its workgroup is already running and publishes its
+      // aggregate without waiting; tiles before the segment start count as prefix 0.
+      const int first = (int)g - tile;
+      int hi = (int)g - 1;
       while (true) {
-        const unsigned long long w =
-            __hip_atomic_load(&plan.flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t we = (uint32_t)(w >> 34);
-        const uint32_t st = (uint32_t)(w >> 32) & 3u;
-        if (we != epoch || st == 0u) {
+        const int j = hi - lane;
+        uint32_t st = 2u, val = 0u;
+        if (j >= first) {
+          const unsigned long long w =
+              __hip_atomic_load(&plan.flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          st = ((uint32_t)(w >> 34) == epoch) ? ((uint32_t)(w >> 32) & 3u) : 0u;
+          val = (uint32_t)w;
+        }
+        const unsigned long long incl = __ballot(st == 2u);
+        const int stop = incl ? (__ffsll((long long)incl) - 1) : (kWave - 1);
+        const unsigned long long need =
+            (stop == kWave - 1) ? ~0ull : ((1ull << (stop + 1)) - 1ull);
+        if (__ballot(st == 0u) & need) {
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        prefix += (uint32_t)w;
-        if (st == 2u) break;
-        --j;
+        uint32_t v = (lane <= stop) ? val : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, kWave);
+        prefix += v;
+        if (incl) break;
+        hi -= kWave;
       }
-      __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 2u, prefix + total), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0)
+        __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 2u, prefix + total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
-    s_prefix = prefix;
+    if (lane == 0) s_prefix = prefix;
   }
   __syncthreads();
   uint32_t run = s_prefix + texcl;
@@ -458,7 +1132,8 @@ __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
   if (p - sd.p0 >= sd.n) return;
   const uint32_t c = a.key[p];
   const uint32_t me = a.src_id[p];
-  a.tmp_id[a.start[c] + a.arrival[p]] = me;
+  const uint32_t arr = a.arrival[p];
+  a.tmp_id[a.start[c] + arr] = me;
   a.hist[c] = 0u;  // every writer stores 0; pad cells are never keys and keep their size
   if (a.edges) {
     const uint32_t off = a.eoff[me];
@@ -476,12 +1151,24 @@ __global__ void __launch_bounds__(256) k_rank_scatter(SortArgs a) {
   if (p - sd.p0 >= sd.n) return;
   const uint32_t c = a.key[p];
   const uint32_t me = a.src_id[p];
+  const float4 sp = a.src_pos[p];
+  const float4 sv = a.src_vel[p];
   const uint32_t s0 = a.start[c], s1 = a.start[c + 1];
   uint32_t rank = 0;
-  for (uint32_t q = s0; q < s1; ++q) rank += (a.tmp_id[q] < me) ? 1u : 0u;
+  if (sd.ident) {
+    rank = a.arrival[p];  // source already in index order
+  } else {
+    uint32_t q = s0;
+    for (; q + 4 <= s1; q += 4) {  // four independent loads per trip
+      const uint32_t i0 = a.tmp_id[q], i1 = a.tmp_id[q + 1], i2 = a.tmp_id[q + 2],
+                     i3 = a.tmp_id[q + 3];
+      rank += (i0 < me) + (i1 < me) + (i2 < me) + (i3 < me);
+    }
+    for (; q < s1; ++q) rank += (a.tmp_id[q] < me) ? 1u : 0u;
+  }
   const uint32_t dst = s0 + rank;
-  a.dst_pos[dst] = a.src_pos[p];
-  a.dst_vel[dst] = a.src_vel[p];
+  a.dst_pos[dst] = sp;
+  a.dst_vel[dst] = sv;
   a.dst_id[dst] = me;
 }
 
@@ -510,6 +1197,12 @@ __global__ void __launch_bounds__(256) k_bin(BinArgs a) {
   const SimDesc sd = a.sims[a.wave_sim[gw]];
   const int p = gw * kWave + (threadIdx.x & (kWave - 1));
   if (p - sd.p0 >= sd.n) return;
+  if (sd.ident) {  // one cell in index order: no atomics, the count is known
+    a.key[p] = (uint32_t)sd.c0;
+    a.arrival[p] = (uint32_t)(p - sd.p0);
+    if (p == sd.p0) a.hist[sd.c0] = (uint32_t)sd.n;
+    return;
+  }
   const float4 pi = a.pos[p];
   const uint32_t c = (uint32_t)(sd.c0 + bin_axis(pi.x, sd.ihx, sd.nx) +
                                 sd.nx * (bin_axis(pi.y, sd.ihy, sd.ny) +
@@ -662,7 +1355,43 @@ __global__ void k_line_vertices(LineArgs a) {
 
 static inline int grid_for(int nwaves, int block) { return (nwaves * kWave + block - 1) / block; }
 
-hipError_t launch_force(const ForceArgs& a, bool edges, int block, hipStream_t st) {
+template <int R, int G>
+static void launch_grp(const ForceArgs& a, bool edges, hipStream_t st) {
+  const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
+  const dim3 g((unsigned)((waves + 3) / 4)), b(256);
+  if (edges) hipLaunchKernelGGL((k_force_grp<true, R, G>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((k_force_grp<false, R, G>), g, b, 0, st, a);
+}
+
+hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
+                        int lanes, hipStream_t st) {
+  if (variant == 0) {
+    if (reach == 1) {
+      if (lanes == 4) launch_grp<1, 4>(a, edges, st);
+      else if (lanes == 8) launch_grp<1, 8>(a, edges, st);
+      else if (lanes == 16) launch_grp<1, 16>(a, edges, st);
+      else if (lanes == 32) launch_grp<1, 32>(a, edges, st);
+      else return hipErrorInvalidValue;
+    } else {
+      if (lanes == 4) launch_grp<2, 4>(a, edges, st);
+      else if (lanes == 8) launch_grp<2, 8>(a, edges, st);
+      else if (lanes == 16) launch_grp<2, 16>(a, edges, st);
+      else if (lanes == 32) launch_grp<2, 32>(a, edges, st);
+      else return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+  if (variant == 2) {
+    const dim3 g(a.nwaves), b(kWave);
+    if (reach == 1) {
+      if (edges) hipLaunchKernelGGL((k_force_lds<true, 1>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_force_lds<false, 1>), g, b, 0, st, a);
+    } else {
+      if (edges) hipLaunchKernelGGL((k_force_lds<true, 2>), g, b, 0, st, a);
+      else hipLaunchKernelGGL((k_force_lds<false, 2>), g, b, 0, st, a);
+    }
+    return hipGetLastError();
+  }
   if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
   const int g = grid_for(a.nwaves, block);
   if (edges)

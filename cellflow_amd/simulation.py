@@ -273,14 +273,17 @@ class ParticleSimulation:
         keys = names.value.decode().split(",")
         return {k: ms[i] for i, k in enumerate(keys[: nk.value])}
 
-    def setTuning(self, cell_divisor=1, block_size=256, steps_per_graph=32):
-        t = _abi.cf_tuning(cell_divisor, block_size, steps_per_graph)
+    def setTuning(self, cell_divisor=1, force_variant=0, lanes_per_particle=16, block_size=256,
+                  steps_per_graph=32):
+        t = _abi.cf_tuning(cell_divisor, force_variant, lanes_per_particle, block_size,
+                           steps_per_graph)
         check(lib().cf_set_tuning(self._h, C.byref(t)))
 
     def getTuning(self):
         t = _abi.cf_tuning()
         check(lib().cf_get_tuning(self._h, C.byref(t)))
-        return dict(cell_divisor=t.cell_divisor, block_size=t.block_size,
+        return dict(cell_divisor=t.cell_divisor, force_variant=t.force_variant,
+                    lanes_per_particle=t.lanes_per_particle, block_size=t.block_size,
                     steps_per_graph=t.steps_per_graph)
 
 
@@ -323,8 +326,10 @@ class Ensemble:
         check(lib().cf_ensemble_get_neighbor_counts(self._h, s, _ip(c), self.n[s]))
         return c
 
-    def setTuning(self, cell_divisor=1, block_size=256, steps_per_graph=32):
-        t = _abi.cf_tuning(cell_divisor, block_size, steps_per_graph)
+    def setTuning(self, cell_divisor=1, force_variant=0, lanes_per_particle=16, block_size=256,
+                  steps_per_graph=32):
+        t = _abi.cf_tuning(cell_divisor, force_variant, lanes_per_particle, block_size,
+                           steps_per_graph)
         check(lib().cf_ensemble_set_tuning(self._h, C.byref(t)))
 
     def close(self):

@@ -82,7 +82,10 @@ typedef struct cf_edge_opts {
 /* Launch/grid choices made by the autotuner (no reference counterpart). */
 typedef struct cf_tuning {
   int cell_divisor;             /* 1: cell >= max cutoff, 27-cell stencil; 2: half cells, 125 */
-  int block_size;               /* threads per workgroup of the force kernel: 64..1024 */
+  int force_variant;            /* 0: G lanes per particle; 1: one lane per particle;
+                                   2: LDS-staged stencil, one wave per workgroup */
+  int lanes_per_particle;       /* G of variant 0: 4, 8, 16 or 32 */
+  int block_size;               /* threads per workgroup of variant 1: 64, 128, 256 */
   int steps_per_graph;          /* steps captured per HIP graph replay (>= 1) */
 } cf_tuning;
 

@@ -124,6 +124,19 @@ def test_edges_truncation_matches_oracle(orc, maxc):
         assert np.array_equal(edges, ref), f"dist={dist} maxc={maxc}"
 
 
+@pytest.mark.parametrize("lanes", [4, 16])
+def test_fused_edges_dense_blobs(orc, lanes):
+    """Fused edge candidates on dense blobs: many owners exceed the per-particle LDS list
+    and take the streaming fallback; the graph must still equal the oracle's."""
+    st, phys, params = clustered_state(n=5000, blobs=4, sd=0.006)
+    sim = make_sim(st, phys)
+    sim.setTuning(force_variant=0, lanes_per_particle=lanes)
+    for k in range(2):
+        sim.simulate(params, 1, edges=(0.01, 16))
+        got = sim.getParticleData()
+        assert np.array_equal(sim.getEdges(), orc.edges(got["pos"], got["ptype"], 0.01, 16, mode=0))
+
+
 def test_fused_edges_every_step(orc):
     """With edges on, each step's graph is of the post-step state (the last one is kept)."""
     cfg = small_cfg(n=4000, side=0.5)
@@ -219,3 +232,31 @@ def test_edge_capacity_error():
     buf = np.zeros((1, 2), np.int32)
     rc = _abi.lib().cf_get_edges(sim._h, buf.ctypes.data_as(C.POINTER(C.c_int32)), 1, C.byref(cnt))
     assert rc == _abi.CF_ERR_CAPACITY and cnt.value > 1
+
+
+VARIANTS = [(0, 16, 1), (0, 4, 1), (0, 8, 2), (0, 32, 1), (0, 32, 2), (1, 16, 1), (1, 16, 2),
+            (2, 16, 1), (2, 16, 2)]
+
+
+@pytest.mark.parametrize("variant,lanes,divisor", VARIANTS)
+@pytest.mark.parametrize("name", ["uniform_3k", "small_grid_generic", "dense_2k"])
+def test_variants_match_oracle(orc, variant, lanes, divisor, name):
+    """Every force-kernel variant, lane grouping and stencil reach gives the oracle's counts,
+    forces and edges."""
+    cfg = small_cfg(**CASES[name])
+    st, phys, params = state_of(cfg)
+    rng = np.random.default_rng(11)
+    st["prev"] = rng.integers(0, 600, len(st["type"])).astype(np.int32)
+    sim = make_sim(st, phys)
+    sim.setTuning(cell_divisor=divisor, force_variant=variant, lanes_per_particle=lanes,
+                  block_size=128)
+    sim.simulate(params, 1, edges=(0.02, 6))
+    ref = orc.step(st["pos"], st["vel"], st["type"], st["prev"], phys, st["table"], st["rbt"], mode=0)
+    assert np.array_equal(sim.getNeighborCounts(), ref["count"])
+    got = sim.getParticleData()
+    check_step(got, ref, phys, st["vel"])
+    assert np.array_equal(sim.getEdges(), orc.edges(got["pos"], got["ptype"], 0.02, 6, mode=0))
+    # a second step exercises the fused edge candidates of the pre-step state
+    sim.simulate(params, 1, edges=(0.02, 6))
+    got2 = sim.getParticleData()
+    assert np.array_equal(sim.getEdges(), orc.edges(got2["pos"], got2["ptype"], 0.02, 6, mode=0))
