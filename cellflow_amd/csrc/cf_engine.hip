@@ -168,11 +168,26 @@ struct Engine {
   uint32_t* slot_of = nullptr;
   cf_tuning tun{1, 0, 16, 256, 32};
   uint32_t* hsmall = nullptr;  // pinned readback words
+  // Captured step graphs.  A graph bakes kernel arguments (buffer pointers, sizes, tuning),
+  // so the cache is keyed on them and dropped whenever a buffer is reallocated.
+  struct GraphEntry {
+    std::vector<long long> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<GraphEntry> graphs;
+  long long buf_gen = 0;  // bumped on every reallocation of a buffer a graph references
 
   ~Engine() { release(); }
 
+  void drop_graphs() {
+    for (auto& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+
   void release() {
     if (st) (void)hipStreamSynchronize(st);
+    drop_graphs();
     for (int i = 0; i < 2; ++i) {
       dfree(b.pos[i]);
       dfree(b.vel[i]);
@@ -358,6 +373,8 @@ struct Engine {
   // Cell arrays for the current C; the histogram holds zeros and each pad cell's size.
   int ensure_cells() {
     HIPR(hipStreamSynchronize(st));
+    drop_graphs();
+    ++buf_gen;
     if (C > Ccap) {
       dfree(b.hist);
       dfree(b.start[0]);
@@ -481,6 +498,11 @@ struct Engine {
 
   int ensure_edge_bufs(int maxc) {
     const int cap = 2 * maxc;
+    if (cap > edge_cap_alloc || maxc > edge_maxc_alloc) {
+      HIPR(hipStreamSynchronize(st));
+      drop_graphs();
+      ++buf_gen;
+    }
     if (cap > edge_cap_alloc) {
       HIPR(hipStreamSynchronize(st));
       dfree(b.ecand_d2);
@@ -607,6 +629,45 @@ struct Engine {
     return CF_OK;
   }
 
+  // `count` steps (fused edge candidates when `fused`), replaying a captured graph of
+  // 2*m steps where possible.  Even graph lengths return the state to the same buffer, so
+  // one graph serves every replay.
+  int run_steps(int count, bool fused, int cap) {
+    const int span = std::max(2, (tun.steps_per_graph + 1) / 2 * 2);
+    if (tun.steps_per_graph >= 2 && count >= span) {
+      std::vector<long long> key = {buf_gen, cur, fused ? 1 : 0, cap, span, tun.cell_divisor,
+                                    tun.force_variant, tun.lanes_per_particle, tun.block_size, C,
+                                    P};
+      hipGraphExec_t exec = nullptr;
+      for (auto& g : graphs)
+        if (g.key == key) exec = g.exec;
+      if (!exec) {
+        hipGraph_t graph = nullptr;
+        HIPR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int cur0 = cur;
+        int rc = CF_OK;
+        for (int k = 0; k < span && rc == CF_OK; ++k) rc = step(fused, cap);
+        hipError_t ce = hipStreamEndCapture(st, &graph);
+        cur = cur0;
+        if (rc != CF_OK) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return rc;
+        }
+        HIPR(ce);
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPR(ie);
+        graphs.push_back(GraphEntry{key, exec});
+      }
+      while (count >= span) {
+        HIPR(hipGraphLaunch(exec, st));
+        count -= span;
+      }
+    }
+    for (int k = 0; k < count; ++k) CFR(step(fused, cap));
+    return CF_OK;
+  }
+
   bool fusable(const EdgeReq& er) const {
     if (!er.on) return false;
     for (const SimDesc& d : hdesc) {
@@ -637,9 +698,16 @@ struct Engine {
     const bool fuse = fusable(er);
     const int cap = 2 * er.maxc;
     edges_count = -1;
-    for (int k = 0; k < nsteps; ++k) {
-      CFR(step(fuse && k > 0, cap));
-      if (er.on && !fuse) CFR(edge_pass(cap));
+    if (er.on && !fuse) {
+      for (int k = 0; k < nsteps; ++k) {
+        CFR(step(false, cap));
+        CFR(edge_pass(cap));
+      }
+    } else if (nsteps > 0) {
+      // the first step has no pre-step graph to build; later steps build the graph of
+      // their pre-step state (= the previous step's post-step state) inside the force pass
+      CFR(step(false, cap));
+      CFR(run_steps(nsteps - 1, er.on, cap));
     }
     if (er.on && (fuse || nsteps == 0)) CFR(edge_pass(cap));
     if (er.on) edges_maxc = er.maxc;

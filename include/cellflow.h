@@ -86,7 +86,8 @@ typedef struct cf_tuning {
                                    2: LDS-staged stencil, one wave per workgroup */
   int lanes_per_particle;       /* G of variant 0: 4, 8, 16 or 32 */
   int block_size;               /* threads per workgroup of variant 1: 64, 128, 256 */
-  int steps_per_graph;          /* steps captured per HIP graph replay (>= 1) */
+  int steps_per_graph;          /* steps per captured HIP graph (rounded up to even);
+                                   1 launches every kernel directly */
 } cf_tuning;
 
 typedef struct cf_sim cf_sim;
