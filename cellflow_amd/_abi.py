@@ -93,6 +93,9 @@ SIGNATURES = {
     "cf_get_sort_order": (_i, [_vp, P(C.c_uint32), P(C.c_uint32), _i]),
     "cf_profile_steps": (_i, [_vp, P(cf_params), _i, P(cf_edge_opts), P(C.c_double), _i, P(_i),
                               C.c_char_p, _i]),
+    "cf_pair_histogram": (_i, [_vp, _i, _f, P(C.c_double)]),
+    "cf_rollout_grad": (_i, [_vp, P(cf_params), _i, _i, _i, _f, P(C.c_double), P(C.c_double),
+                             P(C.c_double), P(C.c_double), P(C.c_double)]),
     "cf_set_tuning": (_i, [_vp, P(cf_tuning)]),
     "cf_get_tuning": (_i, [_vp, P(cf_tuning)]),
     "cf_grid_dims": (_i, [P(cf_params), P(_f), _i, _i, P(_i)]),

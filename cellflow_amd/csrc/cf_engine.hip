@@ -933,6 +933,230 @@ struct Engine {
     if (er.on) edges_maxc = er.maxc;
     return finish_call(er.on);
   }
+
+  // ---- differentiable rollout -----------------------------------------------------------
+
+  struct StateBuf {
+    float4* pos = nullptr;
+    float4* vel = nullptr;
+    uint32_t* id = nullptr;
+    uint32_t* start = nullptr;
+  };
+
+  int alloc_state(StateBuf& s) {
+    CFR(dalloc(&s.pos, P));
+    CFR(dalloc(&s.vel, P));
+    CFR(dalloc(&s.id, P));
+    CFR(dalloc(&s.start, C + 1));
+    return CF_OK;
+  }
+
+  static void free_state(StateBuf& s) {
+    dfree(s.pos);
+    dfree(s.vel);
+    dfree(s.id);
+    dfree(s.start);
+  }
+
+  int save_state(StateBuf& s) {
+    HIPR(hipMemcpyAsync(s.pos, b.pos[cur], P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPR(hipMemcpyAsync(s.vel, b.vel[cur], P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPR(hipMemcpyAsync(s.id, b.id[cur], P * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    HIPR(hipMemcpyAsync(s.start, b.start[cur], (C + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                        st));
+    return CF_OK;
+  }
+
+  int load_state(const StateBuf& s) {
+    HIPR(hipMemcpyAsync(b.pos[cur], s.pos, P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPR(hipMemcpyAsync(b.vel[cur], s.vel, P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    HIPR(hipMemcpyAsync(b.id[cur], s.id, P * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    HIPR(hipMemcpyAsync(b.start[cur], s.start, (C + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                        st));
+    return CF_OK;
+  }
+
+  HistArgs hist_args(const StateBuf& s, int bins, double r_max, double* partial, long stride) {
+    const SimDesc& d = hdesc[0];
+    HistArgs h{};
+    h.pos = s.pos;
+    h.id = s.id;
+    h.start = s.start;
+    h.sims = b.sims;
+    h.bins = bins;
+    h.r_max = r_max;
+    const double reach = r_max * (1.0 + 2.0 / bins) * (1.0 + 1e-3);
+    const double ih[3] = {d.ihx, d.ihy, d.ihz};
+    int r = 1;
+    for (int a = 0; a < 3; ++a) r = std::max(r, (int)std::ceil(reach * ih[a]));
+    h.reach = r;
+    h.partial = partial;
+    h.stride = stride;
+    return h;
+  }
+
+  // Soft pair histogram of the current state (FP64, deterministic).
+  int histogram(int bins, double r_max, double* hist_out) {
+    if (sims.size() != 1) return fail(CF_ERR_STATE, "histograms are defined for one simulation");
+    if (!sims[0].have_prm) return fail(CF_ERR_STATE, "parameters not set");
+    EdgeReq none;
+    CFR(prepare(none));
+    const int n = sims[0].n;
+    const long stride = ((long)n + 127) / 128 * 128;
+    StateBuf s;
+    s.pos = b.pos[cur];
+    s.id = b.id[cur];
+    s.start = b.start[cur];
+    double *partial = nullptr, *hd = nullptr;
+    CFR(dalloc(&partial, (size_t)bins * stride));
+    CFR(dalloc(&hd, bins));
+    HIPR(launch_hist(hist_args(s, bins, r_max, partial, stride), n, st));
+    HIPR(launch_reduce_rows(partial, stride, n, bins, 1.0 / n, hd, 0, st));
+    HIPR(hipMemcpyAsync(hist_out, hd, bins * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    dfree(partial);
+    dfree(hd);
+    return CF_OK;
+  }
+
+  // Gradient of the final-state histogram loss after `nsteps` steps with respect to the
+  // force table and the ratio, by the adjoint method with checkpoints every `ckpt` steps.
+  int rollout_grad(int nsteps, int ckpt, int bins, double r_max, const double* target,
+                   double* loss_out, double* hist_out, double* gtab, double* grat) {
+    if (sims.size() != 1) return fail(CF_ERR_STATE, "gradients are defined for one simulation");
+    SimHost& h = sims[0];
+    if (!h.have_prm) return fail(CF_ERR_STATE, "parameters not set");
+    EdgeReq none;
+    CFR(prepare(none));
+    const int n = h.n, T = h.T;
+    const long stride = ((long)n + 127) / 128 * 128;
+    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    std::vector<StateBuf> ck(std::max(nseg, 1)), seg(ckpt + 1);
+    StateBuf fin;
+    double4 *lx[2] = {nullptr, nullptr}, *lv[2] = {nullptr, nullptr}, *phi = nullptr;
+    int* cnt = nullptr;
+    double *partial = nullptr, *hd = nullptr, *gh = nullptr, *gacc = nullptr;
+    float* thr = nullptr;
+    int rc = CF_OK;
+    auto cleanup = [&]() {
+      for (auto& s : ck) free_state(s);
+      for (auto& s : seg) free_state(s);
+      free_state(fin);
+      for (int i = 0; i < 2; ++i) {
+        dfree(lx[i]);
+        dfree(lv[i]);
+      }
+      dfree(phi);
+      dfree(cnt);
+      dfree(partial);
+      dfree(hd);
+      dfree(gh);
+      dfree(gacc);
+      dfree(thr);
+    };
+    auto body = [&]() -> int {
+      for (auto& s : ck) CFR(alloc_state(s));
+      for (auto& s : seg) CFR(alloc_state(s));
+      CFR(alloc_state(fin));
+      for (int i = 0; i < 2; ++i) {
+        CFR(dalloc(&lx[i], P));
+        CFR(dalloc(&lv[i], P));
+      }
+      CFR(dalloc(&phi, P));
+      CFR(dalloc(&cnt, P));
+      const int rows = std::max(bins, T * T + 1);
+      CFR(dalloc(&partial, (size_t)rows * stride));
+      CFR(dalloc(&hd, bins));
+      CFR(dalloc(&gh, bins));
+      CFR(dalloc(&gacc, T * T + 1));
+      CFR(dalloc(&thr, T * T));
+      std::vector<float> hthr(T * T);
+      for (int i = 0; i < T * T; ++i) hthr[i] = hpairs[hdesc[0].tab + i].thr;
+      HIPR(hipMemcpyAsync(thr, hthr.data(), T * T * sizeof(float), hipMemcpyHostToDevice, st));
+      HIPR(hipMemsetAsync(b.flag, 0, sizeof(uint32_t), st));
+      // forward with checkpoints
+      for (int s = 0; s < nsteps; ++s) {
+        if (s % ckpt == 0) CFR(save_state(ck[s / ckpt]));
+        CFR(step(false, 0));
+      }
+      CFR(save_state(fin));
+      // loss of the final state
+      HIPR(launch_hist(hist_args(fin, bins, r_max, partial, stride), n, st));
+      HIPR(launch_reduce_rows(partial, stride, n, bins, 1.0 / n, hd, 0, st));
+      std::vector<double> hh(bins), g(bins);
+      HIPR(hipMemcpyAsync(hh.data(), hd, bins * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPR(hipStreamSynchronize(st));
+      double loss = 0.0;
+      for (int bb = 0; bb < bins; ++bb) {
+        loss += (hh[bb] - target[bb]) * (hh[bb] - target[bb]);
+        g[bb] = 2.0 * (hh[bb] - target[bb]);
+      }
+      *loss_out = loss;
+      if (hist_out) std::copy(hh.begin(), hh.end(), hist_out);
+      HIPR(hipMemcpyAsync(gh, g.data(), bins * sizeof(double), hipMemcpyHostToDevice, st));
+      HistArgs ha = hist_args(fin, bins, r_max, partial, stride);
+      ha.gh = gh;
+      ha.lam_x = lx[0];
+      ha.lam_v = lv[0];
+      HIPR(launch_hist_grad(ha, n, st));
+      HIPR(hipMemsetAsync(gacc, 0, (T * T + 1) * sizeof(double), st));
+      AdjArgs aa{};
+      aa.sims = b.sims;
+      aa.thr = thr;
+      aa.c.T = T;
+      aa.c.radius = h.prm.radius;
+      aa.c.ratio = h.prm.ratio_with_lfo;
+      aa.c.rep = h.prm.repulsion;
+      aa.c.att = h.prm.attraction;
+      aa.c.k = h.prm.k;
+      for (int t = 0; t < T; ++t) aa.c.rbt[t] = h.rbt[t];
+      for (int t = 0; t < T * T; ++t) aa.c.table[t] = h.table[t];
+      aa.cnt_by_id = cnt;
+      aa.phi = phi;
+      aa.partial = partial;
+      aa.stride = stride;
+      int L = 0;  // lx[L] holds dL/dx of the state after the step being reversed
+      for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int k0 = sg * ckpt, k1 = std::min(nsteps, k0 + ckpt);
+        CFR(load_state(ck[sg]));
+        CFR(save_state(seg[0]));
+        for (int s = k0; s < k1; ++s) {
+          CFR(step(false, 0));
+          CFR(save_state(seg[s - k0 + 1]));
+        }
+        for (int s = k1 - 1; s >= k0; --s) {
+          const StateBuf& pre = seg[s - k0];
+          const StateBuf& post = seg[s - k0 + 1];
+          aa.pos = pre.pos;
+          aa.vel = pre.vel;
+          aa.id = pre.id;
+          aa.start = pre.start;
+          aa.vel_next = post.vel;
+          aa.id_next = post.id;
+          aa.lam_x_next = lx[L];
+          aa.lam_v_next = lv[L];
+          aa.lam_x = lx[1 - L];
+          aa.lam_v = lv[1 - L];
+          HIPR(launch_adj_step(aa, n, st));
+          HIPR(launch_reduce_rows(partial, stride, n, T * T + 1, 1.0, gacc, 1, st));
+          L = 1 - L;
+        }
+      }
+      std::vector<double> gg(T * T + 1);
+      HIPR(hipMemcpyAsync(gg.data(), gacc, (T * T + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+      CFR(load_state(fin));
+      HIPR(hipMemcpyAsync(hsmall, b.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      HIPR(hipStreamSynchronize(st));
+      std::copy(gg.begin(), gg.begin() + T * T, gtab);
+      *grat = gg[T * T];
+      if (hsmall[0] & 1u) return fail(CF_ERR_NONFINITE, "a position or velocity became non-finite");
+      return CF_OK;
+    };
+    rc = body();
+    (void)hipStreamSynchronize(st);
+    cleanup();
+    return rc;
+  }
 };
 
 }  // namespace cf
@@ -1375,6 +1599,27 @@ int cf_get_tuning(cf_sim* sim, cf_tuning* tuning) {
   if (!tuning) return cf::fail(CF_ERR_ARG, "tuning is null");
   *tuning = sim->e->tun;
   return CF_OK;
+}
+
+int cf_pair_histogram(cf_sim* sim, int bins, float r_max, double* hist) {
+  CFR(check(sim));
+  if (!hist || bins < 1 || bins > cf::kMaxBins || !(r_max > 0.0f))
+    return cf::fail(CF_ERR_ARG, "bins must be 1..%d and r_max > 0", cf::kMaxBins);
+  return sim->e->histogram(bins, r_max, hist);
+}
+
+int cf_rollout_grad(cf_sim* sim, const cf_params* params, int nsteps, int checkpoint_every,
+                    int bins, float r_max, const double* target_hist, double* loss, double* hist,
+                    double* grad_table, double* grad_ratio) {
+  CFR(check(sim));
+  if (nsteps < 1 || checkpoint_every < 1 || !target_hist || !loss || !grad_table || !grad_ratio ||
+      bins < 1 || bins > cf::kMaxBins || !(r_max > 0.0f))
+    return cf::fail(CF_ERR_ARG, "bad arguments");
+  CFR(check_params(params, sim->e->sims[0].T));
+  sim->e->sims[0].prm = *params;
+  sim->e->sims[0].have_prm = true;
+  return sim->e->rollout_grad(nsteps, checkpoint_every, bins, r_max, target_hist, loss, hist,
+                              grad_table, grad_ratio);
 }
 
 // ---- ensemble ------------------------------------------------------------------------

@@ -219,6 +219,57 @@ struct LineArgs {
   int p0;
 };
 
+// ---- differentiable rollout (cf_adjoint.hip) ----
+constexpr int kMaxBins = 64;
+
+struct AdjConst {
+  int T;
+  double radius, ratio, rep, att, k;
+  double rbt[kMaxTypes];
+  double table[kMaxTypes * kMaxTypes];
+};
+
+struct HistArgs {
+  const float4* pos;
+  const uint32_t* id;
+  const uint32_t* start;
+  const SimDesc* sims;
+  int bins;
+  double r_max;
+  int reach;            // cells covering r_max + 2 bin widths
+  double* partial;      // [bins][stride] per-particle histograms
+  long stride;
+  const double* gh;     // dL/dh_b
+  double4* lam_x;
+  double4* lam_v;
+};
+
+struct AdjArgs {
+  const float4* pos;        // pre-step state S_k (sorted)
+  const float4* vel;
+  const uint32_t* id;
+  const uint32_t* start;
+  const float4* vel_next;   // post-step state S_k+1 (its previous counts are this step's counts)
+  const uint32_t* id_next;
+  const SimDesc* sims;
+  const float* thr;         // T*T cutoff thresholds of the forward
+  AdjConst c;
+  int* cnt_by_id;
+  const double4* lam_x_next;
+  const double4* lam_v_next;
+  double4* lam_x;
+  double4* lam_v;
+  double4* phi;
+  double* partial;          // [T*T + 1][stride] per-particle parameter partials
+  long stride;
+};
+
+hipError_t launch_hist(const HistArgs& a, int n, hipStream_t st);
+hipError_t launch_hist_grad(const HistArgs& a, int n, hipStream_t st);
+hipError_t launch_reduce_rows(const double* in, long stride, int n, int rows, double scale,
+                              double* out, int accumulate, hipStream_t st);
+hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st);
+
 // host-side launchers (cf_kernels.hip)
 hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
                         int lanes, hipStream_t st);

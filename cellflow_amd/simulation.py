@@ -246,6 +246,31 @@ class ParticleSimulation:
                                          len(col), _fp(out), 6 * vc.value, C.byref(vc)))
         return out[: 6 * vc.value].reshape(-1, 6)
 
+    # -- differentiable rollout ------------------------------------------------------------
+    def pairHistogram(self, bins: int, r_max: float) -> np.ndarray:
+        """Soft pair histogram of the current state (cubic B-spline bins, FP64)."""
+        h = np.zeros(bins)
+        check(lib().cf_pair_histogram(self._h, int(bins), float(r_max),
+                                      h.ctypes.data_as(C.POINTER(C.c_double))))
+        return h
+
+    def rolloutGrad(self, params: SimulationParams, steps: int, checkpoint_every: int, bins: int,
+                    r_max: float, target):
+        """Roll `steps` steps from the current state and return (loss, histogram,
+        dL/dtable (T x T), dL/dratio) of L = sum_b (h_b - target_b)^2 at the final state."""
+        T = self.getNumParticleTypes()
+        tgt = np.ascontiguousarray(target, np.float64)
+        loss = C.c_double()
+        h = np.zeros(bins)
+        gt = np.zeros(T * T)
+        gr = C.c_double()
+        p = params.to_c()
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        check(lib().cf_rollout_grad(self._h, C.byref(p), int(steps), int(checkpoint_every),
+                                    int(bins), float(r_max), dp(tgt), C.byref(loss), dp(h),
+                                    dp(gt), C.byref(gr)))
+        return loss.value, h, gt.reshape(T, T), gr.value
+
     # -- introspection --------------------------------------------------------------------
     def getGrid(self):
         d = (C.c_int * 3)()

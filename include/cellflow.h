@@ -185,6 +185,21 @@ int cf_profile_steps(cf_sim* sim, const cf_params* params, int nsteps, const cf_
                      double* ms_out, int max_kernels, int* nkernels, char* names_buf,
                      int names_cap);
 
+/* ---- differentiable rollout (no reference counterpart) ------------------------------- */
+
+/* Soft pair histogram of the current state: h_b = (1/N) sum_i sum_{j != i} B3((r_ij - c_b)/w),
+ * w = r_max / bins, c_b = (b + 1/2) w, B3 the cubic B-spline on [-2, 2], r_ij the minimum-
+ * image distance.  bins <= 64.  FP64, deterministic. */
+int cf_pair_histogram(cf_sim* sim, int bins, float r_max, double* hist);
+/* Runs `nsteps` steps from the current state (leaving the simulation at the final state)
+ * and returns the loss L = sum_b (h_b - target_b)^2 of the final histogram and its gradient
+ * with respect to the num_types^2 force-table entries (row-major) and ratio_with_lfo, by
+ * a hand-written adjoint with state checkpoints every `checkpoint_every` steps.  The
+ * rollout's discrete decisions (which pairs interact, neighbour counts) are held fixed. */
+int cf_rollout_grad(cf_sim* sim, const cf_params* params, int nsteps, int checkpoint_every,
+                    int bins, float r_max, const double* target_hist, double* loss,
+                    double* hist, double* grad_table, double* grad_ratio);
+
 /* ---- tuning ------------------------------------------------------------------------- */
 
 int cf_set_tuning(cf_sim* sim, const cf_tuning* tuning);

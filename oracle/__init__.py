@@ -190,3 +190,110 @@ def eff_radius(phys, rbt, ti, tj):
     p = make_params(phys)
     rbt = np.ascontiguousarray(rbt, np.float32)
     return lib().orc_eff_radius(C.byref(p), _p(rbt, C.c_float), ti, tj)
+
+
+# ---- differentiable rollout ---------------------------------------------------------------
+
+def _bind_adjoint(L):
+    P = C.POINTER
+    if getattr(L, "_adj_bound", False):
+        return L
+    L.orc_hist.argtypes = [C.c_int, P(C.c_double), P(C.c_float), C.c_int, C.c_double,
+                           P(C.c_double), C.c_int]
+    L.orc_pairs.argtypes = [C.c_int, P(C.c_float), P(C.c_uint32), P(OrcParams), P(C.c_float),
+                            P(C.c_int64), P(C.c_int32), C.c_int64, C.c_int]
+    L.orc_frozen_rollout.argtypes = [C.c_int, P(C.c_double), P(C.c_double), P(C.c_uint32),
+                                     P(C.c_int32), P(OrcParams), P(C.c_float), P(C.c_double),
+                                     C.c_double, P(C.c_float), C.c_int, P(P(C.c_int64)),
+                                     P(P(C.c_int32)), C.c_int, C.c_double, P(C.c_double),
+                                     P(C.c_double), P(C.c_double), P(C.c_double), C.c_int]
+    L._adj_bound = True
+    return L
+
+
+def hist(pos, L, bins, r_max, nthreads=0):
+    """Soft pair histogram h_b = (1/N) sum_{i != j} B3((r_ij - c_b)/w) of FP64 positions."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    Lf = np.ascontiguousarray(L, np.float32)
+    out = np.zeros(bins)
+    r_max = float(np.float32(r_max))  # the engine's ABI takes r_max as FP32
+    _bind_adjoint(lib()).orc_hist(len(pos), _p(pos, C.c_double), _p(Lf, C.c_float), bins,
+                                  r_max, _p(out, C.c_double), nthreads)
+    return out
+
+
+def pairs(pos, ptype, phys, rbt, nthreads=0):
+    """Neighbour lists (CSR offsets, indices) of one state by the FP32 predicate."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    ptype = np.ascontiguousarray(ptype, np.uint32)
+    rbt = np.ascontiguousarray(rbt, np.float32)
+    n = len(ptype)
+    p = make_params(phys)
+    cap = n * 64
+    while True:
+        offs = np.zeros(n + 1, np.int64)
+        nbr = np.zeros(max(cap, 1), np.int32)
+        rc = _bind_adjoint(lib()).orc_pairs(n, _p(pos, C.c_float), _p(ptype, C.c_uint32),
+                                            C.byref(p), _p(rbt, C.c_float), _p(offs, C.c_int64),
+                                            _p(nbr, C.c_int32), cap, nthreads)
+        if rc == 0:
+            return offs, nbr[: offs[-1]].copy()
+        cap *= 4
+
+
+class FrozenRollout:
+    """FP64 rollouts whose per-step neighbour sets (and counts) are fixed in advance, e.g. to
+    the decisions of a given trajectory.  loss(table, ratio) is then a smooth function of the
+    parameters, which central finite differences can differentiate."""
+
+    def __init__(self, pos0, vel0, ptype, prev0, phys, rbt, lists, bins, r_max, target,
+                 nthreads=0):
+        self.pos0 = np.ascontiguousarray(pos0, np.float64)
+        self.vel0 = np.ascontiguousarray(vel0, np.float64)
+        self.ptype = np.ascontiguousarray(ptype, np.uint32)
+        self.prev0 = np.ascontiguousarray(prev0, np.int32)
+        self.p = make_params(phys)
+        self.rbt = np.ascontiguousarray(rbt, np.float32)
+        self.lists = lists
+        self.steps = len(lists)
+        self._offs = (C.POINTER(C.c_int64) * self.steps)(*[_p(o, C.c_int64) for o, _ in lists])
+        self._nbr = (C.POINTER(C.c_int32) * self.steps)(*[_p(b, C.c_int32) for _, b in lists])
+        self.bins, self.r_max = bins, float(np.float32(r_max))  # FP32 like the engine's ABI
+        self.target = np.ascontiguousarray(target, np.float64)
+        self.nthreads = nthreads
+
+    def loss(self, table, ratio, want_hist=False):
+        table = np.ascontiguousarray(table, np.float64).ravel()
+        tf = table.astype(np.float32)
+        out = C.c_double()
+        h = np.zeros(self.bins)
+        _bind_adjoint(lib()).orc_frozen_rollout(
+            len(self.ptype), _p(self.pos0, C.c_double), _p(self.vel0, C.c_double),
+            _p(self.ptype, C.c_uint32), _p(self.prev0, C.c_int32), C.byref(self.p),
+            _p(tf, C.c_float), _p(table, C.c_double), float(ratio), _p(self.rbt, C.c_float),
+            self.steps, self._offs, self._nbr, self.bins, self.r_max,
+            _p(self.target, C.c_double), C.byref(out), None, _p(h, C.c_double), self.nthreads)
+        return (out.value, h) if want_hist else out.value
+
+    def fd_gradient(self, table, ratio, rel_step=1e-4, abs_step=1e-6):
+        """Central differences with respect to the T*T table entries and the ratio."""
+        table = np.asarray(table, np.float64)
+        T = table.shape[0]
+        g = np.zeros(T * T + 1)
+        for k in range(T * T + 1):
+            if k < T * T:
+                x0 = table.flat[k]
+            else:
+                x0 = ratio
+            h = max(abs(x0) * rel_step, abs_step)
+            vals = []
+            for sgn in (1, -1):
+                tb = table.copy()
+                rt = ratio
+                if k < T * T:
+                    tb.flat[k] = x0 + sgn * h
+                else:
+                    rt = x0 + sgn * h
+                vals.append(self.loss(tb, rt))
+            g[k] = (vals[0] - vals[1]) / (2 * h)
+        return g[: T * T].reshape(T, T), g[T * T]

@@ -22,7 +22,11 @@
 //   orc_bin / orc_order the engine's own uniform-grid rule (no reference counterpart: the
 //                       reference enumerates all pairs, :86)
 //   orc_rollout         repeated orc_step with the neighbour-count ping-pong of :541-556
-//   orc_hist / orc_loss the differentiable-rollout loss (engine extension, no reference code)
+//   orc_hist            soft pair histogram of the differentiable-rollout loss (engine
+//                       extension, no reference code)
+//   orc_pairs           one step's neighbour lists by the FP32 predicate
+//   orc_frozen_rollout  FP64 rollout with given (frozen) neighbour lists, for central finite
+//                       differences of the rollout loss
 //
 // Pinning: every function above is checked in tests/test_oracle.py against closed forms
 // (two-particle force, isolated particle, balance=1, zero table), invariants (pair
@@ -402,6 +406,147 @@ void orc_force_table(const float* raw, int count, float range, float bias, float
     float transformed = tanhf(raw[i] * offset) * range + bias;
     out[i] = fmaxf(-1.0f, fminf(1.0f, transformed));
   }
+}
+
+// ---- differentiable rollout (engine extension, no reference counterpart) ---------------
+//
+// Soft pair histogram: h_b = (1/N) sum_i sum_{j != i} B3((r_ij - c_b) / w), c_b = (b+0.5) w,
+// w = r_max / bins, r_ij the minimum-image distance (no softening), B3 the cubic B-spline
+// on [-2, 2].  Loss = sum_b (h_b - target_b)^2.
+
+static inline double bspline3(double u) {
+  u = std::fabs(u);
+  if (u >= 2.0) return 0.0;
+  if (u >= 1.0) {
+    const double t = 2.0 - u;
+    return t * t * t / 6.0;
+  }
+  return (4.0 - 6.0 * u * u + 3.0 * u * u * u) / 6.0;
+}
+
+// FP64 soft histogram of FP64 positions (all pairs within r_max + 2w, grid enumeration).
+int orc_hist(int n, const double* pos, const float* L, int bins, double r_max, double* hist,
+             int nthreads) {
+  set_threads(nthreads);
+  const double w = r_max / bins;
+  const double reach = r_max + 2.0 * w;
+  std::vector<float> posf(3 * (size_t)n);
+  for (size_t k = 0; k < 3 * (size_t)n; ++k) posf[k] = (float)pos[k];
+  Grid g = build_grid(n, posf.data(), L, reach);
+  int r = 1;
+  for (int a = 0; a < 3; ++a) r = std::max(r, (int)std::ceil(reach / g.h[a] + 1e-9));
+  std::vector<double> part((size_t)n * bins, 0.0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    double* hi = &part[(size_t)i * bins];
+    for_each_near(g, posf.data(), i, r, true, [&](int j) {
+      if (j == i) return;
+      double s = 0;
+      for (int a = 0; a < 3; ++a) {
+        const double d = min_image_d(pos[3 * j + a] - pos[3 * i + a], (double)L[a]);
+        s += d * d;
+      }
+      const double rij = std::sqrt(s);
+      if (rij >= reach) return;
+      const double u = rij / w - 0.5;
+      const int b0 = (int)std::floor(u) - 1;
+      for (int b = std::max(b0, 0); b <= std::min(b0 + 3, bins - 1); ++b) hi[b] += bspline3(u - b);
+    });
+  }
+  for (int b = 0; b < bins; ++b) {
+    double acc = 0;
+    for (int i = 0; i < n; ++i) acc += part[(size_t)i * bins + b];
+    hist[b] = acc / n;
+  }
+  return 0;
+}
+
+// Neighbour lists of one step by the FP32 predicate on FP32 positions (the decisions the
+// engine makes for that state): CSR offsets (n+1) and neighbour indices, j ascending.
+int orc_pairs(int n, const float* pos, const uint32_t* type, const orc_params* pp,
+              const float* rbt, int64_t* offs, int32_t* nbr, int64_t cap, int nthreads) {
+  const orc_params& p = *pp;
+  set_threads(nthreads);
+  double reach = max_eff_radius(p, rbt);
+  std::vector<std::vector<int32_t>> lists(n);
+  if (reach > 0) {
+    Grid g = build_grid(n, pos, p.L, reach);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+      auto& li = lists[i];
+      for_each_near(g, pos, i, 1, true, [&](int j) {
+        if (j != i && pair_in_range_f(p, rbt, &pos[3 * i], &pos[3 * j], type[i], type[j]))
+          li.push_back(j);
+      });
+      std::sort(li.begin(), li.end());
+    }
+  }
+  offs[0] = 0;
+  for (int i = 0; i < n; ++i) offs[i + 1] = offs[i] + (int64_t)lists[i].size();
+  if (offs[n] > cap) return 3;
+  for (int i = 0; i < n; ++i)
+    std::copy(lists[i].begin(), lists[i].end(), nbr + offs[i]);
+  return 0;
+}
+
+// Frozen-decision FP64 rollout: step k uses the neighbour lists (offs[k], nbr[k]) and their
+// sizes as counts, whatever the continuous state does; table and ratio enter only the
+// continuous force.  Returns the loss of the final state and (optionally) the final state.
+int orc_frozen_rollout(int n, const double* pos0, const double* vel0, const uint32_t* type,
+                       const int32_t* prev0, const orc_params* pp, const float* table_f,
+                       const double* table_d, double ratio, const float* rbt, int steps,
+                       const int64_t* const* offs, const int32_t* const* nbr, int bins,
+                       double r_max, const double* target, double* loss_out, double* pos_out,
+                       double* hist_out, int nthreads) {
+  const orc_params& p = *pp;
+  set_threads(nthreads);
+  std::vector<double> pos(pos0, pos0 + 3 * (size_t)n), vel(vel0, vel0 + 3 * (size_t)n);
+  std::vector<double> pos2(3 * (size_t)n), vel2(3 * (size_t)n);
+  std::vector<int32_t> prev(prev0, prev0 + n);
+  const int T = p.T;
+  (void)table_f;
+  for (int s = 0; s < steps; ++s) {
+    const int64_t* o = offs[s];
+    const int32_t* nb = nbr[s];
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+      const unsigned ti = type[i];
+      double F[3] = {0, 0, 0};
+      for (int64_t q = o[i]; q < o[i + 1]; ++q) {
+        const int j = nb[q];
+        const unsigned tj = type[j];
+        double d[3];
+        for (int a = 0; a < 3; ++a) d[a] = min_image_d(pos[3 * j + a] - pos[3 * i + a], (double)p.L[a]);
+        const double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + (double)0.0001f);
+        const double R = p.radius;
+        const double eff = (R * (1.0 + (double)rbt[ti] * ratio) + R * (1.0 + (double)rbt[tj] * ratio)) * 0.5;
+        const double r = dist / eff;
+        const double net = ((double)p.repulsion * std::exp(-(double)p.k * r * r) - (double)p.attraction * r) * table_d[ti * T + tj];
+        for (int a = 0; a < 3; ++a) F[a] += d[a] / dist * net;
+      }
+      const int count = (int)(o[i + 1] - o[i]);
+      const double avg = (double)(count + prev[i]) * 0.5;
+      const double dens = std::fmin(avg / (double)p.max_expected_neighbors, 1.0);
+      const double fac = 1.0 - (1.0 - (double)p.balance) * dens;
+      for (int a = 0; a < 3; ++a) {
+        const double v = vel[3 * i + a] * (double)p.friction + F[a] * (double)p.force_multiplier * fac * (double)p.dt;
+        const double L = (double)p.L[a];
+        vel2[3 * i + a] = v;
+        pos2[3 * i + a] = std::fmod(pos[3 * i + a] + v * (double)p.dt + L, L);
+      }
+    }
+    pos.swap(pos2);
+    vel.swap(vel2);
+    for (int i = 0; i < n; ++i) prev[i] = (int32_t)(o[i + 1] - o[i]);
+  }
+  std::vector<double> h(bins);
+  orc_hist(n, pos.data(), p.L, bins, r_max, h.data(), nthreads);
+  double loss = 0;
+  for (int b = 0; b < bins; ++b) loss += (h[b] - target[b]) * (h[b] - target[b]);
+  *loss_out = loss;
+  if (pos_out) std::copy(pos.begin(), pos.end(), pos_out);
+  if (hist_out) std::copy(h.begin(), h.end(), hist_out);
+  return 0;
 }
 
 }  // extern "C"

@@ -1,0 +1,154 @@
+"""GPU: the differentiable rollout (hand-written adjoint) against FP64 finite differences.
+
+The engine's gradient is of the rollout with its discrete decisions (interacting pairs,
+neighbour counts) held fixed.  The oracle reproduces exactly that function: its neighbour
+lists for step k are computed with the engine's FP32 predicate from the engine's own FP32
+state at step k (bit-identical decisions), and its continuous arithmetic is FP64.  Central
+finite differences of that FP64 function are the reference gradient.
+
+Tolerance: max_k |g_engine_k - g_fd_k| <= 1e-4 * max_k |g_fd_k| over the T*T + 1 components
+(plus a per-component check for components above 1% of the largest).
+
+The torch.autograd.Function below wraps the C ABI for the test harness only.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from cellflow_amd import SimulationParams, configs
+from helpers import make_sim
+
+pytestmark = pytest.mark.gpu
+
+GRAD_RTOL = 1e-4
+
+
+class RolloutLoss(torch.autograd.Function):
+    """loss(table, ratio) of an engine rollout; backward returns the adjoint gradient."""
+
+    @staticmethod
+    def forward(ctx, table, ratio, run):
+        loss, _, gt, gr = run(table.detach().cpu().numpy(), float(ratio))
+        ctx.save_for_backward(torch.from_numpy(gt), torch.tensor(gr, dtype=torch.float64))
+        return torch.tensor(loss, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, g):
+        gt, gr = ctx.saved_tensors
+        return g * gt, g * gr, None
+
+
+def setup(cfg):
+    st = configs.make_state(cfg)
+    ph = configs.physics(cfg)
+    return st, ph
+
+
+def target_hist(cfg, steps, bins, r_max):
+    st, ph = setup(dict(cfg, seed=cfg.get("target_seed", 1)))
+    sim = make_sim(st, ph)
+    sim.simulate(SimulationParams.from_physics(ph), steps)
+    return sim.pairHistogram(bins, r_max)
+
+
+def engine_states(st, ph, steps):
+    """FP32 positions and previous counts of every pre-step state 0..steps-1."""
+    sim = make_sim(st, ph)
+    prm = SimulationParams.from_physics(ph)
+    pos = []
+    for _ in range(steps):
+        pos.append(sim.getParticleData()["pos"].copy())
+        sim.simulate(prm, 1)
+    return pos
+
+
+def check_against_fd(cfg, steps, ckpt, bins, r_max):
+    st, ph = setup(cfg)
+    tgt = target_hist(cfg, steps, bins, r_max)
+    sim = make_sim(st, ph)
+    loss, h, gt, gr = sim.rolloutGrad(SimulationParams.from_physics(ph), steps, ckpt, bins, r_max,
+                                      tgt)
+    # oracle: FP64 rollout with the engine trajectory's decisions
+    pos = engine_states(st, ph, steps)
+    lists = [oracle.pairs(p, st["type"], ph, st["rbt"]) for p in pos]
+    fr = oracle.FrozenRollout(st["pos"], st["vel"], st["type"], st["prev"], ph, st["rbt"], lists,
+                              bins, r_max, tgt)
+    loss64, h64 = fr.loss(st["table"].astype(np.float64), ph["ratio"], want_hist=True)
+    fd_t, fd_r = fr.fd_gradient(st["table"].astype(np.float64), ph["ratio"])
+    g = np.concatenate([gt.ravel(), [gr]])
+    f = np.concatenate([fd_t.ravel(), [fd_r]])
+    scale = np.abs(f).max()
+    err = np.abs(g - f) / scale
+    big = np.abs(f) > 0.01 * scale
+    rel_big = (np.abs(g - f)[big] / np.abs(f)[big]).max()
+    print(f"loss engine {loss:.10g} oracle {loss64:.10g}; max grad err {err.max():.3g} of max |g|;"
+          f" worst relative on large components {rel_big:.3g}")
+    assert abs(loss - loss64) <= 1e-4 * abs(loss64)
+    np.testing.assert_allclose(h, h64, rtol=1e-4, atol=1e-6)
+    assert err.max() <= GRAD_RTOL
+    assert rel_big <= 10 * GRAD_RTOL
+    return g
+
+
+def small_cfg(n=3000, T=4, seed=0):
+    cfg = dict(configs.load("c5_adjoint_20k"))
+    side = (n / 20000.0) ** (1.0 / 3.0)
+    cfg.update(n=n, num_types=T, canvas=[side] * 3, seed=seed)
+    return cfg
+
+
+def test_histogram_matches_oracle():
+    cfg = small_cfg()
+    st, ph = setup(cfg)
+    sim = make_sim(st, ph)
+    sim.simulate(SimulationParams.from_physics(ph), 5)
+    h = sim.pairHistogram(32, 0.1)
+    ref = oracle.hist(sim.getParticleData()["pos"].astype(np.float64), ph["canvas"], 32, 0.1)
+    np.testing.assert_allclose(h, ref, rtol=1e-12, atol=1e-14)
+    assert abs(h.sum() - ref.sum()) < 1e-9
+
+
+@pytest.mark.parametrize("steps,ckpt", [(40, 10), (37, 7)])
+def test_adjoint_matches_finite_differences_small(steps, ckpt):
+    check_against_fd(small_cfg(), steps, ckpt, 16, 0.1)
+
+
+def test_checkpoint_interval_and_repeat_bit_identical():
+    cfg = small_cfg(n=2000)
+    st, ph = setup(cfg)
+    tgt = np.linspace(0.5, 3.0, 16)
+    prm = SimulationParams.from_physics(ph)
+    outs = []
+    for ckpt in (5, 30, 5):
+        sim = make_sim(st, ph)
+        loss, h, gt, gr = sim.rolloutGrad(prm, 30, ckpt, 16, 0.1, tgt)
+        outs.append((loss, gt.tobytes(), gr))
+    assert outs[0] == outs[1] == outs[2]
+
+
+def test_autograd_function_wraps_adjoint():
+    cfg = small_cfg(n=2000)
+    st, ph = setup(cfg)
+    tgt = np.linspace(0.5, 3.0, 16)
+
+    def run(table, ratio):
+        s2 = dict(st, table=table.astype(np.float32))
+        sim = make_sim(s2, ph)
+        return sim.rolloutGrad(SimulationParams.from_physics(dict(ph, ratio=ratio)), 20, 10, 16,
+                               0.1, tgt)
+
+    table = torch.tensor(st["table"], dtype=torch.float64, requires_grad=True)
+    ratio = torch.tensor(ph["ratio"], dtype=torch.float64, requires_grad=True)
+    loss = RolloutLoss.apply(table, ratio, run)
+    gt, gr = torch.autograd.grad(3.0 * loss, [table, ratio])
+    _, _, gt0, gr0 = run(st["table"], ph["ratio"])
+    np.testing.assert_array_equal(gt.numpy(), 3.0 * gt0)
+    assert gr.item() == 3.0 * gr0
+
+
+def test_adjoint_full_config_against_finite_differences():
+    """The 20,000-particle, 200-step configuration with checkpoints every 20 steps."""
+    cfg = configs.load("c5_adjoint_20k")
+    check_against_fd(cfg, cfg["steps"], cfg["checkpoint_every"], cfg["bins"], cfg["r_max"])
