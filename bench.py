@@ -17,6 +17,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -45,7 +46,7 @@ def read_traffic(path):
     return None
 
 
-def cpu_baseline(cfg, budget_s=12.0):
+def cpu_baseline(cfg, budget_s=10.0):
     import numpy as np
 
     import oracle
@@ -67,7 +68,7 @@ def cpu_baseline(cfg, budget_s=12.0):
                      mode=1, nthreads=threads)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 200:
+        if el >= budget_s or steps >= 5000:
             break
     return {"value": cfg["n"] * steps / el, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
@@ -84,7 +85,10 @@ def ensemble_leg(steps, warm=3):
     sims = configs.make_ensemble(cfg)
     total = sum(len(st["type"]) for _, st in sims)
     log(f"[bench] ensemble: {len(sims)} simulations, {total} particles, {steps} steps")
+    cache = os.path.join(tempfile.mkdtemp(prefix="cf_tune_"), "tuning.json")
     ens = Ensemble(sims)
+    tuned = ens.autotune(cache=cache)
+    log(f"[bench] ensemble tuning {tuned['tuning']} ({tuned['us_per_step']:.1f} us/step)")
     ens.simulate(warm)
     ens.getNeighborCounts(0)
     t0 = time.perf_counter()
@@ -99,6 +103,7 @@ def ensemble_leg(steps, warm=3):
         p.setRadioByType(st["rbt"])
         p.setParticles(particles_from_state(st["pos"], st["vel"], st["type"]))
         prm = SimulationParams.from_physics(ph)
+        p.autotune(prm, cache=cache)  # plain runs are tuned per workload bucket too
         p.simulate(prm, warm)
         plain.append((p, prm))
     plain[-1][0].getNeighborCounts()
@@ -113,7 +118,53 @@ def ensemble_leg(steps, warm=3):
             "sequential_s": t_seq, "speedup": t_seq / t_ens,
             "ensemble_particle_steps_per_s": total * steps / t_ens,
             "sequential_particle_steps_per_s": total * steps / t_seq,
-            "workload": cfg["name"]}
+            "workload": cfg["name"], "tuning": tuned["tuning"]}
+
+
+def adjoint_leg():
+    """Differentiable rollout (20k particles, 200 steps, checkpoints every 20): time of the
+    gradient rollout against the forward rollout; central differences over the 37
+    parameters would need 74 forward rollouts."""
+    from cellflow_amd import ParticleSimulation, SimulationParams, configs, particles_from_state
+    cfg = configs.load("c5_adjoint_20k")
+
+    def make(seed):
+        st = configs.make_state(cfg, seed)
+        ph = configs.physics(cfg)
+        sim = ParticleSimulation(cfg["n"], cfg["num_types"])
+        sim.setForceTable(st["table"])
+        sim.setRadioByType(st["rbt"])
+        sim.setParticles(particles_from_state(st["pos"], st["vel"], st["type"]))
+        return sim, st, SimulationParams.from_physics(ph)
+
+    steps, ck, bins, rmax = cfg["steps"], cfg["checkpoint_every"], cfg["bins"], cfg["r_max"]
+    tsim, _, tprm = make(cfg["target_seed"])
+    tsim.simulate(tprm, steps)
+    target = tsim.pairHistogram(bins, rmax)
+    tsim.close()
+    sim, st, prm = make(cfg["seed"])
+    parts = particles_from_state(st["pos"], st["vel"], st["type"])
+    sim.simulate(prm, 2)  # warm: grid, graphs
+    sim.rolloutGrad(prm, 20, 10, bins, rmax, target)
+    t_fwd, t_grad = [], []
+    for _ in range(3):
+        sim.setParticles(parts)
+        sim.getNeighborCounts()
+        t0 = time.perf_counter()
+        sim.simulate(prm, steps)
+        sim.pairHistogram(bins, rmax)
+        t_fwd.append(time.perf_counter() - t0)
+        sim.setParticles(parts)
+        sim.getNeighborCounts()
+        t0 = time.perf_counter()
+        loss, _, _, _ = sim.rolloutGrad(prm, steps, ck, bins, rmax, target)
+        t_grad.append(time.perf_counter() - t0)
+    sim.close()
+    tf, tg = min(t_fwd), min(t_grad)
+    nparam = cfg["num_types"] ** 2 + 1
+    return {"workload": cfg["name"], "particles": cfg["n"], "steps": steps,
+            "checkpoint_every": ck, "forward_s": tf, "gradient_s": tg, "cost_ratio": tg / tf,
+            "fd_rollouts": 2 * nparam, "speedup_vs_fd": 2 * nparam * tf / tg, "loss": loss}
 
 
 def main():
@@ -125,6 +176,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -162,6 +214,8 @@ def main():
     sim.setForceTable(st["table"])
     sim.setRadioByType(st["rbt"])
     sim.setParticles(particles_from_state(st["pos"], st["vel"], st["type"]))
+    tuned = sim.autotune(params, edges=edges, trial_steps=64)
+    log(f"[bench] tuning {tuned['tuning']} ({tuned['us_per_step']:.1f} us/step in trials)")
     log(f"[bench] warmup {args.warmup} steps")
     sim.simulate(params, args.warmup, edges=edges)
     torch.cuda.synchronize()
@@ -193,10 +247,18 @@ def main():
            "config": {"workload": cfg["name"], "particles": n, "types": cfg["num_types"],
                       "box": cfg["canvas"][0], "radius": cfg["radius"], "ratio": cfg["ratio"],
                       "edges": e, "edges_last_step": nedges, "grid": sim.getGrid()[0]},
-           "roofline": roof}
+           "roofline": roof,
+           "autotune": {"tuning": tuned["tuning"], "key": tuned["key"],
+                        "us_per_step": tuned["us_per_step"],
+                        "trials": [{k: t[k] for k in ("cell_divisor", "force_variant",
+                                                      "lanes_per_particle", "block_size",
+                                                      "us_per_step")} for t in tuned["trials"]]}}
     sim.close()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_steps)
+    if not args.no_adjoint:
+        log("[bench] differentiable rollout")
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(cfg)

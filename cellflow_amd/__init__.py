@@ -8,4 +8,5 @@ generators in :mod:`cellflow_amd.configs`.
 from ._abi import CellFlowError, LIB_PATH  # noqa: F401
 from .simulation import (PARTICLE_DTYPE, Ensemble, ParticleSimulation,  # noqa: F401
                          SimulationParams, cutoff_threshold, device_count,
-                         force_table_transform, grid_dims, particles_from_state)
+                         force_table_transform, grid_dims, particles_from_state,
+                         tuning_cache_lookup, tuning_cache_store)

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libcellflow.so")
 CF_OK, CF_ERR_ARG, CF_ERR_DEVICE, CF_ERR_CAPACITY, CF_ERR_NONFINITE, CF_ERR_HANDLE, \
     CF_ERR_STATE = range(7)
 CF_MAX_TYPES = 10
+CF_TUNE_FORCE = 1
 CF_MAX_CONNECTIONS = 32
 
 ERROR_NAMES = {1: "CF_ERR_ARG", 2: "CF_ERR_DEVICE", 3: "CF_ERR_CAPACITY",
@@ -98,6 +99,11 @@ SIGNATURES = {
                              P(C.c_double), P(C.c_double), P(C.c_double)]),
     "cf_set_tuning": (_i, [_vp, P(cf_tuning)]),
     "cf_get_tuning": (_i, [_vp, P(cf_tuning)]),
+    "cf_autotune": (_i, [_vp, P(cf_params), P(cf_edge_opts), _i, C.c_char_p, _i, P(cf_tuning),
+                         P(C.c_double), C.c_char_p, _i]),
+    "cf_tuning_key": (_i, [_vp, P(cf_params), P(cf_edge_opts), C.c_char_p, _i]),
+    "cf_tuning_cache_lookup": (_i, [C.c_char_p, C.c_char_p, P(cf_tuning), P(C.c_double), P(_i)]),
+    "cf_tuning_cache_store": (_i, [C.c_char_p, C.c_char_p, P(cf_tuning), C.c_double]),
     "cf_grid_dims": (_i, [P(cf_params), P(_f), _i, _i, P(_i)]),
     "cf_cutoff_threshold": (_f, [_f]),
     "cf_force_table_transform": (None, [P(_f), _i, _f, _f, _f, P(_f)]),
@@ -111,6 +117,9 @@ SIGNATURES = {
     "cf_ensemble_get_particles": (_i, [_vp, _i, P(cf_particle), _i]),
     "cf_ensemble_get_neighbor_counts": (_i, [_vp, _i, P(C.c_int32), _i]),
     "cf_ensemble_set_tuning": (_i, [_vp, P(cf_tuning)]),
+    "cf_ensemble_get_tuning": (_i, [_vp, P(cf_tuning)]),
+    "cf_ensemble_autotune": (_i, [_vp, _i, C.c_char_p, _i, P(cf_tuning), P(C.c_double),
+                                  C.c_char_p, _i]),
 }
 
 _lib = None

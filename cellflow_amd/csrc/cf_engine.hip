@@ -8,12 +8,14 @@
 
 #include "../../include/cellflow.h"
 #include "cf_internal.h"
+#include "cf_tune_cache.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -1157,7 +1159,236 @@ struct Engine {
     cleanup();
     return rc;
   }
+
+  // ---- autotuner -----------------------------------------------------------------------
+
+  // Bucket key of the current workload: device arch, log2 of the particle count (all
+  // simulations), half-octaves of the mean occupancy a particle sees in its own cell of the
+  // divisor-1 grid (sum of occupancy^2 over cells / particles; this tracks clustering, which
+  // a mean density would miss), the largest type count, the edge cap and log2 of the number
+  // of simulations.  Leaves the state binned for the divisor-1 grid; the next step re-sorts
+  // it for the tuning in force (the canonical order depends on positions only).
+  int tune_key(const EdgeReq& er, std::string* key) {
+    for (const SimHost& h : sims)
+      if (!h.have_prm) return fail(CF_ERR_STATE, "parameters not set for every simulation");
+    const cf_tuning saved = tun;
+    tun.cell_divisor = 1;
+    int rc = prepare(er);
+    tun = saved;
+    CFR(rc);
+    std::vector<uint32_t> starts(C + 1);
+    HIPR(hipMemcpyAsync(starts.data(), b.start[cur], (C + 1) * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    double sq = 0.0, ntot = 0.0;
+    int tmax = 1;
+    for (const SimHost& h : sims) {
+      const long nc = (long)h.dims[0] * h.dims[1] * h.dims[2];
+      for (long c = h.c0; c < h.c0 + nc; ++c) {
+        const double occ = (double)starts[c + 1] - (double)starts[c];
+        sq += occ * occ;
+      }
+      ntot += h.n;
+      tmax = std::max(tmax, h.T);
+    }
+    const double seen = sq / std::max(1.0, ntot);
+    const int nb = (int)std::floor(std::log2(std::max(1.0, ntot)));
+    const int rb = (int)std::floor(2.0 * std::log2(std::max(seen, 1.0 / 1024.0)));
+    const int sb = (int)std::floor(std::log2((double)sims.size()));
+    const int eb = !er.on ? 0 : (er.maxc <= 16 ? 16 : 32);
+    std::string arch = "unknown";
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+      arch = prop.gcnArchName;
+      const size_t colon = arch.find(':');
+      if (colon != std::string::npos) arch.resize(colon);
+    }
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s/sims=%d/n=%d/rho=%d/T=%d/e=%d", arch.c_str(), sb, nb, rb,
+                  tmax, eb);
+    *key = buf;
+    return CF_OK;
+  }
+
+  // State of the call (binned for `orig`), restored before every trial and at the end.
+  int restore(const StateBuf& snap, const cf_tuning& orig, const EdgeReq& er) {
+    tun = orig;
+    bool changed = false;
+    CFR(build_descs(er, &changed));
+    return load_state(snap);
+  }
+
+  // Average microseconds per step over `nsteps` steps as cf_simulate runs them after its
+  // first step (graph replays where the tuning asks for them), timed with HIP events on
+  // the engine's stream.
+  int time_steps(const EdgeReq& er, int nsteps, double* us) {
+    const bool fuse = fusable(er);
+    const int cap = 2 * er.maxc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPR(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return fail(CF_ERR_DEVICE, "hipEventCreate failed");
+    }
+    auto body = [&]() -> int {
+      HIPR(hipEventRecord(e0, st));
+      if (er.on && !fuse) {
+        for (int k = 0; k < nsteps; ++k) {
+          CFR(step(false, cap));
+          CFR(edge_pass(cap));
+        }
+      } else {
+        CFR(run_steps(nsteps, er.on, cap));
+      }
+      HIPR(hipEventRecord(e1, st));
+      HIPR(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIPR(hipEventElapsedTime(&ms, e0, e1));
+      *us = 1e3 * (double)ms / nsteps;
+      return CF_OK;
+    };
+    const int rc = body();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+  }
+
+  struct Trial {
+    cf_tuning t;
+    double us;
+    bool pruned;
+  };
+
+  // Times every candidate tuning and leaves the fastest in force.  Each candidate first runs
+  // one directly launched probe step; one whose probe takes over three times the fastest
+  // probe so far is not timed further (its probe time is reported).
+  int autotune(const EdgeReq& er, int trial_steps, cf_tuning* best_out, double* best_us,
+               std::vector<Trial>* trials) {
+    for (const SimHost& h : sims)
+      if (!h.have_prm) return fail(CF_ERR_STATE, "parameters not set for every simulation");
+    if (er.on) CFR(ensure_edge_bufs(er.maxc));
+    const cf_tuning orig = tun;
+    CFR(prepare(er));
+    std::vector<cf_tuning> cands;
+    for (int d = 1; d <= 2; ++d) {
+      for (int g : {16, 8, 4, 32}) cands.push_back(cf_tuning{d, 0, g, orig.block_size,
+                                                             orig.steps_per_graph});
+      for (int bs : {64, 128, 256}) cands.push_back(cf_tuning{d, 1, orig.lanes_per_particle, bs,
+                                                              orig.steps_per_graph});
+      cands.push_back(cf_tuning{d, 2, orig.lanes_per_particle, orig.block_size,
+                                orig.steps_per_graph});
+    }
+    const int span = orig.steps_per_graph >= 2 ? (orig.steps_per_graph + 1) / 2 * 2 : 2;
+    const int nsteps = std::max(span, (std::max(1, trial_steps) + span - 1) / span * span);
+    StateBuf snap;
+    float4* acc_snap = nullptr;  // the reported acceleration is state too (trials rewrite it)
+    auto body = [&]() -> int {
+      CFR(alloc_state(snap));
+      CFR(save_state(snap));
+      CFR(dalloc(&acc_snap, P));
+      HIPR(hipMemcpyAsync(acc_snap, b.acc, P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+      double best = -1.0, min_probe = -1.0;
+      cf_tuning bt = orig;
+      for (const cf_tuning& c : cands) {
+        CFR(restore(snap, orig, er));
+        tun = c;
+        CFR(prepare(er));
+        HIPR(hipMemsetAsync(b.flag, 0, sizeof(uint32_t), st));
+        CFR(step(false, 2 * er.maxc));  // the first step of a call builds no graph
+        double probe = 0.0, us = 0.0;
+        CFR(time_steps(er, 1, &probe));
+        if (min_probe > 0.0 && probe > 3.0 * min_probe) {
+          trials->push_back(Trial{c, probe, true});
+          continue;
+        }
+        min_probe = (min_probe < 0.0) ? probe : std::min(min_probe, probe);
+        CFR(time_steps(er, span, &us));  // captures the step graph
+        CFR(time_steps(er, nsteps, &us));
+        trials->push_back(Trial{c, us, false});
+        if (best < 0.0 || us < best) {
+          best = us;
+          bt = c;
+        }
+      }
+      CFR(restore(snap, orig, er));
+      HIPR(hipMemcpyAsync(b.acc, acc_snap, P * sizeof(float4), hipMemcpyDeviceToDevice, st));
+      HIPR(hipMemsetAsync(b.flag, 0, sizeof(uint32_t), st));
+      HIPR(hipStreamSynchronize(st));
+      tun = bt;
+      *best_out = bt;
+      *best_us = best;
+      return CF_OK;
+    };
+    const int rc = body();
+    (void)hipStreamSynchronize(st);
+    if (rc != CF_OK && snap.pos) {
+      // leave the call's state in place even when a trial failed
+      const std::string msg = g_err;
+      if (restore(snap, orig, er) == CF_OK && acc_snap)
+        (void)hipMemcpyAsync(b.acc, acc_snap, P * sizeof(float4), hipMemcpyDeviceToDevice, st);
+      (void)hipStreamSynchronize(st);
+      g_err = msg;
+    }
+    if (rc != CF_OK) tun = orig;
+    free_state(snap);
+    dfree(acc_snap);
+    drop_graphs();
+    edges_count = -1;
+    return rc;
+  }
 };
+
+static bool check_tuning_values(const cf_tuning& t) { return tunecache::valid(t); }
+
+// Lookup-or-measure driver shared by cf_autotune and cf_ensemble_autotune.
+static int tune_call(Engine& e, const EdgeReq& er, int trial_steps, const char* path, int flags,
+                     cf_tuning* chosen, double* us_out, char* report, int report_cap) {
+  std::string key;
+  CFR(e.tune_key(er, &key));
+  std::string why;
+  std::map<std::string, tunecache::Entry> ents;
+  int st = tunecache::kMissing;
+  if (path) {
+    st = tunecache::read_file(path, ents, why);
+    if (st == tunecache::kBad)
+      return fail(CF_ERR_ARG, "tuning cache %s: %s", path, why.c_str());
+  }
+  std::string rep = "{\"key\": \"" + tunecache::escape(key) + "\", ";
+  cf_tuning t = e.tun;
+  double us = 0.0;
+  auto hit = ents.find(key);
+  if (path && !(flags & CF_TUNE_FORCE) && hit != ents.end() && check_tuning_values(hit->second.tun)) {
+    t = hit->second.tun;
+    t.steps_per_graph = e.tun.steps_per_graph;
+    e.tun = t;
+    us = hit->second.us;
+    rep += "\"cached\": true}";
+  } else {
+    std::vector<Engine::Trial> trials;
+    CFR(e.autotune(er, trial_steps, &t, &us, &trials));
+    rep += "\"cached\": false, \"trials\": [";
+    for (size_t i = 0; i < trials.size(); ++i) {
+      char b[256];
+      const cf_tuning& c = trials[i].t;
+      std::snprintf(b, sizeof b,
+                    "%s{\"cell_divisor\": %d, \"force_variant\": %d, \"lanes_per_particle\": %d, "
+                    "\"block_size\": %d, \"us_per_step\": %.3f, \"pruned\": %s}",
+                    i ? ", " : "", c.cell_divisor, c.force_variant, c.lanes_per_particle,
+                    c.block_size, trials[i].us, trials[i].pruned ? "true" : "false");
+      rep += b;
+    }
+    rep += "]}";
+    if (path) {
+      ents[key] = tunecache::Entry{t, us};
+      if (!tunecache::write_file(path, ents, why))
+        return fail(CF_ERR_ARG, "cannot write tuning cache %s: %s", path, why.c_str());
+    }
+  }
+  if (chosen) *chosen = t;
+  if (us_out) *us_out = us;
+  if (report && report_cap > 0) std::snprintf(report, report_cap, "%s", rep.c_str());
+  return CF_OK;
+}
 
 }  // namespace cf
 
@@ -1601,6 +1832,71 @@ int cf_get_tuning(cf_sim* sim, cf_tuning* tuning) {
   return CF_OK;
 }
 
+int cf_autotune(cf_sim* sim, const cf_params* params, const cf_edge_opts* edges,
+                int trial_steps, const char* cache_path, int flags, cf_tuning* chosen,
+                double* us_per_step, char* report, int report_cap) {
+  CFR(check(sim));
+  if (trial_steps < 1 || (!report && report_cap > 0))
+    return cf::fail(CF_ERR_ARG, "bad arguments");
+  CFR(check_params(params, sim->e->sims[0].T));
+  cf::EdgeReq er;
+  CFR(edge_req(edges, &er));
+  sim->e->sims[0].prm = *params;
+  sim->e->sims[0].have_prm = true;
+  return cf::tune_call(*sim->e, er, trial_steps, cache_path, flags, chosen, us_per_step, report,
+                       report_cap);
+}
+
+int cf_tuning_key(cf_sim* sim, const cf_params* params, const cf_edge_opts* edges, char* key,
+                  int key_cap) {
+  CFR(check(sim));
+  if (!key || key_cap < 1) return cf::fail(CF_ERR_ARG, "bad arguments");
+  CFR(check_params(params, sim->e->sims[0].T));
+  cf::EdgeReq er;
+  CFR(edge_req(edges, &er));
+  sim->e->sims[0].prm = *params;
+  sim->e->sims[0].have_prm = true;
+  std::string k;
+  CFR(sim->e->tune_key(er, &k));
+  if ((int)k.size() >= key_cap)
+    return cf::fail(CF_ERR_CAPACITY, "key needs %d bytes", (int)k.size() + 1);
+  std::snprintf(key, key_cap, "%s", k.c_str());
+  return CF_OK;
+}
+
+int cf_tuning_cache_lookup(const char* cache_path, const char* key, cf_tuning* tuning,
+                           double* us_per_step, int* found) {
+  if (!cache_path || !key || !tuning || !found) return cf::fail(CF_ERR_ARG, "null argument");
+  *found = 0;
+  std::map<std::string, cf::tunecache::Entry> ents;
+  std::string why;
+  const int st = cf::tunecache::read_file(cache_path, ents, why);
+  if (st == cf::tunecache::kBad)
+    return cf::fail(CF_ERR_ARG, "tuning cache %s: %s", cache_path, why.c_str());
+  auto it = ents.find(key);
+  if (it == ents.end() || !cf::check_tuning_values(it->second.tun)) return CF_OK;
+  *tuning = it->second.tun;
+  if (us_per_step) *us_per_step = it->second.us;
+  *found = 1;
+  return CF_OK;
+}
+
+int cf_tuning_cache_store(const char* cache_path, const char* key, const cf_tuning* tuning,
+                          double us_per_step) {
+  if (!cache_path || !key) return cf::fail(CF_ERR_ARG, "null argument");
+  CFR(check_tuning(tuning));
+  if (!std::isfinite(us_per_step)) return cf::fail(CF_ERR_ARG, "us_per_step must be finite");
+  std::map<std::string, cf::tunecache::Entry> ents;
+  std::string why;
+  const int st = cf::tunecache::read_file(cache_path, ents, why);
+  if (st == cf::tunecache::kBad)
+    return cf::fail(CF_ERR_ARG, "tuning cache %s: %s", cache_path, why.c_str());
+  ents[key] = cf::tunecache::Entry{*tuning, us_per_step};
+  if (!cf::tunecache::write_file(cache_path, ents, why))
+    return cf::fail(CF_ERR_ARG, "cannot write tuning cache %s: %s", cache_path, why.c_str());
+  return CF_OK;
+}
+
 int cf_pair_histogram(cf_sim* sim, int bins, float r_max, double* hist) {
   CFR(check(sim));
   if (!hist || bins < 1 || bins > cf::kMaxBins || !(r_max > 0.0f))
@@ -1688,6 +1984,23 @@ int cf_ensemble_set_tuning(cf_ensemble* ens, const cf_tuning* tuning) {
   CFR(check_tuning(tuning));
   ens->e->tun = *tuning;
   return CF_OK;
+}
+
+int cf_ensemble_get_tuning(cf_ensemble* ens, cf_tuning* tuning) {
+  CFR(check(ens));
+  if (!tuning) return cf::fail(CF_ERR_ARG, "tuning is null");
+  *tuning = ens->e->tun;
+  return CF_OK;
+}
+
+int cf_ensemble_autotune(cf_ensemble* ens, int trial_steps, const char* cache_path, int flags,
+                         cf_tuning* chosen, double* us_per_step, char* report, int report_cap) {
+  CFR(check(ens));
+  if (trial_steps < 1 || (!report && report_cap > 0))
+    return cf::fail(CF_ERR_ARG, "bad arguments");
+  cf::EdgeReq none;
+  return cf::tune_call(*ens->e, none, trial_steps, cache_path, flags, chosen, us_per_step, report,
+                       report_cap);
 }
 
 }  // extern "C"

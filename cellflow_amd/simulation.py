@@ -10,6 +10,8 @@ Arrays are numpy (host memory); device memory stays inside the engine.
 from __future__ import annotations
 
 import ctypes as C
+import json
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -307,9 +309,33 @@ class ParticleSimulation:
     def getTuning(self):
         t = _abi.cf_tuning()
         check(lib().cf_get_tuning(self._h, C.byref(t)))
-        return dict(cell_divisor=t.cell_divisor, force_variant=t.force_variant,
-                    lanes_per_particle=t.lanes_per_particle, block_size=t.block_size,
-                    steps_per_graph=t.steps_per_graph)
+        return _tuning_dict(t)
+
+    def autotune(self, params: SimulationParams, edges=None, trial_steps: int = 32, cache=None,
+                 force: bool = False) -> dict:
+        """Measure every force-kernel tuning on a copy of the current state and apply the
+        fastest (the state is left unchanged).  `cache` is a JSON file path: a hit skips the
+        measurement, a miss is stored.  Returns {"tuning", "us_per_step", "key", "cached",
+        "trials"}."""
+        p = params.to_c()
+        eo = None if edges is None else _abi.cf_edge_opts(float(edges[0]), int(edges[1]))
+        t = _abi.cf_tuning()
+        us = C.c_double()
+        rep = C.create_string_buffer(16384)
+        check(lib().cf_autotune(self._h, C.byref(p), None if eo is None else C.byref(eo),
+                                int(trial_steps), _path(cache),
+                                _abi.CF_TUNE_FORCE if force else 0, C.byref(t), C.byref(us),
+                                rep, len(rep)))
+        return _tune_result(t, us, rep)
+
+    def tuningKey(self, params: SimulationParams, edges=None) -> str:
+        """The workload bucket key under which :meth:`autotune` caches its result."""
+        p = params.to_c()
+        eo = None if edges is None else _abi.cf_edge_opts(float(edges[0]), int(edges[1]))
+        buf = C.create_string_buffer(256)
+        check(lib().cf_tuning_key(self._h, C.byref(p), None if eo is None else C.byref(eo), buf,
+                                  len(buf)))
+        return buf.value.decode()
 
 
 class Ensemble:
@@ -357,6 +383,21 @@ class Ensemble:
                            steps_per_graph)
         check(lib().cf_ensemble_set_tuning(self._h, C.byref(t)))
 
+    def getTuning(self):
+        t = _abi.cf_tuning()
+        check(lib().cf_ensemble_get_tuning(self._h, C.byref(t)))
+        return _tuning_dict(t)
+
+    def autotune(self, trial_steps: int = 32, cache=None, force: bool = False) -> dict:
+        """:meth:`ParticleSimulation.autotune` for the whole ensemble."""
+        t = _abi.cf_tuning()
+        us = C.c_double()
+        rep = C.create_string_buffer(16384)
+        check(lib().cf_ensemble_autotune(self._h, int(trial_steps), _path(cache),
+                                         _abi.CF_TUNE_FORCE if force else 0, C.byref(t),
+                                         C.byref(us), rep, len(rep)))
+        return _tune_result(t, us, rep)
+
     def close(self):
         if self._h:
             check(lib().cf_ensemble_destroy(self._h))
@@ -368,6 +409,40 @@ class Ensemble:
                 lib().cf_ensemble_destroy(self._h)
         except Exception:
             pass
+
+
+def _tuning_dict(t) -> dict:
+    return dict(cell_divisor=t.cell_divisor, force_variant=t.force_variant,
+                lanes_per_particle=t.lanes_per_particle, block_size=t.block_size,
+                steps_per_graph=t.steps_per_graph)
+
+
+def _path(p):
+    return None if p is None else os.fsencode(p)
+
+
+def _tune_result(t, us, rep) -> dict:
+    r = json.loads(rep.value.decode())
+    return dict(tuning=_tuning_dict(t), us_per_step=us.value, key=r["key"], cached=r["cached"],
+                trials=r.get("trials", []))
+
+
+def tuning_cache_lookup(path, key: str):
+    """(tuning dict, us_per_step) stored under `key` in the cache file, or None."""
+    t = _abi.cf_tuning()
+    us = C.c_double()
+    found = C.c_int()
+    check(lib().cf_tuning_cache_lookup(os.fsencode(path), key.encode(), C.byref(t), C.byref(us),
+                                       C.byref(found)))
+    return (_tuning_dict(t), us.value) if found.value else None
+
+
+def tuning_cache_store(path, key: str, tuning: dict, us_per_step: float) -> None:
+    t = _abi.cf_tuning(tuning["cell_divisor"], tuning["force_variant"],
+                       tuning["lanes_per_particle"], tuning["block_size"],
+                       tuning["steps_per_graph"])
+    check(lib().cf_tuning_cache_store(os.fsencode(path), key.encode(), C.byref(t),
+                                      float(us_per_step)))
 
 
 def particles_from_state(pos, vel, type_) -> np.ndarray:

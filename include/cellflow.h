@@ -205,6 +205,33 @@ int cf_rollout_grad(cf_sim* sim, const cf_params* params, int nsteps, int checkp
 int cf_set_tuning(cf_sim* sim, const cf_tuning* tuning);
 int cf_get_tuning(cf_sim* sim, cf_tuning* tuning);
 
+/* Autotuner (no reference counterpart).  Picks the fastest cell divisor, force-kernel
+ * variant, lanes per particle and block size for this simulation's workload and applies
+ * it (steps_per_graph is kept).  Each candidate runs `trial_steps` timed steps (HIP events,
+ * whole step) on a copy of the current state; the state and neighbour counts are restored
+ * afterwards, so tuning never changes a trajectory.  An edge list built before the call
+ * is discarded.
+ *
+ * With a non-null `cache_path` the result is looked up first under the workload's bucket
+ * key (device arch, log2 particle count, half-octave density, types, edge cap; see
+ * cf_tuning_key) and stored there after measuring; flags CF_TUNE_FORCE re-measures even
+ * on a hit.  `report` (optional, report_cap bytes) receives a JSON object with the key,
+ * whether the cache answered and the per-candidate microseconds per step. */
+#define CF_TUNE_FORCE 1
+int cf_autotune(cf_sim* sim, const cf_params* params, const cf_edge_opts* edges,
+                int trial_steps, const char* cache_path, int flags, cf_tuning* chosen,
+                double* us_per_step, char* report, int report_cap);
+/* The cache key cf_autotune uses for this workload. */
+int cf_tuning_key(cf_sim* sim, const cf_params* params, const cf_edge_opts* edges, char* key,
+                  int key_cap);
+/* Host-only access to a tuning cache file.  Lookup sets *found (0/1); a missing file is
+ * an empty cache; a file that is not a tuning cache is CF_ERR_ARG.  Store adds or replaces
+ * one entry (written through a temporary file and a rename). */
+int cf_tuning_cache_lookup(const char* cache_path, const char* key, cf_tuning* tuning,
+                           double* us_per_step, int* found);
+int cf_tuning_cache_store(const char* cache_path, const char* key, const cf_tuning* tuning,
+                          double us_per_step);
+
 /* ---- host-only helpers (no GPU needed) ---------------------------------------------- */
 
 /* The grid rule: cells per axis for a parameter set, type radii and particle count. */
@@ -233,6 +260,10 @@ int cf_ensemble_simulate(cf_ensemble* ens, int nsteps);
 int cf_ensemble_get_particles(cf_ensemble* ens, int sim, cf_particle* particles, int count);
 int cf_ensemble_get_neighbor_counts(cf_ensemble* ens, int sim, int32_t* counts, int count);
 int cf_ensemble_set_tuning(cf_ensemble* ens, const cf_tuning* tuning);
+int cf_ensemble_get_tuning(cf_ensemble* ens, cf_tuning* tuning);
+/* cf_autotune for the whole ensemble (every member with its own parameters). */
+int cf_ensemble_autotune(cf_ensemble* ens, int trial_steps, const char* cache_path, int flags,
+                         cf_tuning* chosen, double* us_per_step, char* report, int report_cap);
 
 #ifdef __cplusplus
 }

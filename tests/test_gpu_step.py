@@ -234,13 +234,14 @@ def test_edge_capacity_error():
     assert rc == _abi.CF_ERR_CAPACITY and cnt.value > 1
 
 
-VARIANTS = [(0, 16, 1), (0, 4, 1), (0, 8, 2), (0, 32, 1), (0, 32, 2), (1, 16, 1), (1, 16, 2),
-            (2, 16, 1), (2, 16, 2)]
+# every combination the autotuner times: (variant, lanes, divisor, block size)
+VARIANTS = [(0, g, d, 128) for d in (1, 2) for g in (4, 8, 16, 32)] + \
+    [(1, 16, d, b) for d in (1, 2) for b in (64, 128, 256)] + [(2, 16, 1, 128), (2, 16, 2, 128)]
 
 
-@pytest.mark.parametrize("variant,lanes,divisor", VARIANTS)
+@pytest.mark.parametrize("variant,lanes,divisor,block", VARIANTS)
 @pytest.mark.parametrize("name", ["uniform_3k", "small_grid_generic", "dense_2k"])
-def test_variants_match_oracle(orc, variant, lanes, divisor, name):
+def test_variants_match_oracle(orc, variant, lanes, divisor, block, name):
     """Every force-kernel variant, lane grouping and stencil reach gives the oracle's counts,
     forces and edges."""
     cfg = small_cfg(**CASES[name])
@@ -249,7 +250,7 @@ def test_variants_match_oracle(orc, variant, lanes, divisor, name):
     st["prev"] = rng.integers(0, 600, len(st["type"])).astype(np.int32)
     sim = make_sim(st, phys)
     sim.setTuning(cell_divisor=divisor, force_variant=variant, lanes_per_particle=lanes,
-                  block_size=128)
+                  block_size=block)
     sim.simulate(params, 1, edges=(0.02, 6))
     ref = orc.step(st["pos"], st["vel"], st["type"], st["prev"], phys, st["table"], st["rbt"], mode=0)
     assert np.array_equal(sim.getNeighborCounts(), ref["count"])
