@@ -77,6 +77,7 @@ SIGNATURES = {
     "cf_set_neighbor_counts": (_i, [_vp, P(C.c_int32), _i]),
     "cf_get_neighbor_counts": (_i, [_vp, P(C.c_int32), _i]),
     "cf_init_particles": (_i, [_vp, _u64, P(_f), _f]),
+    "cf_move_universe": (_i, [_vp, _f, _f, _f, P(_f)]),
     "cf_set_force_table": (_i, [_vp, P(_f), _i]),
     "cf_get_force_table": (_i, [_vp, P(_f), _i]),
     "cf_set_raw_force_table": (_i, [_vp, P(_f), _i]),

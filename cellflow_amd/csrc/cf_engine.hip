@@ -809,6 +809,21 @@ struct Engine {
     return CF_OK;
   }
 
+  // Shift one simulation's positions with the reference's wrap and re-bin.  The state is
+  // loaded into the temporaries (slots p0..p0+n-1 hold the simulation's particles), moved
+  // there and sorted again by the next flush.
+  int move(int s, float dx, float dy, float dz, const float box[3]) {
+    CFR(flush());
+    CFR(begin_load());
+    IoArgs a = io_args(s);
+    a.shift[0] = dx;
+    a.shift[1] = dy;
+    a.shift[2] = dz;
+    for (int k = 0; k < 3; ++k) a.box[k] = box[k];
+    HIPR(launch_io(5, a, sims[s].n, st));
+    return flush();
+  }
+
   int init_particles(int s, uint64_t seed, const float canvas[3], float spawn) {
     CFR(begin_load());
     // descriptors must carry this simulation's T before the init kernel runs
@@ -1657,6 +1672,17 @@ int cf_init_particles(cf_sim* sim, uint64_t seed, const float canvas[3], float s
     if (!(canvas[a] > 0.0f) || !std::isfinite(canvas[a])) return cf::fail(CF_ERR_ARG, "bad canvas");
   if (!(spawn_region > 0.0f)) return cf::fail(CF_ERR_ARG, "spawn_region must be > 0");
   return sim->e->init_particles(0, seed, canvas, spawn_region);
+}
+
+int cf_move_universe(cf_sim* sim, float dx, float dy, float dz, const float canvas[3]) {
+  CFR(check(sim));
+  if (!canvas) return cf::fail(CF_ERR_ARG, "canvas is null");
+  const float sh[3] = {dx, dy, dz};
+  for (int k = 0; k < 3; ++k) {
+    if (!(canvas[k] > 0.0f) || !std::isfinite(canvas[k]) || !std::isfinite(sh[k]))
+      return cf::fail(CF_ERR_ARG, "canvas must be finite and > 0, shifts finite");
+  }
+  return sim->e->move(0, dx, dy, dz, canvas);
 }
 
 int cf_set_force_table(cf_sim* sim, const float* table, int num_types) {

@@ -193,6 +193,8 @@ struct IoArgs {
   uint32_t* index_out;
   const SimDesc* sims;
   int sim;
+  float shift[3];          // universe shift (k_move)
+  float box[3];
 };
 
 struct InitArgs {

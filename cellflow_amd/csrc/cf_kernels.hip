@@ -1231,6 +1231,20 @@ __global__ void k_load_aos(IoArgs a) {
   a.pad[p] = r.pad;
 }
 
+// Universe shift of one simulation's loaded temporaries, moveParticlesKernel
+// (ParticleSimulation.cu:169-185): x = fmodf(x + dx + L, L) per axis in FP32, left to right.
+__global__ void k_move(IoArgs a) {
+  const SimDesc sd = a.sims[a.sim];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int p = sd.p0 + k;
+  float4 q = a.tpos[p];
+  q.x = fmodf(q.x + a.shift[0] + a.box[0], a.box[0]);
+  q.y = fmodf(q.y + a.shift[1] + a.box[1], a.box[1]);
+  q.z = fmodf(q.z + a.shift[2] + a.box[2], a.box[2]);
+  a.tpos[p] = q;
+}
+
 // Sorted state -> index-order AoS of one simulation.
 __global__ void k_unload_aos(IoArgs a) {
   const SimDesc sd = a.sims[a.sim];
@@ -1440,6 +1454,7 @@ hipError_t launch_io(int which, const IoArgs& a, int n, hipStream_t st) {
     case 2: hipLaunchKernelGGL(k_get_counts, dim3(g), dim3(256), 0, st, a); break;
     case 3: hipLaunchKernelGGL(k_set_counts, dim3(g), dim3(256), 0, st, a); break;
     case 4: hipLaunchKernelGGL(k_sort_order, dim3(g), dim3(256), 0, st, a); break;
+    case 5: hipLaunchKernelGGL(k_move, dim3(g), dim3(256), 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -161,6 +161,13 @@ class ParticleSimulation:
         cv = np.asarray(canvas, np.float32)
         check(lib().cf_init_particles(self._h, int(seed), _fp(cv), float(spawnRegionSize)))
 
+    def moveUniverse(self, dx: float, dy: float, dz: float = 0.0,
+                     canvas=(8000.0, 8000.0, 8000.0)):
+        """moveUniverse (ParticleSimulation.cu:585-592): x = fmodf(x + d + L, L) per axis.
+        The reference always uses a default SimulationParams' canvas (8000^3) and dz = 0."""
+        cv = np.asarray(canvas, np.float32)
+        check(lib().cf_move_universe(self._h, float(dx), float(dy), float(dz), _fp(cv)))
+
     # -- force table and radii (ParticleSimulation.cuh:24-57) --------------------------------
     def setForceTable(self, table):
         t = np.ascontiguousarray(table, np.float32)

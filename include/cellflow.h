@@ -130,6 +130,12 @@ int cf_get_neighbor_counts(cf_sim* sim, int32_t* counts, int count);
  * seed gives the same state on every device. */
 int cf_init_particles(cf_sim* sim, uint64_t seed, const float canvas[3], float spawn_region);
 
+/* moveUniverse (ParticleSimulation.cu:585-592, moveParticlesKernel :169-185): every
+ * position becomes fmodf(x + d + L, L) per axis in FP32 with L = canvas (the reference
+ * passes a default-constructed SimulationParams' 8000^3 canvas and dz = 0).  Velocities,
+ * counts and accelerations are kept; the state is re-binned. */
+int cf_move_universe(cf_sim* sim, float dx, float dy, float dz, const float canvas[3]);
+
 /* ---- force table and per-type radius modifiers -------------------------------------- */
 
 /* d_forceTable upload (.cu:529-530) with a caller-given final num_types^2 table

@@ -11,6 +11,7 @@ reference root):
 * :func:`rollout`       ParticleSimulation.cu:541-556 (count ping-pong over steps)
 * :func:`edges`         ParticleSimulation.cu:187-246
 * :func:`force_table`   ParticleSimulation.cu:521-531
+* :func:`move`          ParticleSimulation.cu:169-185
 * :func:`bin_cells`, :func:`order`   the engine's grid rule (no reference counterpart)
 """
 
@@ -65,6 +66,9 @@ def lib():
         L.orc_force_table.argtypes = [P(C.c_float), C.c_int, C.c_float, C.c_float, C.c_float,
                                       P(C.c_float)]
         L.orc_force_table.restype = None
+        L.orc_move.argtypes = [C.c_int, P(C.c_float), C.c_float, C.c_float, C.c_float,
+                               P(C.c_float)]
+        L.orc_move.restype = None
         L.orc_pair_in_range.argtypes = [P(OrcParams), P(C.c_float), P(C.c_float), P(C.c_float),
                                         C.c_uint, C.c_uint]
         L.orc_eff_radius.argtypes = [P(OrcParams), P(C.c_float), C.c_uint, C.c_uint]
@@ -174,6 +178,14 @@ def force_table(raw, rng, bias, offset):
     raw = np.ascontiguousarray(raw, np.float32).ravel()
     out = np.zeros_like(raw)
     lib().orc_force_table(_p(raw, C.c_float), len(raw), rng, bias, offset, _p(out, C.c_float))
+    return out
+
+
+def move(pos, dx, dy, dz, L):
+    """moveUniverse's wrap (ParticleSimulation.cu:169-185) on a copy of FP32 positions."""
+    out = np.array(pos, np.float32, order="C")
+    Lf = np.ascontiguousarray(L, np.float32)
+    lib().orc_move(len(out), _p(out, C.c_float), dx, dy, dz, _p(Lf, C.c_float))
     return out
 
 

@@ -19,6 +19,7 @@
 //                       2*maxC by index, stable sort by distance, keep maxC); output is the
 //                       (owner, rank) order instead of the reference's atomicAdd order
 //   orc_force_table     ParticleSimulation.cu:521-531 (updateForceTable transform)
+//   orc_move            ParticleSimulation.cu:169-185 (moveUniverse's wrap)
 //   orc_bin / orc_order the engine's own uniform-grid rule (no reference counterpart: the
 //                       reference enumerates all pairs, :86)
 //   orc_rollout         repeated orc_step with the neighbour-count ping-pong of :541-556
@@ -32,7 +33,12 @@
 // (two-particle force, isolated particle, balance=1, zero table), invariants (pair
 // symmetry, permutation and periodic-shift equivariance), brute force on tiny inputs
 // (all-pairs vs grid enumeration, edge truncation cases built by hand) and the four
-// reference preset files.
+// reference preset files.  The rollout helpers are pinned the same way: orc_hist by the
+// cubic B-spline's exact values at integer and half-integer offsets, its partition of
+// unity and the shell sums of a simple cubic lattice; orc_pairs by brute force and by
+// orc_step's counts; orc_frozen_rollout by equality with orc_rollout when given that
+// rollout's own decisions, and its finite differences by step-size convergence and exact
+// zeros for unused table entries.
 
 #include <algorithm>
 #include <cmath>
@@ -406,6 +412,14 @@ void orc_force_table(const float* raw, int count, float range, float bias, float
     float transformed = tanhf(raw[i] * offset) * range + bias;
     out[i] = fmaxf(-1.0f, fminf(1.0f, transformed));
   }
+}
+
+// moveParticlesKernel, ParticleSimulation.cu:169-185: x = fmodf(x + d + L, L) per axis, FP32,
+// evaluated left to right.
+void orc_move(int n, float* pos, float dx, float dy, float dz, const float* L) {
+  const float d[3] = {dx, dy, dz};
+  for (int i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) pos[3 * i + a] = fmodf(pos[3 * i + a] + d[a] + L[a], L[a]);
 }
 
 // ---- differentiable rollout (engine extension, no reference counterpart) ---------------

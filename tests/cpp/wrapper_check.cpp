@@ -1,0 +1,79 @@
+// Drives include/cellflow.hpp the way CellFlowWidget drives the reference class
+// (CellFlowWidget.cpp:410-427 step per frame, :551-575 proximity graph after the step,
+// :906-917 raw-table edits + updateForceTable, :893 setRadioByTypeValue) and writes the
+// inputs it ended up with and the outputs to <dir>, so a test can replay the same calls
+// through another binding and compare bit for bit.
+//
+// Exit status: 0 success; 10 + cellflow status on a cellflow::Error (12 = no device).
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "cellflow.hpp"
+
+namespace {
+
+template <class T>
+void dump(const std::string& path, const T* data, size_t count) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot write " + path);
+  std::fwrite(data, sizeof(T), count, f);
+  std::fclose(f);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const std::string dir = argc > 1 ? argv[1] : ".";
+  try {
+    const int n = 6000, frames = 12;
+    cellflow::ParticleSimulation sim(n, 0.6f, 0.6f);  // canvas 0.6^3; spawn cube = canvas
+    cellflow::SimulationParams p;
+    p.radius = 0.05f;
+    p.delta_t = 0.01f;
+    p.friction = 0.1f;
+    p.k = 1.0f;
+    p.canvasWidth = p.canvasHeight = p.canvasDepth = 0.6f;
+    p.ratioWithLFO = 0.3f;
+    // UI edits: one raw entry, then the transform; one radius modifier, then a rotation
+    sim.getRawForceTableValues()[3] = 0.5f;
+    sim.updateForceTable(0.6f, 0.1f, 1.5f);
+    sim.setRadioByTypeValue(2, 0.25f);
+    sim.setRadioByTypeValue(99, 1.0f);  // out of range: ignored, as in the reference
+    sim.rotateRadioByType();
+
+    std::vector<cellflow::Particle> init;
+    sim.getParticleData(init);
+    std::vector<float> table(36), raw(sim.getRawForceTableValues(),
+                                      sim.getRawForceTableValues() + 36);
+    cellflow::check(cf_get_force_table(sim.handle(), table.data(), 36));
+    const std::vector<float> rbt = sim.getRadioByType();
+
+    const std::vector<cellflow::ParticleColor> colors = {
+        {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}};
+    std::vector<float> verts;
+    int vc = 0;
+    for (int f = 0; f < frames; ++f) {
+      sim.simulate(p);
+      sim.generateProximityGraph(verts, vc, 0.02f, 16, colors);
+    }
+    std::vector<cellflow::Particle> fin;
+    sim.getParticleData(fin);
+    const std::vector<int32_t> counts = sim.getNeighborCounts();
+    const std::vector<int32_t> edges = sim.getEdges();
+
+    dump(dir + "/init.bin", init.data(), init.size());
+    dump(dir + "/table.bin", table.data(), table.size());
+    dump(dir + "/raw.bin", raw.data(), raw.size());
+    dump(dir + "/rbt.bin", rbt.data(), rbt.size());
+    dump(dir + "/final.bin", fin.data(), fin.size());
+    dump(dir + "/counts.bin", counts.data(), counts.size());
+    dump(dir + "/verts.bin", verts.data(), verts.size());
+    dump(dir + "/edges.bin", edges.data(), edges.size());
+    std::printf("ok n=%d frames=%d vertices=%d edges=%zu\n", n, frames, vc, edges.size() / 2);
+    return 0;
+  } catch (const cellflow::Error& e) {
+    std::printf("%s\n", e.what());
+    return 10 + e.code();
+  }
+}

@@ -211,6 +211,41 @@ def test_seeded_init_matches_generator():
     assert not p["vel"].any() and not p["acc"].any()
 
 
+def test_move_universe_matches_oracle(orc):
+    """moveUniverse: positions shift with the reference's FP32 wrap (bit-exact), velocities,
+    counts and accelerations stay, the binning is redone canonically and the next step
+    still matches the oracle."""
+    cfg = small_cfg(n=4000, side=0.5)
+    st, phys, params = state_of(cfg)
+    sim = make_sim(st, phys)
+    sim.simulate(params, 2)
+    before = sim.getParticleData()
+    counts = sim.getNeighborCounts()
+    sim.moveUniverse(0.137, -0.41, 0.05, canvas=phys["canvas"])
+    after = sim.getParticleData()
+    want = orc.move(before["pos"], 0.137, -0.41, 0.05, phys["canvas"])
+    assert np.array_equal(after["pos"], want)
+    for f in ("vel", "acc", "ptype", "pad"):
+        assert np.array_equal(after[f], before[f]), f
+    assert np.array_equal(sim.getNeighborCounts(), counts)
+    dims, _ = sim.getGrid()
+    cells, index = sim.getSortOrder()
+    ref_cells = orc.bin_cells(want, dims, phys["canvas"])
+    assert np.array_equal(index, orc.order(ref_cells))
+    ref = orc.step(want, after["vel"], after["ptype"], counts, phys, st["table"], st["rbt"],
+                   mode=0)
+    sim.simulate(params, 1)
+    assert np.array_equal(sim.getNeighborCounts(), ref["count"])
+    check_step(sim.getParticleData(), ref, phys, after["vel"])
+    # the reference's default: dz = 0 on the default 8000^3 canvas
+    big = ParticleSimulation(1000, 3)
+    big.initializeParticles(5)
+    p0 = big.getParticleData()["pos"]
+    big.moveUniverse(-2500.0, 7000.0)
+    assert np.array_equal(big.getParticleData()["pos"], orc.move(p0, -2500.0, 7000.0, 0.0,
+                                                                 [8000.0] * 3))
+
+
 def test_nonfinite_state_reported():
     cfg = small_cfg(n=500, side=0.5)
     st, phys, params = state_of(cfg)

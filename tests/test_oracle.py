@@ -352,3 +352,148 @@ def test_generator_ranges():
     for ph, s in ens:
         assert 2000 <= len(s["type"]) <= 50000 and 4 <= ph["num_types"] <= 8
         assert 0.03 <= ph["radius"] <= 0.06
+
+
+# ---- differentiable-rollout helpers (soft histogram, neighbour lists, frozen rollout) --------
+
+
+def test_hist_two_particles_known_spline_values(orc):
+    """Two particles at distance r: each sees the other once, so h_b = B3(r/w - 1/2 - b).
+    The cubic B-spline's values at integer and half-integer offsets are known exactly:
+    B3(0) = 2/3, B3(+-1) = 1/6, B3(+-2) = 0, B3(+-1/2) = 23/48, B3(+-3/2) = 1/48.
+    (r_max = 1/8 is exact in FP32, the precision the engine's ABI takes it in.)"""
+    bins, r_max = 16, 0.125
+    w = r_max / bins
+    L = [1.0, 1.0, 1.0]
+    for k, half in [(5, 0.0), (9, 0.5), (0, 0.0), (14, 0.5)]:
+        r = (k + 0.5 + half) * w
+        for pos in ([[0.2, 0.3, 0.4], [0.2 + r, 0.3, 0.4]],
+                    [[0.99, 0.5, 0.5], [0.99 + r - 1.0, 0.5, 0.5]]):  # across the boundary
+            h = orc.hist(np.array(pos), L, bins, r_max)
+            want = np.zeros(bins)
+            vals = {0: 2 / 3, 1: 1 / 6, -1: 1 / 6} if half == 0.0 else \
+                {0: 23 / 48, -1: 23 / 48, 1: 1 / 48, -2: 1 / 48}
+            for off, v in vals.items():
+                b = k + off if half == 0.0 else k + 1 + off
+                if 0 <= b < bins:
+                    want[b] = v
+            np.testing.assert_allclose(h, want, atol=1e-9, err_msg=f"k={k} half={half}")
+
+
+def test_hist_partition_of_unity_and_pair_count(orc):
+    """sum_b B3(u - b) = 1 whenever all four supporting bins exist (1 <= u <= bins - 2), so
+    a pair at such a distance adds exactly 2/N to the histogram's total; a pair beyond
+    r_max + 3w/2 (past the last bin's support) adds nothing."""
+    bins, r_max = 24, 0.12
+    w = r_max / bins
+    rng = np.random.default_rng(3)
+    for r in rng.uniform(1.5 * w, r_max - 1.5 * w, 20):
+        d = rng.normal(size=3)
+        d *= r / np.linalg.norm(d)
+        pos = np.array([[0.5, 0.5, 0.5], 0.5 + d, [0.5, 0.5, 0.85]])  # third one out of reach
+        h = orc.hist(pos, [1.0, 1.0, 1.0], bins, r_max)
+        assert abs(h.sum() - 2 / 3) < 1e-9, r
+    far = np.array([[0.5, 0.5, 0.5], [0.5, 0.5 + r_max + 1.6 * w, 0.5]])
+    assert not orc.hist(far, [1.0, 1.0, 1.0], bins, r_max).any()
+
+
+def test_hist_lattice_shells(orc):
+    """A simple cubic lattice of spacing a in a periodic box: every particle sees 6
+    neighbours at a, 12 at a*sqrt(2) and 8 at a*sqrt(3) inside r_max, so the histogram is
+    the shell sum, independent of how pairs are enumerated (periodic images included)."""
+    m, a = 10, 0.1
+    g = (np.arange(m) + 0.25) * a
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    bins, r_max = 20, 0.18
+    h = orc.hist(pos, [m * a] * 3, bins, r_max)
+    w = r_max / bins
+    two = orc.hist(np.array([[0.2, 0.2, 0.2], [0.2 + a, 0.2, 0.2]]), [1.0] * 3, bins, r_max)
+    two2 = orc.hist(np.array([[0.2, 0.2, 0.2], [0.2 + a, 0.2 + a, 0.2]]), [1.0] * 3, bins, r_max)
+    two3 = orc.hist(np.array([[0.2, 0.2, 0.2], [0.2 + a, 0.2 + a, 0.2 + a]]), [1.0] * 3, bins,
+                    r_max)
+    assert r_max + 2 * w < 2 * a  # the 2a shell is out of reach
+    np.testing.assert_allclose(h, 6 * two + 12 * two2 + 8 * two3, rtol=1e-9, atol=1e-12)
+
+
+def test_pairs_match_step_counts_and_brute_force(orc, c1_small):
+    st, ph = c1_small
+    pos32 = st["pos"].astype(np.float32)
+    offs, nbr = orc.pairs(pos32, st["type"], ph, st["rbt"])
+    out = one_step(orc, st["pos"], st["type"], st["table"], st["rbt"], ph)
+    np.testing.assert_array_equal(np.diff(offs), out["count"])
+    for i in range(0, len(pos32), 97):  # brute force on a sample of owners
+        want = [j for j in range(len(pos32)) if j != i and
+                orc.pair_in_range(ph, st["rbt"], pos32[i], pos32[j], int(st["type"][i]),
+                                  int(st["type"][j]))]
+        assert list(nbr[offs[i]:offs[i + 1]]) == want, i
+
+
+def _rollout_lists(orc, st, ph, steps):
+    pos, vel, prev = st["pos"].astype(np.float64), st["vel"].astype(np.float64), st["prev"]
+    lists = []
+    for _ in range(steps):
+        lists.append(orc.pairs(pos.astype(np.float32), st["type"], ph, st["rbt"]))
+        pos, vel, prev = orc.rollout(pos, vel, st["type"], prev, ph, st["table"], st["rbt"], 1,
+                                     mode=0)
+    return lists, pos
+
+
+def test_frozen_rollout_equals_rollout_on_its_own_decisions(orc):
+    """With the neighbour lists of the rollout's own trajectory, the frozen-decision rollout
+    is the ordinary FP64 rollout: same final histogram and loss."""
+    cfg = dict(configs.load("c5_adjoint_20k"), n=600, canvas=[0.31, 0.31, 0.31])
+    st, ph = configs.make_state(cfg), configs.physics(cfg)
+    lists, final = _rollout_lists(orc, st, ph, 6)
+    tgt = np.linspace(0.0, 2.0, 16)
+    fr = orc.FrozenRollout(st["pos"], st["vel"], st["type"], st["prev"], ph, st["rbt"], lists,
+                           16, 0.1, tgt)
+    loss, h = fr.loss(st["table"].astype(np.float64), float(np.float32(ph["ratio"])),
+                      want_hist=True)
+    href = orc.hist(final, ph["canvas"], 16, 0.1)
+    np.testing.assert_allclose(h, href, rtol=1e-11, atol=1e-13)
+    assert abs(loss - ((href - tgt) ** 2).sum()) < 1e-10 * max(1.0, loss)
+
+
+def test_frozen_fd_gradient_converges_and_unused_entries_are_zero(orc):
+    """Central differences at two step sizes agree (the frozen loss is smooth), and a table
+    entry that no interacting pair uses has exactly zero derivative."""
+    cfg = dict(configs.load("c5_adjoint_20k"), n=300, num_types=3, canvas=[0.25, 0.25, 0.25])
+    st, ph = configs.make_state(cfg), configs.physics(cfg)
+    st["type"] = np.where(st["type"] == 2, 1, st["type"]).astype(st["type"].dtype)
+    st["type"][0] = 2  # a single particle of type 2: the (2, 2) entry is never used
+    lists, _ = _rollout_lists(orc, st, ph, 4)
+    tgt = np.linspace(0.0, 2.0, 12)
+    fr = orc.FrozenRollout(st["pos"], st["vel"], st["type"], st["prev"], ph, st["rbt"], lists,
+                           12, 0.1, tgt)
+    tab = st["table"].astype(np.float64)
+    g1, r1 = fr.fd_gradient(tab, ph["ratio"], rel_step=1e-4)
+    g2, r2 = fr.fd_gradient(tab, ph["ratio"], rel_step=2e-5)
+    scale = max(np.abs(g1).max(), abs(r1))
+    assert np.abs(g1 - g2).max() < 1e-5 * scale and abs(r1 - r2) < 1e-5 * scale
+    uses = np.zeros((3, 3), bool)
+    for offs, nbr in lists:
+        own = np.repeat(np.arange(len(offs) - 1), np.diff(offs))
+        uses[st["type"][own], st["type"][nbr]] = True
+    assert not uses.all()
+    assert np.all(g1[~uses] == 0.0) and np.all(np.abs(g1[uses]) > 0)
+
+
+# ---- universe shift (moveUniverse) --------------------------------------------------------
+
+
+def test_move_dyadic_cases_and_inverse(orc):
+    """Dyadic inputs make x + d + L exact, so the wrap's answers are known by hand; shifting
+    back by -d restores such positions exactly; results stay in [0, L)."""
+    L = [1.0, 2.0, 4.0]
+    pos = np.array([[0.25, 1.5, 3.75], [0.75, 0.0, 0.5], [0.0, 1.75, 0.0]], np.float32)
+    got = orc.move(pos, 0.5, -0.75, 0.25, L)
+    want = np.array([[0.75, 0.75, 0.0], [0.25, 1.25, 0.75], [0.5, 1.0, 0.25]], np.float32)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(orc.move(got, -0.5, 0.75, -0.25, L), pos)
+    rng = np.random.default_rng(0)
+    x = (rng.random((5000, 3)) * np.array(L)).astype(np.float32)
+    y = orc.move(x, 123.4, -0.3, 3.9, L)
+    assert np.all(y >= 0) and np.all(y < np.array(L, np.float32))
+    # a zero shift of a position already in [0, L) gives fmodf(x + L, L): x up to rounding
+    z = orc.move(x, 0.0, 0.0, 0.0, L)
+    assert np.all(np.abs(z - x) <= np.spacing(np.float32(2 * 4.0)))
