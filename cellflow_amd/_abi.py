@@ -57,6 +57,24 @@ class cf_tuning(C.Structure):
                 ("steps_per_graph", C.c_int)]
 
 
+class cf_preset(C.Structure):
+    _fields_ = [("present", C.c_uint32), ("particle_count", C.c_int),
+                ("num_particle_types", C.c_int)] + \
+        [(k, C.c_float) for k in ("radius", "delta_t", "friction", "repulsion", "attraction", "k",
+                                  "balance", "force_multiplier", "force_range", "force_bias",
+                                  "force_offset", "ratio", "lfo_a", "lfo_s", "canvas_width",
+                                  "canvas_height", "canvas_depth", "spawn_region_size")] + \
+        [("num_colors", C.c_int), ("colors", cf_color * CF_MAX_TYPES), ("num_radio", C.c_int),
+         ("radio_by_type", C.c_float * CF_MAX_TYPES), ("num_raw", C.c_int),
+         ("raw_force_table", C.c_float * (CF_MAX_TYPES * CF_MAX_TYPES))]
+
+
+# presence bits of cf_preset, in declaration order of include/cellflow.h
+PRESET_KEYS = ["PARTICLE_COUNT", "numParticleTypes", "radius", "delta_t", "friction", "repulsion",
+               "attraction", "k", "balance", "forceMultiplier", "forceRange", "forceBias", "ratio",
+               "lfoA", "lfoS", "forceOffset", "canvasWidth", "canvasHeight", "canvasDepth",
+               "spawnRegionSize", "particleColors", "radioByType", "rawForceTable"]
+
 assert C.sizeof(cf_particle) == 44
 
 P = C.POINTER
@@ -105,6 +123,10 @@ SIGNATURES = {
     "cf_tuning_key": (_i, [_vp, P(cf_params), P(cf_edge_opts), C.c_char_p, _i]),
     "cf_tuning_cache_lookup": (_i, [C.c_char_p, C.c_char_p, P(cf_tuning), P(C.c_double), P(_i)]),
     "cf_tuning_cache_store": (_i, [C.c_char_p, C.c_char_p, P(cf_tuning), C.c_double]),
+    "cf_preset_init": (None, [P(cf_preset)]),
+    "cf_preset_read": (_i, [C.c_char_p, P(cf_preset)]),
+    "cf_preset_write": (_i, [C.c_char_p, P(cf_preset), C.c_char_p]),
+    "cf_preset_params": (_i, [P(cf_preset), P(cf_params)]),
     "cf_grid_dims": (_i, [P(cf_params), P(_f), _i, _i, P(_i)]),
     "cf_cutoff_threshold": (_f, [_f]),
     "cf_force_table_transform": (None, [P(_f), _i, _f, _f, _f, P(_f)]),

@@ -29,7 +29,9 @@ namespace cf {
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char* fmt, ...) {
+// Records the message of a failing call (read back by cf_last_error_string) and returns
+// `code`.  Also used by the host-only sources (cf_preset.cpp).
+int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);

@@ -28,253 +28,19 @@
 #include <unistd.h>
 
 #include "../../include/cellflow.h"
+#include "cf_json.h"
 
 namespace cf {
 namespace tunecache {
 
-struct JVal {
-  enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
-  bool b = false;
-  double num = 0.0;
-  std::string str;
-  std::vector<JVal> arr;
-  std::vector<std::pair<std::string, JVal>> obj;
-
-  const JVal* get(const char* k) const {
-    if (kind != Obj) return nullptr;
-    for (const auto& kv : obj)
-      if (kv.first == k) return &kv.second;
-    return nullptr;
-  }
-};
-
-// Recursive-descent reader for RFC 8259 JSON; nesting is limited to 32 levels.
-class Reader {
- public:
-  Reader(const char* s, size_t n) : p_(s), end_(s + n) {}
-
-  bool parse(JVal& out) {
-    ws();
-    if (!value(out, 0)) return false;
-    ws();
-    return p_ == end_;
-  }
-
- private:
-  const char* p_;
-  const char* end_;
-
-  void ws() {
-    while (p_ < end_ && (*p_ == ' ' || *p_ == '\t' || *p_ == '\n' || *p_ == '\r')) ++p_;
-  }
-
-  bool lit(const char* w) {
-    const size_t n = std::strlen(w);
-    if ((size_t)(end_ - p_) < n || std::strncmp(p_, w, n) != 0) return false;
-    p_ += n;
-    return true;
-  }
-
-  static void utf8(std::string& s, unsigned c) {
-    if (c < 0x80) {
-      s += (char)c;
-    } else if (c < 0x800) {
-      s += (char)(0xC0 | (c >> 6));
-      s += (char)(0x80 | (c & 0x3F));
-    } else if (c < 0x10000) {
-      s += (char)(0xE0 | (c >> 12));
-      s += (char)(0x80 | ((c >> 6) & 0x3F));
-      s += (char)(0x80 | (c & 0x3F));
-    } else {
-      s += (char)(0xF0 | (c >> 18));
-      s += (char)(0x80 | ((c >> 12) & 0x3F));
-      s += (char)(0x80 | ((c >> 6) & 0x3F));
-      s += (char)(0x80 | (c & 0x3F));
-    }
-  }
-
-  bool hex4(unsigned& v) {
-    if (end_ - p_ < 4) return false;
-    v = 0;
-    for (int i = 0; i < 4; ++i) {
-      const char c = *p_++;
-      v <<= 4;
-      if (c >= '0' && c <= '9') v |= (unsigned)(c - '0');
-      else if (c >= 'a' && c <= 'f') v |= (unsigned)(c - 'a' + 10);
-      else if (c >= 'A' && c <= 'F') v |= (unsigned)(c - 'A' + 10);
-      else return false;
-    }
-    return true;
-  }
-
-  bool string(std::string& s) {
-    if (p_ >= end_ || *p_ != '"') return false;
-    ++p_;
-    while (p_ < end_) {
-      const char c = *p_++;
-      if (c == '"') return true;
-      if ((unsigned char)c < 0x20) return false;
-      if (c != '\\') {
-        s += c;
-        continue;
-      }
-      if (p_ >= end_) return false;
-      const char e = *p_++;
-      switch (e) {
-        case '"': s += '"'; break;
-        case '\\': s += '\\'; break;
-        case '/': s += '/'; break;
-        case 'b': s += '\b'; break;
-        case 'f': s += '\f'; break;
-        case 'n': s += '\n'; break;
-        case 'r': s += '\r'; break;
-        case 't': s += '\t'; break;
-        case 'u': {
-          unsigned c1 = 0;
-          if (!hex4(c1)) return false;
-          if (c1 >= 0xD800 && c1 < 0xDC00) {  // surrogate pair
-            unsigned c2 = 0;
-            if (!lit("\\u") || !hex4(c2) || c2 < 0xDC00 || c2 >= 0xE000) return false;
-            c1 = 0x10000 + ((c1 - 0xD800) << 10) + (c2 - 0xDC00);
-          } else if (c1 >= 0xDC00 && c1 < 0xE000) {
-            return false;
-          }
-          utf8(s, c1);
-          break;
-        }
-        default: return false;
-      }
-    }
-    return false;
-  }
-
-  bool number(double& v) {
-    const char* q = p_;
-    if (q < end_ && *q == '-') ++q;
-    if (q >= end_) return false;
-    if (*q == '0') {
-      ++q;
-    } else if (*q >= '1' && *q <= '9') {
-      while (q < end_ && *q >= '0' && *q <= '9') ++q;
-    } else {
-      return false;
-    }
-    if (q < end_ && *q == '.') {
-      ++q;
-      if (q >= end_ || *q < '0' || *q > '9') return false;
-      while (q < end_ && *q >= '0' && *q <= '9') ++q;
-    }
-    if (q < end_ && (*q == 'e' || *q == 'E')) {
-      ++q;
-      if (q < end_ && (*q == '+' || *q == '-')) ++q;
-      if (q >= end_ || *q < '0' || *q > '9') return false;
-      while (q < end_ && *q >= '0' && *q <= '9') ++q;
-    }
-    const std::string text(p_, q);
-    v = std::strtod(text.c_str(), nullptr);
-    p_ = q;
-    return true;
-  }
-
-  bool value(JVal& v, int depth) {
-    if (depth > 32 || p_ >= end_) return false;
-    const char c = *p_;
-    if (c == '{') {
-      ++p_;
-      v.kind = JVal::Obj;
-      ws();
-      if (p_ < end_ && *p_ == '}') {
-        ++p_;
-        return true;
-      }
-      while (true) {
-        ws();
-        std::string k;
-        if (!string(k)) return false;
-        ws();
-        if (p_ >= end_ || *p_ != ':') return false;
-        ++p_;
-        ws();
-        JVal item;
-        if (!value(item, depth + 1)) return false;
-        v.obj.emplace_back(std::move(k), std::move(item));
-        ws();
-        if (p_ < end_ && *p_ == ',') {
-          ++p_;
-          continue;
-        }
-        if (p_ < end_ && *p_ == '}') {
-          ++p_;
-          return true;
-        }
-        return false;
-      }
-    }
-    if (c == '[') {
-      ++p_;
-      v.kind = JVal::Arr;
-      ws();
-      if (p_ < end_ && *p_ == ']') {
-        ++p_;
-        return true;
-      }
-      while (true) {
-        ws();
-        JVal item;
-        if (!value(item, depth + 1)) return false;
-        v.arr.push_back(std::move(item));
-        ws();
-        if (p_ < end_ && *p_ == ',') {
-          ++p_;
-          continue;
-        }
-        if (p_ < end_ && *p_ == ']') {
-          ++p_;
-          return true;
-        }
-        return false;
-      }
-    }
-    if (c == '"') {
-      v.kind = JVal::Str;
-      return string(v.str);
-    }
-    if (lit("true")) {
-      v.kind = JVal::Bool;
-      v.b = true;
-      return true;
-    }
-    if (lit("false")) {
-      v.kind = JVal::Bool;
-      return true;
-    }
-    if (lit("null")) return true;
-    v.kind = JVal::Num;
-    return number(v.num);
-  }
-};
+using json::JVal;
+using json::Reader;
+using json::escape;
 
 struct Entry {
   cf_tuning tun{};
   double us = 0.0;
 };
-
-static inline std::string escape(const std::string& s) {
-  std::string o;
-  for (unsigned char c : s) {
-    if (c == '"' || c == '\\') {
-      o += '\\';
-      o += (char)c;
-    } else if (c < 0x20) {
-      char b[8];
-      std::snprintf(b, sizeof b, "\\u%04x", c);
-      o += b;
-    } else {
-      o += (char)c;
-    }
-  }
-  return o;
-}
 
 static inline bool int_field(const JVal& e, const char* k, int* out) {
   const JVal* v = e.get(k);

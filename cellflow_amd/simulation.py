@@ -168,6 +168,72 @@ class ParticleSimulation:
         cv = np.asarray(canvas, np.float32)
         check(lib().cf_move_universe(self._h, float(dx), float(dy), float(dz), _fp(cv)))
 
+    # -- preset files (CellFlowWidget.cpp:1070-1269) ----------------------------------------
+    _PARAM_KEYS = {"radius": "radius", "delta_t": "delta_t", "friction": "friction",
+                   "repulsion": "repulsion", "attraction": "attraction", "k": "k",
+                   "balance": "balance", "forceMultiplier": "forceMultiplier",
+                   "canvasWidth": "canvasWidth", "canvasHeight": "canvasHeight",
+                   "canvasDepth": "canvasDepth"}
+
+    def loadPreset(self, path, params: "SimulationParams | None" = None, seed: int = 1):
+        """CellFlowWidget::loadPreset on this simulation, in the widget's order: particle
+        count, type count, step parameters, radius modifiers (setRadioByTypeValue for the
+        first min(len, T)), raw table entries (the first min(len, T*T)), then
+        updateForceTable(forceRange, forceBias, forceOffset).  Fields the file lacks keep
+        their values in `params` (default SimulationParams()).  When the count or type count
+        changes the particles are re-initialised from `seed` in the preset's canvas (the
+        reference re-initialises from a time seed).  Returns (params, preset dict)."""
+        from dataclasses import replace
+
+        from .presets import read_preset
+        pr = read_preset(path)
+        prm = replace(params) if params is not None else SimulationParams()
+        reinit = False
+        if "PARTICLE_COUNT" in pr and pr["PARTICLE_COUNT"] != self.getParticleCount():
+            self.setParticleCount(pr["PARTICLE_COUNT"])
+            reinit = True
+        if "numParticleTypes" in pr:
+            T = pr["numParticleTypes"]
+            self.setNumParticleTypes(T)  # raw table and radii restart at zero (the reference
+            reinit = True                 # draws random ones; the preset overwrites them)
+        T = self.getNumParticleTypes()
+        prm.numParticleTypes = T
+        for key, attr in self._PARAM_KEYS.items():
+            if key in pr:
+                setattr(prm, attr, pr[key])
+        if "ratio" in pr:
+            prm.ratioWithLFO = pr["ratio"]  # LFO phase 0
+        for i, v in enumerate(pr.get("radioByType", [])[:T]):
+            self.setRadioByTypeValue(i, float(v))
+        raw = self.getRawForceTableValues()
+        vals = pr.get("rawForceTable", np.zeros(0, np.float32))[: T * T]
+        raw[: len(vals)] = vals
+        self.setRawForceTableValues(raw)
+        self.updateForceTable(pr.get("forceRange", 0.28), pr.get("forceBias", -0.20),
+                              pr.get("forceOffset", 1.0))
+        if reinit:
+            self.initializeParticles(seed, (prm.canvasWidth, prm.canvasHeight, prm.canvasDepth),
+                                     pr.get("spawnRegionSize", 2000.0))
+        return prm, pr
+
+    def savePreset(self, path, params: "SimulationParams", preset: "dict | None" = None,
+                   template=None):
+        """CellFlowWidget::savePreset: the simulation's count, types, radius modifiers and
+        raw table, the step parameters of `params`, and the UI fields of `preset` (force
+        range/bias/offset, ratio, LFO, spawn region, colours).  With `template`, every other
+        member of that file (render and camera settings) is kept."""
+        from .presets import write_preset
+        d = dict(preset or {})
+        T = self.getNumParticleTypes()
+        d["PARTICLE_COUNT"] = self.getParticleCount()
+        d["numParticleTypes"] = T
+        for key, attr in self._PARAM_KEYS.items():
+            d[key] = getattr(params, attr)
+        d.setdefault("ratio", params.ratioWithLFO)
+        d["radioByType"] = self.getRadioByType()
+        d["rawForceTable"] = self.getRawForceTableValues()
+        write_preset(path, d, template)
+
     # -- force table and radii (ParticleSimulation.cuh:24-57) --------------------------------
     def setForceTable(self, table):
         t = np.ascontiguousarray(table, np.float32)

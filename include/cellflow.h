@@ -238,6 +238,68 @@ int cf_tuning_cache_lookup(const char* cache_path, const char* key, cf_tuning* t
 int cf_tuning_cache_store(const char* cache_path, const char* key, const cf_tuning* tuning,
                           double us_per_step);
 
+/* ---- preset files (host only) -------------------------------------------------------- */
+
+/* The reference UI's preset JSON (CellFlowWidget::loadPreset / savePreset,
+ * CellFlowWidget.cpp:1070-1269): the fields that feed the step path.  `present` has one
+ * CF_PRESET_* bit per field the file holds; fields without their bit keep the defaults of
+ * SimulationParams.h:15-35 (particle_count 0).  Arrays keep at most CF_MAX_TYPES colours,
+ * CF_MAX_TYPES radius modifiers and CF_MAX_TYPES^2 raw table values, with their counts. */
+#define CF_PRESET_PARTICLE_COUNT (1u << 0)
+#define CF_PRESET_NUM_TYPES (1u << 1)
+#define CF_PRESET_RADIUS (1u << 2)
+#define CF_PRESET_DELTA_T (1u << 3)
+#define CF_PRESET_FRICTION (1u << 4)
+#define CF_PRESET_REPULSION (1u << 5)
+#define CF_PRESET_ATTRACTION (1u << 6)
+#define CF_PRESET_K (1u << 7)
+#define CF_PRESET_BALANCE (1u << 8)
+#define CF_PRESET_FORCE_MULTIPLIER (1u << 9)
+#define CF_PRESET_FORCE_RANGE (1u << 10)
+#define CF_PRESET_FORCE_BIAS (1u << 11)
+#define CF_PRESET_RATIO (1u << 12)
+#define CF_PRESET_LFO_A (1u << 13)
+#define CF_PRESET_LFO_S (1u << 14)
+#define CF_PRESET_FORCE_OFFSET (1u << 15)
+#define CF_PRESET_CANVAS_WIDTH (1u << 16)
+#define CF_PRESET_CANVAS_HEIGHT (1u << 17)
+#define CF_PRESET_CANVAS_DEPTH (1u << 18)
+#define CF_PRESET_SPAWN_REGION (1u << 19)
+#define CF_PRESET_COLORS (1u << 20)
+#define CF_PRESET_RADIO_BY_TYPE (1u << 21)
+#define CF_PRESET_RAW_FORCE_TABLE (1u << 22)
+#define CF_PRESET_ALL ((1u << 23) - 1u)
+
+typedef struct cf_preset {
+  uint32_t present;
+  int particle_count;           /* "PARTICLE_COUNT" */
+  int num_particle_types;       /* "numParticleTypes" */
+  float radius, delta_t, friction, repulsion, attraction, k, balance, force_multiplier;
+  float force_range, force_bias, force_offset;       /* updateForceTable arguments */
+  float ratio, lfo_a, lfo_s;    /* ratioWithLFO = ratio + lfoA*sin(2*pi*lfoS*t), :414-421 */
+  float canvas_width, canvas_height, canvas_depth, spawn_region_size;
+  int num_colors;               /* "particleColors": [{"r","g","b"}, ...] */
+  cf_color colors[CF_MAX_TYPES];
+  int num_radio;                /* "radioByType" */
+  float radio_by_type[CF_MAX_TYPES];
+  int num_raw;                  /* "rawForceTable", row-major T*T */
+  float raw_force_table[CF_MAX_TYPES * CF_MAX_TYPES];
+} cf_preset;
+
+/* Defaults (present = 0). */
+void cf_preset_init(cf_preset* preset);
+/* Parse a preset file.  Stricter than QJsonValue::toDouble/toInt, which turn a wrong type
+ * into 0: a field of the wrong type (or a non-integral count) is CF_ERR_ARG. */
+int cf_preset_read(const char* path, cf_preset* preset);
+/* Write the present fields in the layout Qt's QJsonDocument::toJson gives the reference's
+ * files (sorted keys, 4-space indent, shortest round-trip numbers).  With a non-null
+ * `template_path`, every other member of that file's object (render and camera settings)
+ * is kept, so load + save of an unchanged preset reproduces the file byte for byte. */
+int cf_preset_write(const char* path, const cf_preset* preset, const char* template_path);
+/* The step parameters of a preset at LFO phase 0 (ratio_with_lfo = ratio, 400 expected
+ * neighbours, SimulationParams.h:29). */
+int cf_preset_params(const cf_preset* preset, cf_params* params);
+
 /* ---- host-only helpers (no GPU needed) ---------------------------------------------- */
 
 /* The grid rule: cells per axis for a parameter set, type radii and particle count. */

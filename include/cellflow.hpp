@@ -115,6 +115,7 @@ class ParticleSimulation {
       types_ = o.types_;
       seed_ = o.seed_;
       for (int a = 0; a < 3; ++a) canvas_[a] = o.canvas_[a];
+      spawn_ = o.spawn_;
       for (int i = 0; i < MAX_PARTICLE_TYPES * MAX_PARTICLE_TYPES; ++i) raw_[i] = o.raw_[i];
     }
     return *this;
@@ -122,7 +123,7 @@ class ParticleSimulation {
 
   // initializeParticles(), .cu:484-492: spawn cube min(2000, canvas), centred.
   void initializeParticles() {
-    check(cf_init_particles(h_, seed_++, canvas_, 2000.0f));
+    check(cf_init_particles(h_, seed_++, canvas_, spawn_));
   }
   // initializeParticles(w, h), .cu:494-505.
   void initializeParticles(float canvasWidth, float canvasHeight) {
@@ -250,6 +251,99 @@ class ParticleSimulation {
     return c;
   }
 
+  // CellFlowWidget::loadPreset (CellFlowWidget.cpp:1070-1203) on this simulation, in the
+  // widget's order: count (re-allocates), types (new random table and radii, as
+  // setNumParticleTypes does), the step fields into `params`, radius modifiers for the
+  // first min(len, T), raw table entries for the first min(len, T*T), then
+  // updateForceTable(forceRange, forceBias, forceOffset).  Fields the file lacks keep their
+  // values in `params`.  After a count or type change the particles are re-initialised in
+  // the preset's canvas and spawn region.  Returns the preset (colours, LFO settings, ...).
+  cf_preset loadPreset(const char* path, SimulationParams& params) {
+    cf_preset pr;
+    check(cf_preset_read(path, &pr));
+    bool reinit = false;
+    if ((pr.present & CF_PRESET_PARTICLE_COUNT) && pr.particle_count != getParticleCount()) {
+      check(cf_set_particle_count(h_, pr.particle_count));
+      reinit = true;
+    }
+    if (pr.present & CF_PRESET_NUM_TYPES) {
+      check(cf_set_num_particle_types(h_, pr.num_particle_types));
+      types_ = pr.num_particle_types;
+      initializeForceTable();
+      initializeRadioByType();
+      reinit = true;
+    }
+    params.numParticleTypes = types_;
+    struct F {
+      uint32_t bit;
+      float cf_preset::*src;
+      float SimulationParams::*dst;
+    };
+    const F fields[] = {
+        {CF_PRESET_RADIUS, &cf_preset::radius, &SimulationParams::radius},
+        {CF_PRESET_DELTA_T, &cf_preset::delta_t, &SimulationParams::delta_t},
+        {CF_PRESET_FRICTION, &cf_preset::friction, &SimulationParams::friction},
+        {CF_PRESET_REPULSION, &cf_preset::repulsion, &SimulationParams::repulsion},
+        {CF_PRESET_ATTRACTION, &cf_preset::attraction, &SimulationParams::attraction},
+        {CF_PRESET_K, &cf_preset::k, &SimulationParams::k},
+        {CF_PRESET_BALANCE, &cf_preset::balance, &SimulationParams::balance},
+        {CF_PRESET_FORCE_MULTIPLIER, &cf_preset::force_multiplier,
+         &SimulationParams::forceMultiplier},
+        {CF_PRESET_CANVAS_WIDTH, &cf_preset::canvas_width, &SimulationParams::canvasWidth},
+        {CF_PRESET_CANVAS_HEIGHT, &cf_preset::canvas_height, &SimulationParams::canvasHeight},
+        {CF_PRESET_CANVAS_DEPTH, &cf_preset::canvas_depth, &SimulationParams::canvasDepth},
+        {CF_PRESET_SPAWN_REGION, &cf_preset::spawn_region_size,
+         &SimulationParams::spawnRegionSize},
+        {CF_PRESET_RATIO, &cf_preset::ratio, &SimulationParams::ratioWithLFO},  // LFO phase 0
+    };
+    for (const F& f : fields)
+      if (pr.present & f.bit) params.*f.dst = pr.*f.src;
+    for (int i = 0; i < pr.num_radio && i < types_; ++i)
+      setRadioByTypeValue(i, pr.radio_by_type[i]);
+    for (int i = 0; i < pr.num_raw && i < types_ * types_; ++i) raw_[i] = pr.raw_force_table[i];
+    updateForceTable(pr.force_range, pr.force_bias, pr.force_offset);
+    if (reinit) {
+      canvas_[0] = params.canvasWidth;
+      canvas_[1] = params.canvasHeight;
+      canvas_[2] = params.canvasDepth;
+      spawn_ = params.spawnRegionSize;
+      initializeParticles();
+    }
+    return pr;
+  }
+
+  // CellFlowWidget::savePreset (:1205-1269): the simulation's count, types, radius modifiers
+  // and raw table and the step fields of `params`, with the UI fields of `ui` (force range,
+  // bias and offset, ratio, LFO, colours).  A non-null `template_path` keeps every other
+  // member of that file (render and camera settings).
+  void savePreset(const char* path, const SimulationParams& params, cf_preset ui,
+                  const char* template_path = nullptr) {
+    ui.present |= CF_PRESET_PARTICLE_COUNT | CF_PRESET_NUM_TYPES | CF_PRESET_RADIUS |
+                  CF_PRESET_DELTA_T | CF_PRESET_FRICTION | CF_PRESET_REPULSION |
+                  CF_PRESET_ATTRACTION | CF_PRESET_K | CF_PRESET_BALANCE |
+                  CF_PRESET_FORCE_MULTIPLIER | CF_PRESET_CANVAS_WIDTH | CF_PRESET_CANVAS_HEIGHT |
+                  CF_PRESET_CANVAS_DEPTH | CF_PRESET_RADIO_BY_TYPE | CF_PRESET_RAW_FORCE_TABLE;
+    ui.particle_count = getParticleCount();
+    ui.num_particle_types = types_;
+    ui.radius = params.radius;
+    ui.delta_t = params.delta_t;
+    ui.friction = params.friction;
+    ui.repulsion = params.repulsion;
+    ui.attraction = params.attraction;
+    ui.k = params.k;
+    ui.balance = params.balance;
+    ui.force_multiplier = params.forceMultiplier;
+    ui.canvas_width = params.canvasWidth;
+    ui.canvas_height = params.canvasHeight;
+    ui.canvas_depth = params.canvasDepth;
+    const std::vector<float> rbt = getRadioByType();
+    ui.num_radio = types_;
+    for (int i = 0; i < types_; ++i) ui.radio_by_type[i] = rbt[i];
+    ui.num_raw = types_ * types_;
+    for (int i = 0; i < types_ * types_; ++i) ui.raw_force_table[i] = raw_[i];
+    check(cf_preset_write(path, &ui, template_path));
+  }
+
   cf_sim* handle() const { return h_; }
 
  private:
@@ -257,6 +351,7 @@ class ParticleSimulation {
   int types_ = 6;
   uint64_t seed_ = 1;
   float canvas_[3] = {8000.0f, 8000.0f, 8000.0f};
+  float spawn_ = 2000.0f;
   float raw_[MAX_PARTICLE_TYPES * MAX_PARTICLE_TYPES] = {};
 };
 

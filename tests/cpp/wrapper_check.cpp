@@ -2,7 +2,8 @@
 // (CellFlowWidget.cpp:410-427 step per frame, :551-575 proximity graph after the step,
 // :906-917 raw-table edits + updateForceTable, :893 setRadioByTypeValue) and writes the
 // inputs it ended up with and the outputs to <dir>, so a test can replay the same calls
-// through another binding and compare bit for bit.
+// through another binding and compare bit for bit.  With a preset path as the second
+// argument it also loads that preset, steps, and saves it to <dir>/saved.json.
 //
 // Exit status: 0 success; 10 + cellflow status on a cellflow::Error (12 = no device).
 #include <cstdio>
@@ -71,6 +72,20 @@ int main(int argc, char** argv) {
     dump(dir + "/verts.bin", verts.data(), verts.size());
     dump(dir + "/edges.bin", edges.data(), edges.size());
     std::printf("ok n=%d frames=%d vertices=%d edges=%zu\n", n, frames, vc, edges.size() / 2);
+
+    // CellFlowWidget::loadPreset / savePreset: load a preset into a fresh simulation, run a
+    // few frames, save it back with the original as the template.
+    if (argc > 2) {
+      cellflow::ParticleSimulation ps(1000);
+      cellflow::SimulationParams pp;
+      const cf_preset ui = ps.loadPreset(argv[2], pp);
+      for (int f = 0; f < 3; ++f) ps.simulate(pp);
+      ps.savePreset((dir + "/saved.json").c_str(), pp, ui, argv[2]);
+      std::vector<float> t(ps.getNumParticleTypes() * ps.getNumParticleTypes());
+      cellflow::check(cf_get_force_table(ps.handle(), t.data(), (int)t.size()));
+      dump(dir + "/preset_table.bin", t.data(), t.size());
+      std::printf("preset n=%d types=%d\n", ps.getParticleCount(), ps.getNumParticleTypes());
+    }
     return 0;
   } catch (const cellflow::Error& e) {
     std::printf("%s\n", e.what());

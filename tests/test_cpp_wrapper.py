@@ -13,15 +13,19 @@ import numpy as np
 import pytest
 
 from cellflow_amd import PARTICLE_DTYPE, ParticleSimulation, SimulationParams, device_count
-from cellflow_amd import force_table_transform
+from cellflow_amd import force_table_transform, read_preset
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "_build", "wrapper_check")
 
 
+PRESET = os.path.join(ROOT, "tests", "golden", "presets", "eater.json")
+
+
 def run(outdir):
     assert os.path.exists(BIN), "build first: python -c 'import __graft_entry__ as g; g.build()'"
-    return subprocess.run([BIN, str(outdir)], capture_output=True, text=True, timeout=120)
+    return subprocess.run([BIN, str(outdir), PRESET], capture_output=True, text=True,
+                          timeout=120)
 
 
 @pytest.mark.skipif(device_count() > 0, reason="checks the no-device path")
@@ -67,3 +71,9 @@ def test_wrapper_matches_python_binding(tmp_path):
     assert np.array_equal(sim.getNeighborCounts(), load("counts.bin", np.int32))
     assert np.array_equal(sim.getEdges().ravel(), load("edges.bin", np.int32))
     assert np.array_equal(sim.getLineVertices(colors).ravel(), load("verts.bin", np.float32))
+    # the preset loaded, stepped and saved by the C++ wrapper: same file, same table
+    assert (tmp_path / "saved.json").read_bytes() == open(PRESET, "rb").read()
+    pr = read_preset(PRESET)
+    np.testing.assert_array_equal(load("preset_table.bin", np.float32),
+                                  force_table_transform(pr["rawForceTable"], pr["forceRange"],
+                                                        pr["forceBias"], pr["forceOffset"]))

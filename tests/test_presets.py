@@ -1,0 +1,132 @@
+"""Preset files (CellFlowWidget::loadPreset / savePreset, CellFlowWidget.cpp:1070-1269)
+through the native reader/writer, without a GPU.
+
+The reference's four preset files were written by Qt's QJsonDocument::toJson; reading one
+and writing it back with itself as the template must reproduce it byte for byte, which pins
+the writer's layout and number formatting to Qt's.  Python's json module (same layout with
+indent=4, sort_keys=True) cross-checks files written from random presets."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+from hypothesis import given, settings
+from hypothesis import strategies as hs
+
+from cellflow_amd import CellFlowError, _abi, preset_params, read_preset, write_preset
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "presets")
+NAMES = ["settings", "eater", "littlecells", "pulser"]
+
+
+def golden(name):
+    return os.path.join(GOLDEN, name + ".json")
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_read_matches_json(name):
+    doc = json.load(open(golden(name)))
+    pr = read_preset(golden(name))
+    engine_keys = set(_abi.PRESET_KEYS) & set(doc)
+    assert set(pr) == engine_keys
+    for k in engine_keys - {"particleColors", "radioByType", "rawForceTable"}:
+        assert pr[k] == np.float32(doc[k]) if isinstance(pr[k], float) else pr[k] == doc[k], k
+    np.testing.assert_array_equal(pr["rawForceTable"], np.float32(doc["rawForceTable"]))
+    np.testing.assert_array_equal(pr["radioByType"], np.float32(doc["radioByType"]))
+    want = np.float32([[c["r"], c["g"], c["b"]] for c in doc["particleColors"]])
+    np.testing.assert_array_equal(pr["particleColors"], want)
+    T = doc["numParticleTypes"]
+    assert len(pr["rawForceTable"]) == T * T
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_roundtrip_with_template_is_byte_identical(tmp_path, name):
+    out = tmp_path / "out.json"
+    write_preset(out, read_preset(golden(name)), template=golden(name))
+    assert out.read_bytes() == open(golden(name), "rb").read()
+
+
+def test_write_without_template_has_only_engine_fields(tmp_path):
+    pr = read_preset(golden("eater"))
+    out = tmp_path / "p.json"
+    write_preset(out, pr)
+    text = out.read_text()
+    doc = json.loads(text)
+    assert set(doc) == set(pr)
+    assert text == json.dumps(doc, indent=4, sort_keys=True) + "\n"
+    back = read_preset(out)
+    for k in pr:
+        np.testing.assert_array_equal(back[k], pr[k])
+
+
+def test_params_of_a_preset():
+    pr = read_preset(golden("eater"))
+    p = preset_params(pr)
+    assert p.radius == np.float32(397) and p.numParticleTypes == 6
+    assert p.delta_t == pr["delta_t"] and p.ratioWithLFO == pr["ratio"]
+    assert p.maxExpectedNeighbors == 400
+    # fields a preset lacks take SimulationParams' defaults (SimulationParams.h:15-29)
+    q = preset_params({"radius": 5.0})
+    assert q.radius == 5.0 and q.friction == np.float32(0.51) and q.canvasWidth == 8000.0
+
+
+@pytest.mark.parametrize("text,what", [
+    ("not json", "object"), ("[1, 2]", "object"), ('{"radius": "big"}', "number"),
+    ('{"PARTICLE_COUNT": 10.5}', "integer"), ('{"rawForceTable": [1, "x"]}', "non-number"),
+    ('{"particleColors": [1]}', "colour"), ('{"radioByType": {}}', "array"),
+])
+def test_malformed_presets_are_rejected(tmp_path, text, what):
+    p = tmp_path / "bad.json"
+    p.write_text(text)
+    with pytest.raises(CellFlowError) as ei:
+        read_preset(p)
+    assert ei.value.code == _abi.CF_ERR_ARG and what in str(ei.value)
+    with pytest.raises(CellFlowError):
+        write_preset(tmp_path / "o.json", {"radius": 1.0}, template=p) if what == "object" \
+            else read_preset(p)
+    with pytest.raises(CellFlowError):
+        read_preset(tmp_path / "missing.json")
+
+
+_f32 = hs.floats(-1e6, 1e6, allow_nan=False, width=32)
+
+
+@settings(max_examples=60, deadline=None)
+@given(count=hs.integers(1, 2_000_000), T=hs.integers(1, 10), vals=hs.lists(_f32, min_size=18,
+                                                                            max_size=18),
+       data=hs.data())
+def test_property_roundtrip_and_python_layout(tmp_path_factory, count, T, vals, data):
+    keys = [k for k in _abi.PRESET_KEYS[2:20]]
+    pr = {"PARTICLE_COUNT": count, "numParticleTypes": T}
+    pr.update({k: float(v) for k, v in zip(keys, vals)})
+    pr["rawForceTable"] = np.float32(data.draw(hs.lists(_f32, min_size=T * T, max_size=T * T)))
+    pr["radioByType"] = np.float32(data.draw(hs.lists(_f32, min_size=T, max_size=T)))
+    pr["particleColors"] = np.float32(data.draw(hs.lists(_f32, min_size=3 * T,
+                                                         max_size=3 * T))).reshape(T, 3)
+    out = tmp_path_factory.mktemp("p") / "p.json"
+    write_preset(out, pr)
+    back = read_preset(out)
+    assert set(back) == set(pr)
+    for k in pr:
+        np.testing.assert_array_equal(back[k], pr[k], err_msg=k)
+    text = out.read_text()
+    doc = json.loads(text)
+    assert text == json.dumps(doc, indent=4, sort_keys=True) + "\n"
+    assert doc["rawForceTable"] == [float(x) for x in pr["rawForceTable"]]
+
+
+@settings(max_examples=40, deadline=None)
+@given(vals=hs.lists(hs.floats(allow_nan=False, allow_infinity=False, width=32), min_size=100,
+                     max_size=100))
+def test_property_number_text_is_python_repr(tmp_path_factory, vals):
+    """Every float32 value (subnormal to huge) is written as the shortest decimal that reads
+    back to it, in Python repr's layout."""
+    out = tmp_path_factory.mktemp("n") / "p.json"
+    raw = np.float32(vals)
+    write_preset(out, {"numParticleTypes": 10, "rawForceTable": raw})
+    text = out.read_text()
+    assert text == json.dumps(json.loads(text), indent=4, sort_keys=True) + "\n"
+    np.testing.assert_array_equal(read_preset(out)["rawForceTable"], raw)
+    doc = json.loads(text)
+    assert [float(x) for x in doc["rawForceTable"]] == [float(x) for x in raw]
