@@ -121,6 +121,35 @@ def ensemble_leg(steps, warm=3):
             "workload": cfg["name"], "tuning": tuned["tuning"]}
 
 
+def clustered_leg():
+    """The 1,000,000-particle clustered workload (200 Gaussian blobs, radius 0.04, ratio 0.5,
+    balance 1.2, edges 0.02/16) for its full 500 steps after autotuning: the regime where
+    cells hold hundreds of particles and the pair loop dominates."""
+    from cellflow_amd import ParticleSimulation, SimulationParams, configs, particles_from_state
+    cfg = configs.load("c3_1m_clustered")
+    st, ph = configs.make_state(cfg), configs.physics(cfg)
+    e = cfg["edges"]
+    edges = (e["distance"], e["max_connections"])
+    prm = SimulationParams.from_physics(ph)
+    sim = ParticleSimulation(cfg["n"], cfg["num_types"])
+    sim.setForceTable(st["table"])
+    sim.setRadioByType(st["rbt"])
+    sim.setParticles(particles_from_state(st["pos"], st["vel"], st["type"]))
+    tuned = sim.autotune(prm, edges=edges, trial_steps=4)
+    sim.setParticles(particles_from_state(st["pos"], st["vel"], st["type"]))
+    log(f"[bench] clustered 1M: tuning {tuned['tuning']}; {cfg['steps']} steps")
+    t0 = time.perf_counter()
+    sim.simulate(prm, cfg["steps"], edges=edges)
+    dt = time.perf_counter() - t0
+    counts = sim.getNeighborCounts()
+    out = {"workload": cfg["name"], "particles": cfg["n"], "steps": cfg["steps"],
+           "ms_per_step": dt / cfg["steps"] * 1e3,
+           "particle_steps_per_s": cfg["n"] * cfg["steps"] / dt, "tuning": tuned["tuning"],
+           "mean_neighbours_final": float(counts.mean()), "edges_final": len(sim.getEdges())}
+    sim.close()
+    return out
+
+
 def adjoint_leg():
     """Differentiable rollout (20k particles, 200 steps, checkpoints every 20): time of the
     gradient rollout against the forward rollout; central differences over the 37
@@ -177,6 +206,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
+    ap.add_argument("--no-clustered", action="store_true", help="skip the 1M clustered leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -259,6 +289,8 @@ def main():
     if not args.no_adjoint:
         log("[bench] differentiable rollout")
         out["adjoint"] = adjoint_leg()
+    if not args.no_clustered:
+        out["clustered"] = clustered_leg()
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(cfg)
