@@ -953,6 +953,368 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
 }
 
 // ------------------------------------------------------------------------------------
+// k_force_list: candidate lists in LDS.
+//
+// k_force_grp walks every run of the stencil in every lane, which at ~20 candidates per
+// particle costs more instructions than the candidates themselves.  Here each lane builds
+// the runs of its own stencil row(s) in registers, a group prefix scan places them in a
+// flattened order, the lanes copy their slots into a per-particle LDS list (chunks of
+// kList), and the G lanes then take list entries g, g+G, ... with one loop body for
+// shifted and unshifted runs alike.  Group sums use DPP mirror butterflies.
+
+constexpr int kList = 64;
+
+// Sum over aligned groups of G lanes (G in 4, 8, 16, 32), every lane gets its group's sum.
+template <int G>
+__device__ __forceinline__ int dpp_group_sum_i(int v) {
+  // quad_perm [1,0,3,2] and [2,3,0,1]: partners within a quad
+  v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+  if (G >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // half mirror
+  if (G >= 16) v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // row mirror
+  if (G >= 32) v += __shfl_xor(v, 16, kWave);
+  return v;
+}
+
+template <int Ctrl>
+__device__ __forceinline__ float dpp_add(float x) {
+  return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), Ctrl, 0xF, 0xF, false));
+}
+
+template <int G>
+__device__ __forceinline__ float dpp_group_sum_f(float x) {
+  x = dpp_add<0xB1>(x);
+  x = dpp_add<0x4E>(x);
+  if (G >= 8) x = dpp_add<0x141>(x);
+  if (G >= 16) x = dpp_add<0x140>(x);
+  if (G >= 32) x += __shfl_xor(x, 16, kWave);
+  return x;
+}
+
+// Exclusive prefix sum over aligned groups of G lanes (g = lane index in the group).
+template <int G>
+__device__ __forceinline__ uint32_t group_excl_scan(uint32_t v, int g) {
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, kWave);
+    if (g >= d) x += y;
+  }
+  return x - v;
+}
+
+// Per-particle context of the stencil walk.
+struct Own {
+  float4 pi;
+  uint32_t ti;
+  int cx, cy, cz;
+  float fxr, fyr, fzr;  // fractional position inside the cell, per axis
+  float cut2;           // pruning radius^2 of the particle's type
+  bool gen;             // grid too small for image-shifted runs
+  int nyo, nzo;         // stencil rows per axis
+};
+
+// The up to two runs of stencil row r (x-neighbour cells are adjacent slots): the
+// unwrapped part and the part wrapped across the x boundary, each with its image-shift
+// code (x: bits 0-1, y: 2-3, z: 4-5; 1 = -L, 2 = +L).  Rows and x ends farther than the
+// type's reach (with margins) are dropped.  Length 0 means no run.
+template <int R>
+__device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, const Own& c,
+                                         int r, uint32_t& s0, uint32_t& l0, uint32_t& k0,
+                                         uint32_t& s1, uint32_t& l1, uint32_t& k1) {
+  s0 = l0 = k0 = s1 = l1 = k1 = 0u;
+  const int iy = r % c.nyo, iz = r / c.nyo;
+  int y, z, dy = 0, dz = 0;
+  uint32_t cyc = 0u, czc = 0u;
+  if (c.gen && sd.ny < 2 * R + 1) {
+    y = iy;
+  } else {
+    dy = iy - R;
+    y = c.cy + dy;
+    if (y < 0) { y += sd.ny; cyc = 1u; }
+    else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
+  }
+  if (c.gen && sd.nz < 2 * R + 1) {
+    z = iz;
+  } else {
+    dz = iz - R;
+    z = c.cz + dz;
+    if (z < 0) { z += sd.nz; czc = 1u; }
+    else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
+  }
+  float gyz = 0.0f;
+  if (!c.gen) {
+    const float gy = cell_gap(dy, c.fyr) * sd.hy, gz = cell_gap(dz, c.fzr) * sd.hz;
+    gyz = fmaf(gy, gy, gz * gz);
+    if (!(gyz < c.cut2)) return;
+  }
+  const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+  int xlo, xhi;
+  if (c.gen && sd.nx < 2 * R + 1) {
+    xlo = 0;
+    xhi = sd.nx - 1;
+  } else {
+    xlo = c.cx - R;
+    xhi = c.cx + R;
+    if (!c.gen) {
+      while (xlo < c.cx) {
+        const float gx = cell_gap(xlo - c.cx, c.fxr) * sd.hx;
+        if (fmaf(gx, gx, gyz) < c.cut2) break;
+        ++xlo;
+      }
+      while (xhi > c.cx) {
+        const float gx = cell_gap(xhi - c.cx, c.fxr) * sd.hx;
+        if (fmaf(gx, gx, gyz) < c.cut2) break;
+        --xhi;
+      }
+    }
+  }
+  const uint32_t yzc = (cyc << 2) | (czc << 4);
+  int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
+  uint32_t xc1 = 0u;
+  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
+  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
+  s0 = a.start[rowc + a0];
+  l0 = a.start[rowc + b0 + 1] - s0;
+  k0 = yzc;
+  if (a1 <= b1) {
+    s1 = a.start[rowc + a1];
+    l1 = a.start[rowc + b1 + 1] - s1;
+    k1 = yzc | xc1;
+  }
+}
+
+template <bool EDGES, int R, int G>
+__global__ void __launch_bounds__(256) k_force_list(ForceArgs a) {
+  constexpr int kRows = (2 * R + 1) * (2 * R + 1);
+  constexpr int kOwn = kWave / G;
+  constexpr int kRpl = (kRows + G - 1) / G;  // stencil rows per lane
+  __shared__ PairConst tab[4][kMaxTypes * kMaxTypes];
+  __shared__ uint32_t lslot[4][kOwn][kList];
+  __shared__ uint8_t lcode[4][kOwn][kList];
+  __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
+  __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
+  __shared__ uint32_t el_n[4][kOwn];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gw = blockIdx.x * (blockDim.x >> 6) + wv;
+  const long base = (long)gw * kOwn;
+  const bool wave_ok = base < (long)a.nwaves * kWave;
+  const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
+  const SimDesc sd = a.sims[s];
+  if (wave_ok)
+    for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = a.pairs[sd.tab + t];
+  const int o = lane / G, g = lane % G;
+  const int p = (int)base + o;
+  const bool active = wave_ok && (p - sd.p0) < sd.n;
+  Own c;
+  float4 vi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t idi = 0;
+  c.pi = vi;
+  if (active) {
+    c.pi = a.pos[p];
+    vi = a.vel[p];
+    idi = a.id[p];
+  }
+  const float4 pi = c.pi;
+  c.ti = active ? __float_as_uint(pi.w) : 0u;
+  const uint32_t ti = c.ti;
+  c.cx = bin_axis(pi.x, sd.ihx, sd.nx);
+  c.cy = bin_axis(pi.y, sd.ihy, sd.ny);
+  c.cz = bin_axis(pi.z, sd.ihz, sd.nz);
+  c.gen = sd.generic != 0;
+  c.nyo = c.gen && sd.ny < 2 * R + 1 ? sd.ny : 2 * R + 1;
+  c.nzo = c.gen && sd.nz < 2 * R + 1 ? sd.nz : 2 * R + 1;
+  const int nrows = c.nyo * c.nzo;
+  c.fxr = fminf(fmaxf(pi.x * sd.ihx - (float)c.cx, 0.0f), 1.0f);
+  c.fyr = fminf(fmaxf(pi.y * sd.ihy - (float)c.cy, 0.0f), 1.0f);
+  c.fzr = fminf(fmaxf(pi.z * sd.ihz - (float)c.cz, 0.0f), 1.0f);
+  if (EDGES && g == 0) el_n[wv][o] = 0u;
+  __syncthreads();  // pair table ready
+  const PairConst* row = &tab[wv][ti * sd.T];
+  c.cut2 = row[0].pad;
+
+  // ---- this lane's runs, their place in the particle's flattened candidate order ----
+  uint32_t rs[2 * kRpl], rl[2 * kRpl], rk[2 * kRpl];
+  uint32_t mine = 0u;
+#pragma unroll
+  for (int i = 0; i < kRpl; ++i) {
+    const int r = g + i * G;
+    rs[2 * i] = rl[2 * i] = rk[2 * i] = rs[2 * i + 1] = rl[2 * i + 1] = rk[2 * i + 1] = 0u;
+    if (active && r < nrows)
+      row_runs<R>(a, sd, c, r, rs[2 * i], rl[2 * i], rk[2 * i], rs[2 * i + 1], rl[2 * i + 1],
+                  rk[2 * i + 1]);
+    mine += rl[2 * i] + rl[2 * i + 1];
+  }
+  const uint32_t first = group_excl_scan<G>(mine, g);
+  const uint32_t total = (uint32_t)dpp_group_sum_i<G>((int)mine);
+
+  // ---- candidates, kList at a time -------------------------------------------------
+  const bool do_edges = EDGES && a.edges && sd.edge_on && active;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  int cnt = 0;
+  for (uint32_t b0 = 0; b0 < total; b0 += (uint32_t)kList) {
+    const uint32_t b1 = b0 + (uint32_t)kList;
+    uint32_t f0 = first;
+#pragma unroll
+    for (int k = 0; k < 2 * kRpl; ++k) {
+      const uint32_t lo = max(f0, b0), hi = min(f0 + rl[k], b1);
+      for (uint32_t f = lo; f < hi; ++f) {
+        lslot[wv][o][f - b0] = rs[k] + (f - f0);
+        lcode[wv][o][f - b0] = (uint8_t)rk[k];
+      }
+      f0 += rl[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t nb = min(total - b0, (uint32_t)kList);
+    for (uint32_t k = (uint32_t)g; k < nb; k += (uint32_t)G) {
+      const uint32_t q = lslot[wv][o][k];
+      const uint32_t code = lcode[wv][o][k];
+      const float4 pj = a.pos[q];
+      float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+      bool raw;
+      if (!c.gen) {
+        raw = code == 0u;
+        if (!raw) {  // image-shifted run
+          dx += shift_of(code & 3u, sd.Lx);
+          dy += shift_of((code >> 2) & 3u, sd.Ly);
+          dz += shift_of((code >> 4) & 3u, sd.Lz);
+        }
+      } else {  // per-pair minimum image
+        const float ux = dx, uy = dy, uz = dz;
+        dx = min_image(ux, sd.Lx);
+        dy = min_image(uy, sd.Ly);
+        dz = min_image(uz, sd.Lz);
+        raw = (dx == ux) && (dy == uy) && (dz == uz);
+      }
+      const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      const uint32_t tj = __float_as_uint(pj.w);
+      const PairConst pc = row[tj];
+      const float sp = s2 + 1e-4f;
+      if (sp < pc.thr) {
+        cnt++;
+        const float inv = __builtin_amdgcn_rsqf(sp);
+        const float rr = (sp * inv) * pc.inv_eff;
+        const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
+        const float w = (sd.rep * ex - sd.att * rr) * pc.fv * inv;
+        fx += dx * w;
+        fy += dy * w;
+        fz += dz * w;
+      }
+      if (EDGES) {
+        // edges are non-periodic (ParticleSimulation.cu:222-227): unshifted pairs only
+        if (do_edges && raw && tj == ti && s2 < sd.edge_d2) {
+          const uint32_t jid = a.id[q];
+          if (jid > idi) {
+            const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
+            if (kk < (uint32_t)kEdgeList) {
+              el_d[wv][o][kk] = make_float2(s2, 0.0f);
+              el_i[wv][o][kk] = jid;
+            }
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // list reused by the next chunk
+  }
+  fx = dpp_group_sum_f<G>(fx);
+  fy = dpp_group_sum_f<G>(fy);
+  fz = dpp_group_sum_f<G>(fz);
+  cnt = dpp_group_sum_i<G>(cnt);
+  if (active && 1e-4f < row[ti].thr) cnt -= 1;  // the particle itself
+
+  if (EDGES) {
+    // barriers stay outside lane-dependent branches
+    __syncthreads();
+    const uint32_t n = do_edges ? el_n[wv][o] : 0u;
+    const bool fits = n <= (uint32_t)kEdgeList;
+    const int cap = a.edge_cap, maxc = sd.edge_maxc;
+    if (fits)
+      for (uint32_t k = g; k < n; k += G) {
+        const uint32_t ik = el_i[wv][o][k];
+        uint32_t below = 0;
+        for (uint32_t k2 = 0; k2 < n; ++k2) below += (el_i[wv][o][k2] < ik) ? 1u : 0u;
+        el_d[wv][o][k].y = (below < (uint32_t)cap) ? 1.0f : 0.0f;
+      }
+    __syncthreads();
+    if (fits)
+      for (uint32_t k = g; k < n; k += G) {
+        const float2 dk = el_d[wv][o][k];
+        if (dk.y == 0.0f) continue;
+        const uint32_t ik = el_i[wv][o][k];
+        uint32_t rank = 0;
+        for (uint32_t k2 = 0; k2 < n; ++k2) {
+          const float2 d2 = el_d[wv][o][k2];
+          const uint32_t i2 = el_i[wv][o][k2];
+          rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
+        }
+        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + p] = ik;
+      }
+    if (do_edges && g == 0) {
+      if (fits) {
+        a.ecount[idi] = (uint32_t)min(min((int)n, cap), maxc);
+      } else {
+        // more candidates than the LDS list holds: exact streaming selection by one lane,
+        // over the stencil rows again
+        EdgeAcc ea;
+        ea.q = 0;
+        ea.maxid = 0;
+        ea.maxk = 0;
+        for (int r = 0; r < nrows; ++r) {
+          uint32_t st[2], ln[2], cd[2];
+          row_runs<R>(a, sd, c, r, st[0], ln[0], cd[0], st[1], ln[1], cd[1]);
+          for (int h = 0; h < 2; ++h) {
+            if (!c.gen && cd[h] != 0u) continue;  // image-shifted runs hold no edge partner
+            for (uint32_t t = 0; t < ln[h]; ++t) {
+              const uint32_t q = st[h] + t;
+              const float4 pj = a.pos[q];
+              if (__float_as_uint(pj.w) != ti) continue;
+              const float ux = pj.x - pi.x, uy = pj.y - pi.y, uz = pj.z - pi.z;
+              if (c.gen && !(min_image(ux, sd.Lx) == ux && min_image(uy, sd.Ly) == uy &&
+                             min_image(uz, sd.Lz) == uz))
+                continue;
+              const float s2 = fmaf(uz, uz, fmaf(uy, uy, ux * ux));
+              if (!(s2 < sd.edge_d2)) continue;
+              const uint32_t jid = a.id[q];
+              if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
+            }
+          }
+        }
+        a.ecount[idi] =
+            (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p);
+      }
+    }
+  }
+
+  if (!active || g != 0) return;
+  // ParticleSimulation.cu:136-161
+  const int prev = __float_as_int(vi.w);
+  const float avg = (float)(cnt + prev) * 0.5f;
+  const float dens = fminf(avg / sd.max_exp, 1.0f);
+  const float fac = 1.0f - sd.one_minus_balance * dens;
+  const float m = sd.fm * fac;
+  fx *= m;
+  fy *= m;
+  fz *= m;
+  const float vx = vi.x * sd.friction + fx * sd.dt;
+  const float vy = vi.y * sd.friction + fy * sd.dt;
+  const float vz = vi.z * sd.friction + fz * sd.dt;
+  const float x = wrap_pos(pi.x + vx * sd.dt, sd.Lx);
+  const float y = wrap_pos(pi.y + vy * sd.dt, sd.Ly);
+  const float z = wrap_pos(pi.z + vz * sd.dt, sd.Lz);
+  const float fin = fabsf(x) + fabsf(y) + fabsf(z) + fabsf(vx) + fabsf(vy) + fabsf(vz);
+  if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
+  a.tpos[p] = make_float4(x, y, z, pi.w);
+  a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
+  a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  const uint32_t cc = (uint32_t)(sd.c0 + bin_axis(x, sd.ihx, sd.nx) +
+                                 sd.nx * (bin_axis(y, sd.ihy, sd.ny) +
+                                          sd.ny * bin_axis(z, sd.ihz, sd.nz)));
+  a.key[p] = cc;
+  a.arrival[p] = atomicAdd(&a.hist[cc], 1u);
+}
+
+// ------------------------------------------------------------------------------------
 // k_edges: standalone proximity-graph candidates on the current sorted state, with a
 // non-periodic stencil of per-axis reach (the edge distance may exceed the cell size).
 
@@ -1377,6 +1739,14 @@ static void launch_grp(const ForceArgs& a, bool edges, hipStream_t st) {
   else hipLaunchKernelGGL((k_force_grp<false, R, G>), g, b, 0, st, a);
 }
 
+template <int R, int G>
+static void launch_list(const ForceArgs& a, bool edges, hipStream_t st) {
+  const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
+  const dim3 g((unsigned)((waves + 3) / 4)), b(256);
+  if (edges) hipLaunchKernelGGL((k_force_list<true, R, G>), g, b, 0, st, a);
+  else hipLaunchKernelGGL((k_force_list<false, R, G>), g, b, 0, st, a);
+}
+
 hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
                         int lanes, hipStream_t st) {
   if (variant == 0) {
@@ -1391,6 +1761,19 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
       else if (lanes == 8) launch_grp<2, 8>(a, edges, st);
       else if (lanes == 16) launch_grp<2, 16>(a, edges, st);
       else if (lanes == 32) launch_grp<2, 32>(a, edges, st);
+      else return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+  if (variant == 3) {
+    if (reach == 1) {
+      if (lanes == 8) launch_list<1, 8>(a, edges, st);
+      else if (lanes == 16) launch_list<1, 16>(a, edges, st);
+      else if (lanes == 32) launch_list<1, 32>(a, edges, st);
+      else return hipErrorInvalidValue;
+    } else {
+      if (lanes == 16) launch_list<2, 16>(a, edges, st);
+      else if (lanes == 32) launch_list<2, 32>(a, edges, st);
       else return hipErrorInvalidValue;
     }
     return hipGetLastError();

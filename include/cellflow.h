@@ -82,9 +82,11 @@ typedef struct cf_edge_opts {
 /* Launch/grid choices made by the autotuner (no reference counterpart). */
 typedef struct cf_tuning {
   int cell_divisor;             /* 1: cell >= max cutoff, 27-cell stencil; 2: half cells, 125 */
-  int force_variant;            /* 0: G lanes per particle; 1: one lane per particle;
-                                   2: LDS-staged stencil, one wave per workgroup */
-  int lanes_per_particle;       /* G of variant 0: 4, 8, 16 or 32 */
+  int force_variant;            /* 0: G lanes per particle walking the stencil's runs;
+                                   1: one lane per particle; 2: LDS-staged stencil, one wave
+                                   per workgroup; 3: G lanes per particle over a per-particle
+                                   candidate list in LDS (G >= 8 * cell_divisor) */
+  int lanes_per_particle;       /* G of variants 0 and 3: 4, 8, 16 or 32 */
   int block_size;               /* threads per workgroup of variant 1: 64, 128, 256 */
   int steps_per_graph;          /* steps per captured HIP graph (rounded up to even);
                                    1 launches every kernel directly */
@@ -222,7 +224,8 @@ int cf_get_tuning(cf_sim* sim, cf_tuning* tuning);
  * key (device arch, log2 particle count, half-octave density, types, edge cap; see
  * cf_tuning_key) and stored there after measuring; flags CF_TUNE_FORCE re-measures even
  * on a hit.  `report` (optional, report_cap bytes) receives a JSON object with the key,
- * whether the cache answered and the per-candidate microseconds per step. */
+ * whether the cache answered and the per-candidate microseconds per step (truncated to
+ * report_cap - 1 bytes). */
 #define CF_TUNE_FORCE 1
 int cf_autotune(cf_sim* sim, const cf_params* params, const cf_edge_opts* edges,
                 int trial_steps, const char* cache_path, int flags, cf_tuning* chosen,

@@ -124,13 +124,14 @@ def test_edges_truncation_matches_oracle(orc, maxc):
         assert np.array_equal(edges, ref), f"dist={dist} maxc={maxc}"
 
 
-@pytest.mark.parametrize("lanes", [4, 16])
-def test_fused_edges_dense_blobs(orc, lanes):
-    """Fused edge candidates on dense blobs: many owners exceed the per-particle LDS list
-    and take the streaming fallback; the graph must still equal the oracle's."""
+@pytest.mark.parametrize("variant,lanes", [(0, 4), (0, 16), (3, 16), (3, 32)])
+def test_fused_edges_dense_blobs(orc, variant, lanes):
+    """Fused edge candidates on dense blobs: many owners exceed the per-particle LDS edge
+    list and take the streaming fallback (and, in variant 3, hold several candidate-list
+    chunks); the graph must still equal the oracle's."""
     st, phys, params = clustered_state(n=5000, blobs=4, sd=0.006)
     sim = make_sim(st, phys)
-    sim.setTuning(force_variant=0, lanes_per_particle=lanes)
+    sim.setTuning(force_variant=variant, lanes_per_particle=lanes)
     for k in range(2):
         sim.simulate(params, 1, edges=(0.01, 16))
         got = sim.getParticleData()
@@ -271,7 +272,8 @@ def test_edge_capacity_error():
 
 # every combination the autotuner times: (variant, lanes, divisor, block size)
 VARIANTS = [(0, g, d, 128) for d in (1, 2) for g in (4, 8, 16, 32)] + \
-    [(1, 16, d, b) for d in (1, 2) for b in (64, 128, 256)] + [(2, 16, 1, 128), (2, 16, 2, 128)]
+    [(1, 16, d, b) for d in (1, 2) for b in (64, 128, 256)] + [(2, 16, 1, 128), (2, 16, 2, 128)] + \
+    [(3, g, 1, 128) for g in (8, 16, 32)] + [(3, g, 2, 128) for g in (16, 32)]
 
 
 @pytest.mark.parametrize("variant,lanes,divisor,block", VARIANTS)
