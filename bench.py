@@ -34,16 +34,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic(path):
-    """HBM bytes per k_force launch from the committed rocprofv3 PMC summary, if present."""
+def force_kernel_name(t, edges=True):
+    """The force kernel a tuning launches, as rocprofv3 names it."""
+    e = "true" if edges else "false"
+    r, g = t["cell_divisor"], t["lanes_per_particle"]
+    return {0: f"k_force_grp<{e}, {r}, {g}>", 1: f"k_force<{e}>", 2: f"k_force_lds<{e}, {r}>",
+            3: f"k_force_list<{e}, {r}, {g}>"}[t["force_variant"]]
+
+
+def read_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if that
+    summary is of the same kernel."""
+    vals = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if ":" in line and line.split(":", 1)[0] in ("kernel", "traffic_bytes_per_launch"):
+                    k, v = line.split(":", 1)
+                    vals[k] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if vals.get("kernel") != kernel or "traffic_bytes_per_launch" not in vals:
+        return None
+    return float(vals["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(cfg, budget_s=10.0):
@@ -262,13 +275,15 @@ def main():
     force_ms = kt["force"]
     alg_bytes = n * (STEP_BYTES + EDGE_BYTES) + EDGE_PAIR_BYTES * nedges
     achieved = alg_bytes / (force_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "k_force", "achieved": achieved, "peak": HBM_PEAK_GBS,
+    kname = force_kernel_name(tuned["tuning"])
+    roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": read_traffic(os.path.join(ROOT, "profiles", "k_force_c2_traffic.txt")),
+            "traffic": read_traffic(os.path.join(ROOT, "profiles", "k_force_c2_traffic.txt"),
+                                    kname),
             "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": force_ms,
             "kernel_ms_per_launch": {k: round(v, 5) for k, v in kt.items()},
-            "note": "k_force runs every step; the standalone edge pass (edges, edge_scan, "
-                    "edge_write) runs once per call, after the last step"}
+            "note": "the force kernel runs every step; the standalone edge pass (edges, "
+                    "edge_scan, edge_write) runs once per call, after the last step"}
 
     out = {"metric": METRIC, "value": rate, "unit": "particle-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
