@@ -63,93 +63,7 @@ __device__ __forceinline__ int bin_axis_a(float x, float ih, int n) {
   return (int)u;
 }
 
-// Visit every slot of the cells within `r` cells (periodic, each cell once) of cell
-// (cx, cy, cz) in a fixed order.
-template <class Fn>
-__device__ __forceinline__ void for_cells(const SimDesc& sd, const uint32_t* start, int cx, int cy,
-                                          int cz, int r, Fn&& fn) {
-  const int nzo = (sd.nz < 2 * r + 1) ? sd.nz : 2 * r + 1;
-  const int nyo = (sd.ny < 2 * r + 1) ? sd.ny : 2 * r + 1;
-  const int nxo = (sd.nx < 2 * r + 1) ? sd.nx : 2 * r + 1;
-  for (int iz = 0; iz < nzo; ++iz) {
-    int z = (sd.nz < 2 * r + 1) ? iz : cz + iz - r;
-    z = (z % sd.nz + sd.nz) % sd.nz;
-    for (int iy = 0; iy < nyo; ++iy) {
-      int y = (sd.ny < 2 * r + 1) ? iy : cy + iy - r;
-      y = (y % sd.ny + sd.ny) % sd.ny;
-      for (int ix = 0; ix < nxo; ++ix) {
-        int x = (sd.nx < 2 * r + 1) ? ix : cx + ix - r;
-        x = (x % sd.nx + sd.nx) % sd.nx;
-        const int c = sd.c0 + x + sd.nx * (y + sd.ny * z);
-        for (uint32_t q = start[c]; q < start[c + 1]; ++q) fn(q);
-      }
-    }
-  }
-}
-
 // ---- histogram and its gradient ------------------------------------------------------
-
-__global__ void __launch_bounds__(128) k_hist(HistArgs a) {
-  __shared__ double part[128][kMaxBins];
-  const SimDesc sd = a.sims[0];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double* h = part[threadIdx.x];
-  for (int b = 0; b < a.bins; ++b) h[b] = 0.0;
-  if (k < sd.n) {
-    const int p = sd.p0 + k;
-    const float4 pi = a.pos[p];
-    const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
-              cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
-    const double w = a.r_max / a.bins, reach = a.r_max + 2.0 * w;
-    for_cells(sd, a.start, cx, cy, cz, a.reach, [&](uint32_t q) {
-      if ((int)q == p) return;
-      const float4 pj = a.pos[q];
-      const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
-      const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
-      const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      if (r >= reach) return;
-      const double u = r / w - 0.5;
-      const int b0 = (int)floor(u) - 1;
-      for (int b = max(b0, 0); b <= min(b0 + 3, a.bins - 1); ++b) h[b] += bspline3(u - b);
-    });
-  }
-  for (int b = 0; b < a.bins; ++b) a.partial[(long)b * a.stride + k] = (k < sd.n) ? h[b] : 0.0;
-}
-
-// lx_i = dL/dx_i of the final state: (2/N) sum_j sum_b gh_b B3'((r - c_b)/w) / w (x_i - x_j)/r
-__global__ void __launch_bounds__(128) k_hist_grad(HistArgs a) {
-  const SimDesc sd = a.sims[0];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sd.n) return;
-  const int p = sd.p0 + k;
-  const float4 pi = a.pos[p];
-  const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
-            cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
-  const double w = a.r_max / a.bins, reach = a.r_max + 2.0 * w;
-  double gx = 0.0, gy = 0.0, gz = 0.0;
-  for_cells(sd, a.start, cx, cy, cz, a.reach, [&](uint32_t q) {
-    if ((int)q == p) return;
-    const float4 pj = a.pos[q];
-    const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
-    const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
-    const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (r >= reach || r <= 0.0) return;
-    const double u = r / w - 0.5;
-    const int b0 = (int)floor(u) - 1;
-    double s = 0.0;
-    for (int b = max(b0, 0); b <= min(b0 + 3, a.bins - 1); ++b)
-      s += a.gh[b] * bspline3_d(u - b);
-    s *= 2.0 / ((double)sd.n * w * r);  // d r / d x_i = -(d)/r, d = x_j - x_i
-    gx -= s * dx;
-    gy -= s * dy;
-    gz -= s * dz;
-  });
-  const uint32_t id = a.id[p];
-  a.lam_x[id] = make_double4(gx, gy, gz, 0.0);
-  a.lam_v[id] = make_double4(0.0, 0.0, 0.0, 0.0);
-}
 
 // Fixed-order sum of `n` values with stride, one output per block (deterministic).
 __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ in, long stride,
@@ -200,89 +114,252 @@ __global__ void k_adj_phi(AdjArgs a) {
   a.lam_v[id] = make_double4(fr * mx, fr * my, fr * mz, 0.0);
 }
 
-// lx_i = lx'_i + sum_j (J_ji phi_j - J_ij phi_i); parameter partials of particle i
-__global__ void __launch_bounds__(128) k_adj_pair(AdjArgs a) {
+// ---- pair kernels: G lanes per particle -----------------------------------------------
+//
+// One lane per particle would leave most of the GPU idle at a few tens of thousands of
+// particles and walk every candidate serially.  Instead the stencil's cells are split
+// over G lanes (cell index c = g, g+G, ...) and the lanes' partial sums are reduced in a
+// fixed order, so results are deterministic.
+
+constexpr int kAdjG = 16;                      // lanes per particle
+constexpr int kAdjBlock = 256;                 // threads per block
+constexpr int kAdjPer = kAdjBlock / kAdjG;     // particles per block
+constexpr int kAdjVals = 4 + kMaxTypes;        // lx, ly, lz, ratio, table row
+
+// Visit the slots of the stencil cells c = first, first + step, ... of the (2r+1)^3 cube
+// around (cx, cy, cz) (every cell once on axes with fewer than 2r+1 cells), in a fixed
+// order.  Cell coordinates come from float reciprocals: (c + 1/2) / m is at least 1/(2m)
+// away from an integer and m <= 25, far beyond the float error.
+template <class Fn>
+__device__ __forceinline__ void for_cells_split(const SimDesc& sd, const uint32_t* start, int cx,
+                                                int cy, int cz, int r, int first, int step,
+                                                Fn&& fn) {
+  const bool gx = sd.nx < 2 * r + 1, gy = sd.ny < 2 * r + 1, gz = sd.nz < 2 * r + 1;
+  const int nxo = gx ? sd.nx : 2 * r + 1, nyo = gy ? sd.ny : 2 * r + 1,
+            nzo = gz ? sd.nz : 2 * r + 1;
+  const int ncell = nxo * nyo * nzo, nxy = nxo * nyo;
+  const float inxy = 1.0f / (float)nxy, inx = 1.0f / (float)nxo;
+  for (int ci = first; ci < ncell; ci += step) {
+    const int iz = (int)(((float)ci + 0.5f) * inxy);
+    const int rem = ci - iz * nxy;
+    const int iy = (int)(((float)rem + 0.5f) * inx);
+    const int ix = rem - iy * nxo;
+    int x = gx ? ix : cx + ix - r, y = gy ? iy : cy + iy - r, z = gz ? iz : cz + iz - r;
+    if (x < 0) x += sd.nx; else if (x >= sd.nx) x -= sd.nx;
+    if (y < 0) y += sd.ny; else if (y >= sd.ny) y -= sd.ny;
+    if (z < 0) z += sd.nz; else if (z >= sd.nz) z -= sd.nz;
+    const int c = sd.c0 + x + sd.nx * (y + sd.ny * z);
+    const uint32_t q1 = start[c + 1];
+    for (uint32_t q = start[c]; q < q1; ++q) fn(q);
+  }
+}
+
+// Histogram partials, G lanes per particle: per-lane bins in LDS, then a fixed-order sum
+// over the group's lanes.  A pair is skipped early when its FP32 squared distance exceeds
+// the reach squared by more than FP32 rounding can explain.
+__global__ void __launch_bounds__(128) k_hist_g(HistArgs a) {
+  constexpr int G = kAdjG;
+  __shared__ double part[128][kMaxBins];
   const SimDesc sd = a.sims[0];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sd.n) return;
+  const int o = threadIdx.x / G, g = threadIdx.x % G;
+  const int k = blockIdx.x * (128 / G) + o;
+  double* h = part[threadIdx.x];
+  for (int b = 0; b < a.bins; ++b) h[b] = 0.0;
+  const double w = a.r_max / a.bins, reach = a.r_max + 2.0 * w;
+  const float reach2f = (float)(reach * reach * (1.0 + 1e-5));
+  if (k < sd.n) {
+    const int p = sd.p0 + k;
+    const float4 pi = a.pos[p];
+    const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
+              cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
+    for_cells_split(sd, a.start, cx, cy, cz, a.reach, g, G, [&](uint32_t q) {
+      if ((int)q == p) return;
+      const float4 pj = a.pos[q];
+      const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
+                  fz = min_image_f(pj.z - pi.z, sd.Lz);
+      if (fx * fx + fy * fy + fz * fz > reach2f) return;
+      const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
+      const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
+      const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      if (r >= reach) return;
+      const double u = r / w - 0.5;
+      const int b0 = (int)floor(u) - 1;
+      for (int b = max(b0, 0); b <= min(b0 + 3, a.bins - 1); ++b) h[b] += bspline3(u - b);
+    });
+  }
+  __syncthreads();
+  if (k < sd.n)
+    for (int b = g; b < a.bins; b += G) {
+      double acc = 0.0;
+      for (int l = 0; l < G; ++l) acc += part[o * G + l][b];
+      a.partial[(long)b * a.stride + k] = acc;
+    }
+}
+
+// lx of the final state, G lanes per particle (fixed-order group sum by shuffles).
+__global__ void __launch_bounds__(128) k_hist_grad_g(HistArgs a) {
+  constexpr int G = kAdjG;
+  const SimDesc sd = a.sims[0];
+  const int o = threadIdx.x / G, g = threadIdx.x % G;
+  const int k = blockIdx.x * (128 / G) + o;
+  const bool active = k < sd.n;
+  const double w = a.r_max / a.bins, reach = a.r_max + 2.0 * w;
+  const float reach2f = (float)(reach * reach * (1.0 + 1e-5));
+  double gx = 0.0, gy = 0.0, gz = 0.0;
   const int p = sd.p0 + k;
-  const float4 pi = a.pos[p];
-  const uint32_t idi = a.id[p];
-  const int ti = (int)__float_as_uint(pi.w);
+  float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (active) {
+    pi = a.pos[p];
+    const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
+              cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
+    for_cells_split(sd, a.start, cx, cy, cz, a.reach, g, G, [&](uint32_t q) {
+      if ((int)q == p) return;
+      const float4 pj = a.pos[q];
+      const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
+                  fz = min_image_f(pj.z - pi.z, sd.Lz);
+      if (fx * fx + fy * fy + fz * fz > reach2f) return;
+      const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
+      const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
+      const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      if (r >= reach || r <= 0.0) return;
+      const double u = r / w - 0.5;
+      const int b0 = (int)floor(u) - 1;
+      double s = 0.0;
+      for (int b = max(b0, 0); b <= min(b0 + 3, a.bins - 1); ++b)
+        s += a.gh[b] * bspline3_d(u - b);
+      s *= 2.0 / ((double)sd.n * w * r);
+      gx -= s * dx;
+      gy -= s * dy;
+      gz -= s * dz;
+    });
+  }
+#pragma unroll
+  for (int m = G / 2; m > 0; m >>= 1) {
+    gx += __shfl_xor(gx, m, 64);
+    gy += __shfl_xor(gy, m, 64);
+    gz += __shfl_xor(gz, m, 64);
+  }
+  if (active && g == 0) {
+    const uint32_t id = a.id[p];
+    a.lam_x[id] = make_double4(gx, gy, gz, 0.0);
+    a.lam_v[id] = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+}
+
+// One backward step's pair terms, G lanes per particle.  Per particle: lane sums reduced in
+// lane order; per block: the table and ratio partials of its particles summed in particle
+// order into one column of `partial` (row r at partial[r * stride + block]).
+__global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
+  constexpr int G = kAdjG;
+  __shared__ double red[kAdjBlock][kAdjVals];
+  __shared__ int btype[kAdjPer];
+  const SimDesc sd = a.sims[0];
+  const int o = threadIdx.x / G, g = threadIdx.x % G;
+  const int k = blockIdx.x * kAdjPer + o;
+  const bool active = k < sd.n;
   const int T = a.c.T;
-  const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
-            cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
-  const double4 phi_i = a.phi[idi];
-  double lx = 0.0, ly = 0.0, lz = 0.0;
+  double lx = 0.0, ly = 0.0, lz = 0.0, grat = 0.0;
   double gfv[kMaxTypes];
 #pragma unroll
   for (int t = 0; t < kMaxTypes; ++t) gfv[t] = 0.0;
-  double grat = 0.0;
-  const double R = a.c.radius, rho = a.c.ratio, rep = a.c.rep, att = a.c.att, kk = a.c.k;
-  const double eps = (double)1e-4f;
-  for_cells(sd, a.start, cx, cy, cz, sd.reach, [&](uint32_t q) {
-    if ((int)q == p) return;
-    const float4 pj = a.pos[q];
-    const int tj = (int)__float_as_uint(pj.w);
-    // the forward's FP32 decision (ParticleSimulation.cu:92-112 with the engine's op order)
-    const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
-                fz = min_image_f(pj.z - pi.z, sd.Lz);
-    const float s2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-    if (!(s2f + 1e-4f < a.thr[ti * T + tj])) return;
-    const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
-    const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
-    const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
-    const double dist = sqrt(dx * dx + dy * dy + dz * dz + eps);
-    const double dR = 0.5 * R * (a.c.rbt[ti] + a.c.rbt[tj]);  // d effR / d ratio
-    const double eff = R + dR * rho;
-    const double r = dist / eff;
-    const double ex = exp(-kk * r * r);
-    const double g = rep * ex - att * r;
-    const double gp = -2.0 * kk * r * rep * ex - att;  // g'(r)
-    const double h = g / dist;
-    const double hp = gp / (eff * dist) - g / (dist * dist);
-    const double4 phi_j = a.phi[a.id[q]];
-    const double fij = a.c.table[ti * T + tj], fji = a.c.table[tj * T + ti];
-    // J phi = fv (h phi + hp d (d . phi) / dist)
-    const double dpi = (dx * phi_i.x + dy * phi_i.y + dz * phi_i.z) / dist;
-    const double dpj = (dx * phi_j.x + dy * phi_j.y + dz * phi_j.z) / dist;
-    lx += fji * (h * phi_j.x + hp * dx * dpj) - fij * (h * phi_i.x + hp * dx * dpi);
-    ly += fji * (h * phi_j.y + hp * dy * dpj) - fij * (h * phi_i.y + hp * dy * dpi);
-    lz += fji * (h * phi_j.z + hp * dz * dpj) - fij * (h * phi_i.z + hp * dz * dpi);
-    // parameters: phi_i . (g d / dist) and phi_i . fv (dg/deffR) deffR/dratio d / dist
-    const double gd = g * dpi;
+  const int p = sd.p0 + k;
+  int ti = -1;
+  uint32_t idi = 0;
+  if (active) {
+    const float4 pi = a.pos[p];
+    idi = a.id[p];
+    ti = (int)__float_as_uint(pi.w);
+    const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
+              cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
+    const double4 phi_i = a.phi[idi];
+    const double R = a.c.radius, rho = a.c.ratio, rep = a.c.rep, att = a.c.att, kk = a.c.k;
+    const double eps = (double)1e-4f;
+    for_cells_split(sd, a.start, cx, cy, cz, sd.reach, g, G, [&](uint32_t q) {
+      if ((int)q == p) return;
+      const float4 pj = a.pos[q];
+      const int tj = (int)__float_as_uint(pj.w);
+      // the forward's FP32 decision (ParticleSimulation.cu:92-112 with the engine's op order)
+      const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
+                  fz = min_image_f(pj.z - pi.z, sd.Lz);
+      const float s2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+      if (!(s2f + 1e-4f < a.thr[ti * T + tj])) return;
+      const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
+      const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
+      const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
+      const double dist = sqrt(dx * dx + dy * dy + dz * dz + eps);
+      const double dR = 0.5 * R * (a.c.rbt[ti] + a.c.rbt[tj]);
+      const double eff = R + dR * rho;
+      const double r = dist / eff;
+      const double ex = exp(-kk * r * r);
+      const double gr = rep * ex - att * r;
+      const double gp = -2.0 * kk * r * rep * ex - att;
+      const double h = gr / dist;
+      const double hp = gp / (eff * dist) - gr / (dist * dist);
+      const double4 phi_j = a.phi[a.id[q]];
+      const double fij = a.c.table[ti * T + tj], fji = a.c.table[tj * T + ti];
+      const double dpi = (dx * phi_i.x + dy * phi_i.y + dz * phi_i.z) / dist;
+      const double dpj = (dx * phi_j.x + dy * phi_j.y + dz * phi_j.z) / dist;
+      lx += fji * (h * phi_j.x + hp * dx * dpj) - fij * (h * phi_i.x + hp * dx * dpi);
+      ly += fji * (h * phi_j.y + hp * dy * dpj) - fij * (h * phi_i.y + hp * dy * dpi);
+      lz += fji * (h * phi_j.z + hp * dz * dpj) - fij * (h * phi_i.z + hp * dz * dpi);
+      const double gd = gr * dpi;
 #pragma unroll
-    for (int t = 0; t < kMaxTypes; ++t) gfv[t] += (t == tj) ? gd : 0.0;
-    const double dg_deff = -gp * r / eff;
-    grat += fij * dg_deff * dR * dpi;
-  });
-  const double4 lxn = a.lam_x_next[idi];
-  a.lam_x[idi] = make_double4(lxn.x + lx, lxn.y + ly, lxn.z + lz, 0.0);
-  // partials: row (ti * T + t) for the table, row T*T for the ratio; column = particle
+      for (int t = 0; t < kMaxTypes; ++t) gfv[t] += (t == tj) ? gd : 0.0;
+      grat += fij * (-gp * r / eff) * dR * dpi;
+    });
+  }
+  double* mine = red[threadIdx.x];
+  mine[0] = lx;
+  mine[1] = ly;
+  mine[2] = lz;
+  mine[3] = grat;
 #pragma unroll
-  for (int t = 0; t < kMaxTypes; ++t)
-    if (t < T)
-      for (int a2 = 0; a2 < T; ++a2)
-        a.partial[(long)(a2 * T + t) * a.stride + k] = (a2 == ti) ? gfv[t] : 0.0;
-  a.partial[(long)(T * T) * a.stride + k] = grat;
+  for (int t = 0; t < kMaxTypes; ++t) mine[4 + t] = gfv[t];
+  if (g == 0) btype[o] = active ? ti : -1;
+  __syncthreads();
+  // lane g of each group sums value g over the group's lanes (only lane g touches column g
+  // of the group's first row)
+  if (g < kAdjVals) {
+    double acc = 0.0;
+    for (int l = 0; l < G; ++l) acc += red[o * G + l][g];
+    red[o * G][g] = acc;
+  }
+  __syncthreads();
+  if (active && g == 0) {
+    const double4 lxn = a.lam_x_next[idi];
+    a.lam_x[idi] = make_double4(lxn.x + red[o * G][0], lxn.y + red[o * G][1],
+                                lxn.z + red[o * G][2], 0.0);
+  }
+  // block partials: row (a2 * T + t) sums the table term of particles of type a2, row T*T
+  // the ratio term, each over the block's particles in order
+  const int rows = T * T + 1;
+  for (int row = threadIdx.x; row < rows; row += kAdjBlock) {
+    double acc = 0.0;
+    if (row < T * T) {
+      const int a2 = row / T, t = row % T;
+      for (int oo = 0; oo < kAdjPer; ++oo)
+        if (btype[oo] == a2) acc += red[oo * G][4 + t];
+    } else {
+      for (int oo = 0; oo < kAdjPer; ++oo)
+        if (btype[oo] >= 0) acc += red[oo * G][3];
+    }
+    a.partial[(long)row * a.stride + blockIdx.x] = acc;
+  }
 }
 
+int adj_pair_blocks(int n) { return (n + kAdjPer - 1) / kAdjPer; }
+
 hipError_t launch_hist(const HistArgs& a, int n, hipStream_t st) {
-  const int g = (n + 127) / 128;
-  hipLaunchKernelGGL(k_hist, dim3(g), dim3(128), 0, st, a);
+  const int g = (n + (128 / kAdjG) - 1) / (128 / kAdjG);
+  hipLaunchKernelGGL(k_hist_g, dim3(g), dim3(128), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_hist_grad(const HistArgs& a, int n, hipStream_t st) {
-  const int g = (n + 127) / 128;
-  hipLaunchKernelGGL(k_hist_grad, dim3(g), dim3(128), 0, st, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_reduce_rows(const double* in, long stride, int n, int rows, double scale,
-                              double* out, int accumulate, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_rows, dim3(rows), dim3(256), 0, st, in, stride, n, scale, out,
-                     accumulate);
+  const int g = (n + (128 / kAdjG) - 1) / (128 / kAdjG);
+  hipLaunchKernelGGL(k_hist_grad_g, dim3(g), dim3(128), 0, st, a);
   return hipGetLastError();
 }
 
@@ -290,7 +367,14 @@ hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st) {
   const int g = (n + 127) / 128;
   hipLaunchKernelGGL(k_adj_counts, dim3(g), dim3(128), 0, st, a);
   hipLaunchKernelGGL(k_adj_phi, dim3(g), dim3(128), 0, st, a);
-  hipLaunchKernelGGL(k_adj_pair, dim3(g), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(k_adj_pair_g, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_rows(const double* in, long stride, int n, int rows, double scale,
+                              double* out, int accumulate, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_rows, dim3(rows), dim3(256), 0, st, in, stride, n, scale, out,
+                     accumulate);
   return hipGetLastError();
 }
 

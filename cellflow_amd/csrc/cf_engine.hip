@@ -1057,6 +1057,7 @@ struct Engine {
     double *partial = nullptr, *hd = nullptr, *gh = nullptr, *gacc = nullptr;
     float* thr = nullptr;
     int rc = CF_OK;
+    const DevBufs own = b;  // the engine's state buffers (the recompute re-points them)
     auto cleanup = [&]() {
       for (auto& s : ck) free_state(s);
       for (auto& s : seg) free_state(s);
@@ -1075,7 +1076,7 @@ struct Engine {
     };
     auto body = [&]() -> int {
       for (auto& s : ck) CFR(alloc_state(s));
-      for (auto& s : seg) CFR(alloc_state(s));
+      for (size_t i = 1; i < seg.size(); ++i) CFR(alloc_state(seg[i]));  // seg[0] is ck[sg]
       CFR(alloc_state(fin));
       for (int i = 0; i < 2; ++i) {
         CFR(dalloc(&lx[i], P));
@@ -1135,16 +1136,28 @@ struct Engine {
       aa.partial = partial;
       aa.stride = stride;
       int L = 0;  // lx[L] holds dL/dx of the state after the step being reversed
+      const int nblk = adj_pair_blocks(n);
+      aa.stride = nblk;
+      // Recompute writes each intermediate state once, where the backward sweep reads it:
+      // a step reads the engine's current buffers and writes the other ones, so those are
+      // pointed at the checkpoint and at the segment's buffers in turn (and restored after,
+      // also when a call fails: see below).
+      auto point = [&](int side, const StateBuf& s) {
+        b.pos[side] = s.pos;
+        b.vel[side] = s.vel;
+        b.id[side] = s.id;
+        b.start[side] = s.start;
+      };
       for (int sg = nseg - 1; sg >= 0; --sg) {
         const int k0 = sg * ckpt, k1 = std::min(nsteps, k0 + ckpt);
-        CFR(load_state(ck[sg]));
-        CFR(save_state(seg[0]));
+        point(cur, ck[sg]);
         for (int s = k0; s < k1; ++s) {
+          point(1 - cur, seg[s - k0 + 1]);
           CFR(step(false, 0));
-          CFR(save_state(seg[s - k0 + 1]));
         }
+        b = own;
         for (int s = k1 - 1; s >= k0; --s) {
-          const StateBuf& pre = seg[s - k0];
+          const StateBuf& pre = (s == k0) ? ck[sg] : seg[s - k0];
           const StateBuf& post = seg[s - k0 + 1];
           aa.pos = pre.pos;
           aa.vel = pre.vel;
@@ -1157,7 +1170,7 @@ struct Engine {
           aa.lam_x = lx[1 - L];
           aa.lam_v = lv[1 - L];
           HIPR(launch_adj_step(aa, n, st));
-          HIPR(launch_reduce_rows(partial, stride, n, T * T + 1, 1.0, gacc, 1, st));
+          HIPR(launch_reduce_rows(partial, nblk, nblk, T * T + 1, 1.0, gacc, 1, st));
           L = 1 - L;
         }
       }
@@ -1173,6 +1186,7 @@ struct Engine {
     };
     rc = body();
     (void)hipStreamSynchronize(st);
+    b = own;  // the engine's own state buffers, whatever the recompute pointed them at
     cleanup();
     return rc;
   }

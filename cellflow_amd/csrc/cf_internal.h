@@ -262,8 +262,8 @@ struct AdjArgs {
   double4* lam_x;
   double4* lam_v;
   double4* phi;
-  double* partial;          // [T*T + 1][stride] per-particle parameter partials
-  long stride;
+  double* partial;          // [T*T + 1][stride] parameter partials, one column per block
+  long stride;              // >= adj_pair_blocks(n)
 };
 
 hipError_t launch_hist(const HistArgs& a, int n, hipStream_t st);
@@ -271,6 +271,7 @@ hipError_t launch_hist_grad(const HistArgs& a, int n, hipStream_t st);
 hipError_t launch_reduce_rows(const double* in, long stride, int n, int rows, double scale,
                               double* out, int accumulate, hipStream_t st);
 hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st);
+int adj_pair_blocks(int n);  // blocks (columns of AdjArgs::partial) of one backward step
 
 // host-side launchers (cf_kernels.hip)
 hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,

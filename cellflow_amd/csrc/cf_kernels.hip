@@ -11,9 +11,10 @@
 //   k_scan          decoupled look-back exclusive scan of the cell histogram (and of the
 //                   per-owner edge counts), wave64 prefix scans inside each tile.
 //   k_fill          scatters each particle index into its new cell range (slot =
-//                   start + arrival) and writes the canonical edge list.
+//                   start + arrival).
 //   k_rank_scatter  ranks each particle by index inside its new cell and moves its state
-//                   there: the new order is (cell, index), independent of atomic order.
+//                   there: the new order is (cell, index), independent of atomic order;
+//                   also writes the canonical edge list.
 // The state lives in HBM sorted by (cell, index) as float4 SoA, so the stencil reads
 // contiguous runs of slots: for each (dy, dz) row the x-neighbour cells are adjacent in
 // memory.
@@ -53,6 +54,18 @@ __device__ __forceinline__ float wrap_pos(float x, float L) {
   if (y >= 0.0f && y < L) return y;
   if (y >= L && y < 2.0f * L) return y - L;
   return fmodf(y, L);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  This maps
+// the hardware block index to a logical one so that each XCD gets one contiguous range of
+// blocks (a bijection for any grid size): neighbouring particles, whose stencils read the
+// same slots, then share an L2.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_block() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+  return x * q + min(x, r) + b / kXcds;
 }
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
@@ -652,7 +665,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   __shared__ uint32_t el_n[4][kOwn];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int gw = blockIdx.x * (blockDim.x >> 6) + wv;   // wave index in units of kOwn slots
+  const int gw = xcd_block() * (blockDim.x >> 6) + wv;  // wave index in units of kOwn slots
   const long base = (long)gw * kOwn;
   const bool wave_ok = base < (long)a.nwaves * kWave;
   const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
@@ -1097,7 +1110,7 @@ __global__ void __launch_bounds__(256) k_force_list(ForceArgs a) {
   __shared__ uint32_t el_n[4][kOwn];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int gw = blockIdx.x * (blockDim.x >> 6) + wv;
+  const int gw = xcd_block() * (blockDim.x >> 6) + wv;
   const long base = (long)gw * kOwn;
   const bool wave_ok = base < (long)a.nwaves * kWave;
   const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
@@ -1497,12 +1510,6 @@ __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
   const uint32_t arr = a.arrival[p];
   a.tmp_id[a.start[c] + arr] = me;
   a.hist[c] = 0u;  // every writer stores 0; pad cells are never keys and keep their size
-  if (a.edges) {
-    const uint32_t off = a.eoff[me];
-    const int e = (int)(a.eoff[me + 1] - off);
-    for (int k = 0; k < e; ++k)
-      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
-  }
 }
 
 __global__ void __launch_bounds__(256) k_rank_scatter(SortArgs a) {
@@ -1532,6 +1539,14 @@ __global__ void __launch_bounds__(256) k_rank_scatter(SortArgs a) {
   a.dst_pos[dst] = sp;
   a.dst_vel[dst] = sv;
   a.dst_id[dst] = me;
+  // canonical edge list of the pre-step state (offsets from the same scan as the cells);
+  // its loads do not wait on the rank above
+  if (a.edges) {
+    const uint32_t off = a.eoff[me];
+    const int e = (int)(a.eoff[me + 1] - off);
+    for (int k = 0; k < e; ++k)
+      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
+  }
 }
 
 // Canonical edge list of the standalone edge pass: owner index ascending (offsets from

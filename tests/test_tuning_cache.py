@@ -93,8 +93,9 @@ _tunings = hs.fixed_dictionaries(dict(
 
 
 @settings(max_examples=60, deadline=None)
-@given(ents=hs.dictionaries(hs.text(hs.characters(blacklist_characters="\x00"), min_size=1,
-                                    max_size=12), hs.tuples(
+@given(ents=hs.dictionaries(hs.text(hs.characters(blacklist_categories=("Cs",),
+                                                  blacklist_characters="\x00"),
+                                    min_size=1, max_size=12), hs.tuples(
     _tunings, hs.floats(0, 1e7, allow_nan=False).map(lambda x: round(x, 3))), max_size=6),
     indent=hs.sampled_from([None, 0, 2]), ascii_=hs.booleans())
 def test_property_python_json_roundtrip(tmp_path_factory, ents, indent, ascii_):
