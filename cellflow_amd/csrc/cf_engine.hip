@@ -1308,8 +1308,8 @@ struct Engine {
                                                               orig.steps_per_graph});
       cands.push_back(cf_tuning{d, 2, orig.lanes_per_particle, orig.block_size,
                                 orig.steps_per_graph});
-      for (int g : {16, 8, 32})
-        if (g >= 8 * d) cands.push_back(cf_tuning{d, 3, g, orig.block_size, orig.steps_per_graph});
+      for (int g : {16, 8, 4, 32})
+        if (g >= 4 * d) cands.push_back(cf_tuning{d, 3, g, orig.block_size, orig.steps_per_graph});
     }
     const int span = orig.steps_per_graph >= 2 ? (orig.steps_per_graph + 1) / 2 * 2 : 2;
     const int nsteps = std::max(span, (std::max(1, trial_steps) + span - 1) / span * span);
@@ -1858,9 +1858,9 @@ static int check_tuning(const cf_tuning* t) {
   if (t->lanes_per_particle != 4 && t->lanes_per_particle != 8 && t->lanes_per_particle != 16 &&
       t->lanes_per_particle != 32)
     return cf::fail(CF_ERR_ARG, "lanes_per_particle must be 4, 8, 16 or 32");
-  if (t->force_variant == 3 && t->lanes_per_particle < 8 * t->cell_divisor)
-    return cf::fail(CF_ERR_ARG, "force_variant 3 needs lanes_per_particle >= 8 (cell_divisor 1) "
-                                "or >= 16 (cell_divisor 2)");
+  if (t->force_variant == 3 && t->lanes_per_particle < 4 * t->cell_divisor)
+    return cf::fail(CF_ERR_ARG, "force_variant 3 needs lanes_per_particle >= 8 with "
+                                "cell_divisor 2");
   if (t->steps_per_graph < 1) return cf::fail(CF_ERR_ARG, "steps_per_graph must be >= 1");
   return CF_OK;
 }

@@ -11,10 +11,9 @@
 //   k_scan          decoupled look-back exclusive scan of the cell histogram (and of the
 //                   per-owner edge counts), wave64 prefix scans inside each tile.
 //   k_fill          scatters each particle index into its new cell range (slot =
-//                   start + arrival).
+//                   start + arrival) and writes the canonical edge list.
 //   k_rank_scatter  ranks each particle by index inside its new cell and moves its state
-//                   there: the new order is (cell, index), independent of atomic order;
-//                   also writes the canonical edge list.
+//                   there: the new order is (cell, index), independent of atomic order.
 // The state lives in HBM sorted by (cell, index) as float4 SoA, so the stencil reads
 // contiguous runs of slots: for each (dy, dz) row the x-neighbour cells are adjacent in
 // memory.
@@ -1113,22 +1112,24 @@ __global__ void __launch_bounds__(256) k_force_list(ForceArgs a) {
   const int gw = xcd_block() * (blockDim.x >> 6) + wv;
   const long base = (long)gw * kOwn;
   const bool wave_ok = base < (long)a.nwaves * kWave;
-  const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
-  const SimDesc sd = a.sims[s];
-  if (wave_ok)
-    for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = a.pairs[sd.tab + t];
   const int o = lane / G, g = lane % G;
   const int p = (int)base + o;
-  const bool active = wave_ok && (p - sd.p0) < sd.n;
+  // the slot's state loads do not wait for the descriptor: every slot below nwaves * 64
+  // is allocated (a padding slot's values are loaded but never used)
   Own c;
   float4 vi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   uint32_t idi = 0;
   c.pi = vi;
-  if (active) {
+  if (wave_ok) {
     c.pi = a.pos[p];
     vi = a.vel[p];
     idi = a.id[p];
   }
+  const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
+  const SimDesc sd = a.sims[s];
+  if (wave_ok)
+    for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = a.pairs[sd.tab + t];
+  const bool active = wave_ok && (p - sd.p0) < sd.n;
   const float4 pi = c.pi;
   c.ti = active ? __float_as_uint(pi.w) : 0u;
   const uint32_t ti = c.ti;
@@ -1143,9 +1144,9 @@ __global__ void __launch_bounds__(256) k_force_list(ForceArgs a) {
   c.fyr = fminf(fmaxf(pi.y * sd.ihy - (float)c.cy, 0.0f), 1.0f);
   c.fzr = fminf(fmaxf(pi.z * sd.ihz - (float)c.cz, 0.0f), 1.0f);
   if (EDGES && g == 0) el_n[wv][o] = 0u;
-  __syncthreads();  // pair table ready
-  const PairConst* row = &tab[wv][ti * sd.T];
-  c.cut2 = row[0].pad;
+  // the pruning radius straight from the pair table in memory, so the stencil rows (and
+  // their cell-start loads) overlap the copy of the table into LDS
+  c.cut2 = active ? a.pairs[sd.tab + ti * sd.T].pad : 0.0f;
 
   // ---- this lane's runs, their place in the particle's flattened candidate order ----
   uint32_t rs[2 * kRpl], rl[2 * kRpl], rk[2 * kRpl];
@@ -1161,6 +1162,8 @@ __global__ void __launch_bounds__(256) k_force_list(ForceArgs a) {
   }
   const uint32_t first = group_excl_scan<G>(mine, g);
   const uint32_t total = (uint32_t)dpp_group_sum_i<G>((int)mine);
+  __syncthreads();  // pair table ready
+  const PairConst* row = &tab[wv][ti * sd.T];
 
   // ---- candidates, kList at a time -------------------------------------------------
   const bool do_edges = EDGES && a.edges && sd.edge_on && active;
@@ -1510,6 +1513,13 @@ __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
   const uint32_t arr = a.arrival[p];
   a.tmp_id[a.start[c] + arr] = me;
   a.hist[c] = 0u;  // every writer stores 0; pad cells are never keys and keep their size
+  // canonical edge list of the pre-step state (offsets from the same scan as the cells)
+  if (a.edges) {
+    const uint32_t off = a.eoff[me];
+    const int e = (int)(a.eoff[me + 1] - off);
+    for (int k = 0; k < e; ++k)
+      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
+  }
 }
 
 __global__ void __launch_bounds__(256) k_rank_scatter(SortArgs a) {
@@ -1539,14 +1549,6 @@ __global__ void __launch_bounds__(256) k_rank_scatter(SortArgs a) {
   a.dst_pos[dst] = sp;
   a.dst_vel[dst] = sv;
   a.dst_id[dst] = me;
-  // canonical edge list of the pre-step state (offsets from the same scan as the cells);
-  // its loads do not wait on the rank above
-  if (a.edges) {
-    const uint32_t off = a.eoff[me];
-    const int e = (int)(a.eoff[me + 1] - off);
-    for (int k = 0; k < e; ++k)
-      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
-  }
 }
 
 // Canonical edge list of the standalone edge pass: owner index ascending (offsets from
@@ -1782,12 +1784,14 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
   }
   if (variant == 3) {
     if (reach == 1) {
-      if (lanes == 8) launch_list<1, 8>(a, edges, st);
+      if (lanes == 4) launch_list<1, 4>(a, edges, st);
+      else if (lanes == 8) launch_list<1, 8>(a, edges, st);
       else if (lanes == 16) launch_list<1, 16>(a, edges, st);
       else if (lanes == 32) launch_list<1, 32>(a, edges, st);
       else return hipErrorInvalidValue;
     } else {
-      if (lanes == 16) launch_list<2, 16>(a, edges, st);
+      if (lanes == 8) launch_list<2, 8>(a, edges, st);
+      else if (lanes == 16) launch_list<2, 16>(a, edges, st);
       else if (lanes == 32) launch_list<2, 32>(a, edges, st);
       else return hipErrorInvalidValue;
     }

@@ -51,11 +51,11 @@ static inline bool int_field(const JVal& e, const char* k, int* out) {
 }
 
 // The values cf_set_tuning accepts (variant 3 needs at least 8 lanes per particle for the
-// 27-cell stencil and 16 for the 125-cell one: at most two stencil rows per lane).
+// 125-cell stencil: at most four stencil rows per lane).
 static inline bool valid(const cf_tuning& t) {
   const bool lanes_ok = t.lanes_per_particle == 4 || t.lanes_per_particle == 8 ||
                         t.lanes_per_particle == 16 || t.lanes_per_particle == 32;
-  const bool list_ok = t.force_variant != 3 || t.lanes_per_particle >= 8 * t.cell_divisor;
+  const bool list_ok = t.force_variant != 3 || t.lanes_per_particle >= 4 * t.cell_divisor;
   return (t.cell_divisor == 1 || t.cell_divisor == 2) &&
          (t.block_size == 64 || t.block_size == 128 || t.block_size == 256) &&
          (t.force_variant >= 0 && t.force_variant <= 3) && lanes_ok && list_ok &&

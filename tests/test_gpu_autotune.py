@@ -27,7 +27,7 @@ def test_autotune_leaves_state_and_matches_direct_tuning(tmp_path, edges):
     res = sim.autotune(params, edges=edges, trial_steps=8)
     assert snapshot(sim) == before
     trials = res["trials"]
-    assert len(trials) == 21 and not res["cached"]
+    assert len(trials) == 23 and not res["cached"]
     timed = [t for t in trials if not t["pruned"]]
     best = min(timed, key=lambda t: t["us_per_step"])
     assert res["us_per_step"] == pytest.approx(best["us_per_step"], rel=1e-3)
@@ -76,7 +76,7 @@ def test_cache_hit_miss_and_force(tmp_path):
     expect = dict(r1["tuning"], steps_per_graph=8)  # the caller's graph length is kept
     assert r2["tuning"] == expect and b.getTuning() == expect
     r3 = b.autotune(params, trial_steps=4, cache=path, force=True)
-    assert not r3["cached"] and len(r3["trials"]) == 21
+    assert not r3["cached"] and len(r3["trials"]) == 23
 
 
 def test_ensemble_autotune_leaves_state():

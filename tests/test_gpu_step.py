@@ -273,7 +273,7 @@ def test_edge_capacity_error():
 # every combination the autotuner times: (variant, lanes, divisor, block size)
 VARIANTS = [(0, g, d, 128) for d in (1, 2) for g in (4, 8, 16, 32)] + \
     [(1, 16, d, b) for d in (1, 2) for b in (64, 128, 256)] + [(2, 16, 1, 128), (2, 16, 2, 128)] + \
-    [(3, g, 1, 128) for g in (8, 16, 32)] + [(3, g, 2, 128) for g in (16, 32)]
+    [(3, g, 1, 128) for g in (4, 8, 16, 32)] + [(3, g, 2, 128) for g in (8, 16, 32)]
 
 
 @pytest.mark.parametrize("variant,lanes,divisor,block", VARIANTS)
