@@ -974,7 +974,8 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
 // kList), and the G lanes then take list entries g, g+G, ... with one loop body for
 // shifted and unshifted runs alike.  Group sums use DPP mirror butterflies.
 
-constexpr int kList = 64;
+constexpr int kList = 32;       // candidate slots per particle per chunk
+constexpr int kListEdges = 32;  // LDS edge candidates per particle (more: streaming selection)
 
 // Sum over aligned groups of G lanes (G in 4, 8, 16, 32), every lane gets its group's sum.
 template <int G>
@@ -1097,15 +1098,16 @@ __device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, 
 }
 
 template <bool EDGES, int R, int G>
-__global__ void __launch_bounds__(256) k_force_list(ForceArgs a) {
+__global__ void __launch_bounds__(256)
+k_force_list(ForceArgs a) {
   constexpr int kRows = (2 * R + 1) * (2 * R + 1);
   constexpr int kOwn = kWave / G;
   constexpr int kRpl = (kRows + G - 1) / G;  // stencil rows per lane
   __shared__ PairConst tab[4][kMaxTypes * kMaxTypes];
   __shared__ uint32_t lslot[4][kOwn][kList];
   __shared__ uint8_t lcode[4][kOwn][kList];
-  __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
-  __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
+  __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
+  __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_n[4][kOwn];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1223,7 +1225,7 @@ __global__ void __launch_bounds__(256) k_force_list(ForceArgs a) {
           const uint32_t jid = a.id[q];
           if (jid > idi) {
             const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
-            if (kk < (uint32_t)kEdgeList) {
+            if (kk < (uint32_t)kListEdges) {
               el_d[wv][o][kk] = make_float2(s2, 0.0f);
               el_i[wv][o][kk] = jid;
             }
@@ -1243,7 +1245,7 @@ __global__ void __launch_bounds__(256) k_force_list(ForceArgs a) {
     // barriers stay outside lane-dependent branches
     __syncthreads();
     const uint32_t n = do_edges ? el_n[wv][o] : 0u;
-    const bool fits = n <= (uint32_t)kEdgeList;
+    const bool fits = n <= (uint32_t)kListEdges;
     const int cap = a.edge_cap, maxc = sd.edge_maxc;
     if (fits)
       for (uint32_t k = g; k < n; k += G) {
