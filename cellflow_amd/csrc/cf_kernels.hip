@@ -975,7 +975,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
 // shifted and unshifted runs alike.  Group sums use DPP mirror butterflies.
 
 constexpr int kList = 32;       // candidate slots per particle per chunk
-constexpr int kListEdges = 32;  // LDS edge candidates per particle (more: streaming selection)
+constexpr int kListEdges = 24;  // LDS edge candidates per particle (more: streaming selection)
 
 // Sum over aligned groups of G lanes (G in 4, 8, 16, 32), every lane gets its group's sum.
 template <int G>
@@ -1098,7 +1098,8 @@ __device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, 
 }
 
 template <bool EDGES, int R, int G>
-__global__ void __launch_bounds__(256)
+// 27-cell stencil at G = 8 fits 7 waves per SIMD (72 VGPRs, under 22.8 KB of LDS per block)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 1 && G == 8 ? 7 : 1, 8)))
 k_force_list(ForceArgs a) {
   constexpr int kRows = (2 * R + 1) * (2 * R + 1);
   constexpr int kOwn = kWave / G;

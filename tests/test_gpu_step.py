@@ -124,7 +124,7 @@ def test_edges_truncation_matches_oracle(orc, maxc):
         assert np.array_equal(edges, ref), f"dist={dist} maxc={maxc}"
 
 
-@pytest.mark.parametrize("variant,lanes", [(0, 4), (0, 16), (3, 16), (3, 32)])
+@pytest.mark.parametrize("variant,lanes", [(0, 4), (0, 16), (3, 8), (3, 16), (3, 32)])
 def test_fused_edges_dense_blobs(orc, variant, lanes):
     """Fused edge candidates on dense blobs: many owners exceed the per-particle LDS edge
     list and take the streaming fallback (and, in variant 3, hold several candidate-list
