@@ -84,16 +84,8 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ 
 
 // ---- one backward step -----------------------------------------------------------------
 
-// counts c_i of the step (stored as the "previous count" of the post-step state), by index
-__global__ void k_adj_counts(AdjArgs a) {
-  const SimDesc sd = a.sims[0];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sd.n) return;
-  const int p = sd.p0 + k;
-  a.cnt_by_id[a.id_next[p]] = __float_as_int(a.vel_next[p].w);
-}
-
-// phi_i = dt m_i (lv'_i + dt lx'_i),  lv_i = fr (lv'_i + dt lx'_i)
+// phi_i = dt m_i (lv'_i + dt lx'_i),  lv_i = fr (lv'_i + dt lx'_i); phi is stored by the
+// pre-step slot, where the pair kernel finds both ends of a pair without an index lookup
 __global__ void k_adj_phi(AdjArgs a) {
   const SimDesc sd = a.sims[0];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,7 +93,7 @@ __global__ void k_adj_phi(AdjArgs a) {
   const int p = sd.p0 + k;
   const uint32_t id = a.id[p];
   const int prev = __float_as_int(a.vel[p].w);
-  const int cnt = a.cnt_by_id[id];
+  const int cnt = __float_as_int(a.tvel[p].w);  // this step's count, by pre-step slot
   // the forward's FP32 factor (ParticleSimulation.cu:136-143)
   const float avg = (float)(cnt + prev) * 0.5f;
   const float dens = fminf(avg / sd.max_exp, 1.0f);
@@ -110,7 +102,7 @@ __global__ void k_adj_phi(AdjArgs a) {
   const double4 lx = a.lam_x_next[id], lv = a.lam_v_next[id];
   const double dt = (double)sd.dt, fr = (double)sd.friction;
   const double mx = lv.x + dt * lx.x, my = lv.y + dt * lx.y, mz = lv.z + dt * lx.z;
-  a.phi[id] = make_double4(dt * m * mx, dt * m * my, dt * m * mz, 0.0);
+  a.phi[p] = make_double4(dt * m * mx, dt * m * my, dt * m * mz, 0.0);  // by pre-step slot
   a.lam_v[id] = make_double4(fr * mx, fr * my, fr * mz, 0.0);
 }
 
@@ -154,17 +146,26 @@ __device__ __forceinline__ void for_cells_split(const SimDesc& sd, const uint32_
   }
 }
 
-// Histogram partials, G lanes per particle: per-lane bins in LDS, then a fixed-order sum
-// over the group's lanes.  A pair is skipped early when its FP32 squared distance exceeds
-// the reach squared by more than FP32 rounding can explain.
-__global__ void __launch_bounds__(128) k_hist_g(HistArgs a) {
-  constexpr int G = kAdjG;
-  __shared__ double part[128][kMaxBins];
+// Histogram partials, G lanes per particle.  The group's lanes add each pair's four spline
+// weights into the particle's bins in LDS as 64-bit fixed point, rounded once per weight:
+// integer addition does not depend on order, so the result is deterministic whatever the
+// lanes' interleaving.  The unit is 2^-(62 - bitlen(n)): a weight is at most 1 and a bin
+// gathers at most n - 1 of them, so no sum can overflow, and at n = 20k one weight is held
+// to 2^-47.  A pair is skipped early when its FP32 squared distance exceeds the reach
+// squared by more than FP32 rounding can explain.
+constexpr int kHistG = 32;
+constexpr int kHistBlock = 256;
+
+__global__ void __launch_bounds__(kHistBlock) k_hist_g(HistArgs a) {
+  constexpr int G = kHistG, kPer = kHistBlock / G;
+  __shared__ unsigned long long hq[kPer * kMaxBins];
   const SimDesc sd = a.sims[0];
   const int o = threadIdx.x / G, g = threadIdx.x % G;
-  const int k = blockIdx.x * (128 / G) + o;
-  double* h = part[threadIdx.x];
-  for (int b = 0; b < a.bins; ++b) h[b] = 0.0;
+  const int k = blockIdx.x * kPer + o;
+  for (int i = threadIdx.x; i < kPer * kMaxBins; i += kHistBlock) hq[i] = 0ull;
+  __syncthreads();
+  unsigned long long* h = &hq[o * kMaxBins];
+  const double unit = ldexp(1.0, 62 - (32 - __clz(sd.n)));
   const double w = a.r_max / a.bins, reach = a.r_max + 2.0 * w;
   const float reach2f = (float)(reach * reach * (1.0 + 1e-5));
   if (k < sd.n) {
@@ -185,21 +186,18 @@ __global__ void __launch_bounds__(128) k_hist_g(HistArgs a) {
       if (r >= reach) return;
       const double u = r / w - 0.5;
       const int b0 = (int)floor(u) - 1;
-      for (int b = max(b0, 0); b <= min(b0 + 3, a.bins - 1); ++b) h[b] += bspline3(u - b);
+      for (int b = max(b0, 0); b <= min(b0 + 3, a.bins - 1); ++b)
+        atomicAdd(&h[b], (unsigned long long)llrint(bspline3(u - b) * unit));
     });
   }
   __syncthreads();
   if (k < sd.n)
-    for (int b = g; b < a.bins; b += G) {
-      double acc = 0.0;
-      for (int l = 0; l < G; ++l) acc += part[o * G + l][b];
-      a.partial[(long)b * a.stride + k] = acc;
-    }
+    for (int b = g; b < a.bins; b += G) a.partial[(long)b * a.stride + k] = (double)h[b] / unit;
 }
 
 // lx of the final state, G lanes per particle (fixed-order group sum by shuffles).
 __global__ void __launch_bounds__(128) k_hist_grad_g(HistArgs a) {
-  constexpr int G = kAdjG;
+  constexpr int G = kHistG;
   const SimDesc sd = a.sims[0];
   const int o = threadIdx.x / G, g = threadIdx.x % G;
   const int k = blockIdx.x * (128 / G) + o;
@@ -248,13 +246,42 @@ __global__ void __launch_bounds__(128) k_hist_grad_g(HistArgs a) {
   }
 }
 
-// One backward step's pair terms, G lanes per particle.  Per particle: lane sums reduced in
-// lane order; per block: the table and ratio partials of its particles summed in particle
-// order into one column of `partial` (row r at partial[r * stride + block]).
+// Exclusive prefix sum over aligned groups of G lanes (g = lane index in the group); every
+// lane of the wave takes part.
+template <int G>
+__device__ __forceinline__ uint32_t group_excl_scan(uint32_t v, int g) {
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (g >= d) x += y;
+  }
+  return x - v;
+}
+
+// One backward step's pair terms, G lanes per particle.  Few of a particle's stencil
+// candidates are in range (about 1 in 15 on a uniform state), so evaluating the FP64 terms
+// where they are found would run the FP64 body on nearly every step of the walk for some
+// lane of the wave.  Instead the walk only takes the forward's FP32 decision and appends
+// the in-range slots to the particle's list in LDS; the list is put in slot order (rank
+// by counting, slots are distinct) and its entries are evaluated densely, entry e by lane
+// e mod G.  A particle with more than kAdjHits pairs takes them in windows of kAdjHits in
+// lane-major walk order (lane g's pairs after those of lanes below g), one more walk per
+// window.
+// Per particle: lane sums reduced in lane order; per block: the table and ratio partials
+// of its particles summed in particle order and added to one column of `partial` (row r at
+// partial[r * stride + block]), which so sums the steps of the rollout in the order they are
+// reversed.  Every order is fixed by the data, so the results are
+// deterministic.
+constexpr int kAdjHits = 64;
+
 __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
   constexpr int G = kAdjG;
   __shared__ double red[kAdjBlock][kAdjVals];
   __shared__ int btype[kAdjPer];
+  __shared__ uint32_t hit[kAdjPer][kAdjHits];
+  __shared__ uint32_t hsorted[kAdjPer][kAdjHits];
+  __shared__ uint32_t nhit[kAdjPer];
   const SimDesc sd = a.sims[0];
   const int o = threadIdx.x / G, g = threadIdx.x % G;
   const int k = blockIdx.x * kAdjPer + o;
@@ -265,50 +292,97 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
 #pragma unroll
   for (int t = 0; t < kMaxTypes; ++t) gfv[t] = 0.0;
   const int p = sd.p0 + k;
-  int ti = -1;
+  if (g == 0) nhit[o] = 0u;
+  float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   uint32_t idi = 0;
+  int ti = -1, cx = 0, cy = 0, cz = 0;
+  double4 phi_i = make_double4(0.0, 0.0, 0.0, 0.0);
   if (active) {
-    const float4 pi = a.pos[p];
+    pi = a.pos[p];
     idi = a.id[p];
     ti = (int)__float_as_uint(pi.w);
-    const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
-              cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
-    const double4 phi_i = a.phi[idi];
+    cx = bin_axis_a(pi.x, sd.ihx, sd.nx);
+    cy = bin_axis_a(pi.y, sd.ihy, sd.ny);
+    cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
+    phi_i = a.phi[p];
+  }
+  // the forward's FP32 decision (ParticleSimulation.cu:92-112 with the engine's op order)
+  auto in_range = [&](uint32_t q) {
+    if ((int)q == p) return false;
+    const float4 pj = a.pos[q];
+    const int tj = (int)__float_as_uint(pj.w);
+    const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
+                fz = min_image_f(pj.z - pi.z, sd.Lz);
+    const float s2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+    return s2f + 1e-4f < a.thr[ti * T + tj];
+  };
+  // the FP64 terms of one in-range pair
+  auto pair = [&](uint32_t q) {
     const double R = a.c.radius, rho = a.c.ratio, rep = a.c.rep, att = a.c.att, kk = a.c.k;
     const double eps = (double)1e-4f;
-    for_cells_split(sd, a.start, cx, cy, cz, sd.reach, g, G, [&](uint32_t q) {
-      if ((int)q == p) return;
-      const float4 pj = a.pos[q];
-      const int tj = (int)__float_as_uint(pj.w);
-      // the forward's FP32 decision (ParticleSimulation.cu:92-112 with the engine's op order)
-      const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
-                  fz = min_image_f(pj.z - pi.z, sd.Lz);
-      const float s2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-      if (!(s2f + 1e-4f < a.thr[ti * T + tj])) return;
-      const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
-      const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
-      const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
-      const double dist = sqrt(dx * dx + dy * dy + dz * dz + eps);
-      const double dR = 0.5 * R * (a.c.rbt[ti] + a.c.rbt[tj]);
-      const double eff = R + dR * rho;
-      const double r = dist / eff;
-      const double ex = exp(-kk * r * r);
-      const double gr = rep * ex - att * r;
-      const double gp = -2.0 * kk * r * rep * ex - att;
-      const double h = gr / dist;
-      const double hp = gp / (eff * dist) - gr / (dist * dist);
-      const double4 phi_j = a.phi[a.id[q]];
-      const double fij = a.c.table[ti * T + tj], fji = a.c.table[tj * T + ti];
-      const double dpi = (dx * phi_i.x + dy * phi_i.y + dz * phi_i.z) / dist;
-      const double dpj = (dx * phi_j.x + dy * phi_j.y + dz * phi_j.z) / dist;
-      lx += fji * (h * phi_j.x + hp * dx * dpj) - fij * (h * phi_i.x + hp * dx * dpi);
-      ly += fji * (h * phi_j.y + hp * dy * dpj) - fij * (h * phi_i.y + hp * dy * dpi);
-      lz += fji * (h * phi_j.z + hp * dz * dpj) - fij * (h * phi_i.z + hp * dz * dpi);
-      const double gd = gr * dpi;
+    const float4 pj = a.pos[q];
+    const int tj = (int)__float_as_uint(pj.w);
+    const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
+    const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
+    const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz + eps);
+    const double dR = 0.5 * R * (a.c.rbt[ti] + a.c.rbt[tj]);
+    const double eff = R + dR * rho;
+    const double r = dist / eff;
+    const double ex = exp(-kk * r * r);
+    const double gr = rep * ex - att * r;
+    const double gp = -2.0 * kk * r * rep * ex - att;
+    const double h = gr / dist;
+    const double hp = gp / (eff * dist) - gr / (dist * dist);
+    const double4 phi_j = a.phi[q];
+    const double fij = a.c.table[ti * T + tj], fji = a.c.table[tj * T + ti];
+    const double dpi = (dx * phi_i.x + dy * phi_i.y + dz * phi_i.z) / dist;
+    const double dpj = (dx * phi_j.x + dy * phi_j.y + dz * phi_j.z) / dist;
+    lx += fji * (h * phi_j.x + hp * dx * dpj) - fij * (h * phi_i.x + hp * dx * dpi);
+    ly += fji * (h * phi_j.y + hp * dy * dpj) - fij * (h * phi_i.y + hp * dy * dpi);
+    lz += fji * (h * phi_j.z + hp * dz * dpj) - fij * (h * phi_i.z + hp * dz * dpi);
+    const double gd = gr * dpi;
 #pragma unroll
-      for (int t = 0; t < kMaxTypes; ++t) gfv[t] += (t == tj) ? gd : 0.0;
-      grat += fij * (-gp * r / eff) * dR * dpi;
+    for (int t = 0; t < kMaxTypes; ++t) gfv[t] += (t == tj) ? gd : 0.0;
+    grat += fij * (-gp * r / eff) * dR * dpi;
+  };
+  __syncthreads();  // list counters cleared
+  uint32_t nmine = 0u;  // this lane's pairs
+  if (active)
+    for_cells_split(sd, a.start, cx, cy, cz, sd.reach, g, G, [&](uint32_t q) {
+      if (!in_range(q)) return;
+      ++nmine;
+      const uint32_t at = atomicAdd(&nhit[o], 1u);
+      if (at < (uint32_t)kAdjHits) hit[o][at] = q;
     });
+  const uint32_t first = group_excl_scan<G>(nmine, g);
+  __syncthreads();
+  const uint32_t nh = active ? nhit[o] : 0u;
+  const bool listed = nh <= (uint32_t)kAdjHits;
+  if (listed)
+    for (uint32_t e = g; e < nh; e += G) {
+      const uint32_t q = hit[o][e];
+      uint32_t rank = 0;
+      for (uint32_t e2 = 0; e2 < nh; ++e2) rank += (hit[o][e2] < q) ? 1u : 0u;
+      hsorted[o][rank] = q;
+    }
+  __syncthreads();
+  // a group's lanes share one wave: LDS writes and reads between them are ordered by
+  // wavefront fences, so the windows need no block barrier
+  for (uint32_t w0 = 0; w0 < nh; w0 += (uint32_t)kAdjHits) {
+    if (!listed) {
+      uint32_t idx = first;
+      for_cells_split(sd, a.start, cx, cy, cz, sd.reach, g, G, [&](uint32_t q) {
+        if (!in_range(q)) return;
+        if (idx >= w0 && idx < w0 + (uint32_t)kAdjHits) hsorted[o][idx - w0] = q;
+        ++idx;
+      });
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    const uint32_t nb = min(nh - w0, (uint32_t)kAdjHits);
+#pragma nounroll
+    for (uint32_t e = g; e < nb; e += G) pair(hsorted[o][e]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // window reused by the next
   }
   double* mine = red[threadIdx.x];
   mine[0] = lx;
@@ -345,27 +419,26 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
       for (int oo = 0; oo < kAdjPer; ++oo)
         if (btype[oo] >= 0) acc += red[oo * G][3];
     }
-    a.partial[(long)row * a.stride + blockIdx.x] = acc;
+    a.partial[(long)row * a.stride + blockIdx.x] += acc;
   }
 }
 
 int adj_pair_blocks(int n) { return (n + kAdjPer - 1) / kAdjPer; }
 
 hipError_t launch_hist(const HistArgs& a, int n, hipStream_t st) {
-  const int g = (n + (128 / kAdjG) - 1) / (128 / kAdjG);
-  hipLaunchKernelGGL(k_hist_g, dim3(g), dim3(128), 0, st, a);
+  constexpr int per = kHistBlock / kHistG;
+  hipLaunchKernelGGL(k_hist_g, dim3((n + per - 1) / per), dim3(kHistBlock), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_hist_grad(const HistArgs& a, int n, hipStream_t st) {
-  const int g = (n + (128 / kAdjG) - 1) / (128 / kAdjG);
-  hipLaunchKernelGGL(k_hist_grad_g, dim3(g), dim3(128), 0, st, a);
+  constexpr int per = 128 / kHistG;
+  hipLaunchKernelGGL(k_hist_grad_g, dim3((n + per - 1) / per), dim3(128), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st) {
   const int g = (n + 127) / 128;
-  hipLaunchKernelGGL(k_adj_counts, dim3(g), dim3(128), 0, st, a);
   hipLaunchKernelGGL(k_adj_phi, dim3(g), dim3(128), 0, st, a);
   hipLaunchKernelGGL(k_adj_pair_g, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
   return hipGetLastError();

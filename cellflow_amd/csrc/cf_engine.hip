@@ -1051,9 +1051,9 @@ struct Engine {
     const long stride = ((long)n + 127) / 128 * 128;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
     std::vector<StateBuf> ck(std::max(nseg, 1)), seg(ckpt + 1);
+    std::vector<float4*> tv(ckpt, nullptr);  // each segment step's new velocities + counts
     StateBuf fin;
     double4 *lx[2] = {nullptr, nullptr}, *lv[2] = {nullptr, nullptr}, *phi = nullptr;
-    int* cnt = nullptr;
     double *partial = nullptr, *hd = nullptr, *gh = nullptr, *gacc = nullptr;
     float* thr = nullptr;
     int rc = CF_OK;
@@ -1061,13 +1061,13 @@ struct Engine {
     auto cleanup = [&]() {
       for (auto& s : ck) free_state(s);
       for (auto& s : seg) free_state(s);
+      for (auto& t : tv) dfree(t);
       free_state(fin);
       for (int i = 0; i < 2; ++i) {
         dfree(lx[i]);
         dfree(lv[i]);
       }
       dfree(phi);
-      dfree(cnt);
       dfree(partial);
       dfree(hd);
       dfree(gh);
@@ -1077,13 +1077,13 @@ struct Engine {
     auto body = [&]() -> int {
       for (auto& s : ck) CFR(alloc_state(s));
       for (size_t i = 1; i < seg.size(); ++i) CFR(alloc_state(seg[i]));  // seg[0] is ck[sg]
+      for (auto& t : tv) CFR(dalloc(&t, P));
       CFR(alloc_state(fin));
       for (int i = 0; i < 2; ++i) {
         CFR(dalloc(&lx[i], P));
         CFR(dalloc(&lv[i], P));
       }
       CFR(dalloc(&phi, P));
-      CFR(dalloc(&cnt, P));
       const int rows = std::max(bins, T * T + 1);
       CFR(dalloc(&partial, (size_t)rows * stride));
       CFR(dalloc(&hd, bins));
@@ -1094,12 +1094,28 @@ struct Engine {
       for (int i = 0; i < T * T; ++i) hthr[i] = hpairs[hdesc[0].tab + i].thr;
       HIPR(hipMemcpyAsync(thr, hthr.data(), T * T * sizeof(float), hipMemcpyHostToDevice, st));
       HIPR(hipMemsetAsync(b.flag, 0, sizeof(uint32_t), st));
-      // forward with checkpoints
+      // Each step reads the engine's current buffers and writes the other ones (its new
+      // velocities and counts to tvel, by pre-step slot).  Where a step's states are kept,
+      // those are pointed at the kept buffers; the engine's own are restored after, also
+      // when a call fails (see below).
+      auto point = [&](int side, const StateBuf& s) {
+        b.pos[side] = s.pos;
+        b.vel[side] = s.vel;
+        b.id[side] = s.id;
+        b.start[side] = s.start;
+      };
+      // forward with checkpoints; the last segment's states are kept as they are made
+      const int last0 = (nseg - 1) * ckpt;
       for (int s = 0; s < nsteps; ++s) {
         if (s % ckpt == 0) CFR(save_state(ck[s / ckpt]));
+        if (s >= last0) {
+          point(1 - cur, seg[s - last0 + 1]);
+          b.tvel = tv[s - last0];
+        }
         CFR(step(false, 0));
       }
       CFR(save_state(fin));
+      b = own;
       // loss of the final state
       HIPR(launch_hist(hist_args(fin, bins, r_max, partial, stride), n, st));
       HIPR(launch_reduce_rows(partial, stride, n, bins, 1.0 / n, hd, 0, st));
@@ -1119,7 +1135,8 @@ struct Engine {
       ha.lam_x = lx[0];
       ha.lam_v = lv[0];
       HIPR(launch_hist_grad(ha, n, st));
-      HIPR(hipMemsetAsync(gacc, 0, (T * T + 1) * sizeof(double), st));
+      const int nblk = adj_pair_blocks(n);
+      HIPR(hipMemsetAsync(partial, 0, (size_t)(T * T + 1) * nblk * sizeof(double), st));
       AdjArgs aa{};
       aa.sims = b.sims;
       aa.thr = thr;
@@ -1131,49 +1148,37 @@ struct Engine {
       aa.c.k = h.prm.k;
       for (int t = 0; t < T; ++t) aa.c.rbt[t] = h.rbt[t];
       for (int t = 0; t < T * T; ++t) aa.c.table[t] = h.table[t];
-      aa.cnt_by_id = cnt;
       aa.phi = phi;
       aa.partial = partial;
-      aa.stride = stride;
-      int L = 0;  // lx[L] holds dL/dx of the state after the step being reversed
-      const int nblk = adj_pair_blocks(n);
       aa.stride = nblk;
-      // Recompute writes each intermediate state once, where the backward sweep reads it:
-      // a step reads the engine's current buffers and writes the other ones, so those are
-      // pointed at the checkpoint and at the segment's buffers in turn (and restored after,
-      // also when a call fails: see below).
-      auto point = [&](int side, const StateBuf& s) {
-        b.pos[side] = s.pos;
-        b.vel[side] = s.vel;
-        b.id[side] = s.id;
-        b.start[side] = s.start;
-      };
+      int L = 0;  // lx[L] holds dL/dx of the state after the step being reversed
       for (int sg = nseg - 1; sg >= 0; --sg) {
         const int k0 = sg * ckpt, k1 = std::min(nsteps, k0 + ckpt);
-        point(cur, ck[sg]);
-        for (int s = k0; s < k1; ++s) {
-          point(1 - cur, seg[s - k0 + 1]);
-          CFR(step(false, 0));
+        if (sg != nseg - 1) {  // recompute the segment from its checkpoint
+          point(cur, ck[sg]);
+          for (int s = k0; s < k1; ++s) {
+            point(1 - cur, seg[s - k0 + 1]);
+            b.tvel = tv[s - k0];
+            CFR(step(false, 0));
+          }
+          b = own;
         }
-        b = own;
         for (int s = k1 - 1; s >= k0; --s) {
           const StateBuf& pre = (s == k0) ? ck[sg] : seg[s - k0];
-          const StateBuf& post = seg[s - k0 + 1];
           aa.pos = pre.pos;
           aa.vel = pre.vel;
           aa.id = pre.id;
           aa.start = pre.start;
-          aa.vel_next = post.vel;
-          aa.id_next = post.id;
+          aa.tvel = tv[s - k0];
           aa.lam_x_next = lx[L];
           aa.lam_v_next = lv[L];
           aa.lam_x = lx[1 - L];
           aa.lam_v = lv[1 - L];
           HIPR(launch_adj_step(aa, n, st));
-          HIPR(launch_reduce_rows(partial, nblk, nblk, T * T + 1, 1.0, gacc, 1, st));
           L = 1 - L;
         }
       }
+      HIPR(launch_reduce_rows(partial, nblk, nblk, T * T + 1, 1.0, gacc, 0, st));
       std::vector<double> gg(T * T + 1);
       HIPR(hipMemcpyAsync(gg.data(), gacc, (T * T + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
       CFR(load_state(fin));

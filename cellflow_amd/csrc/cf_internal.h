@@ -251,18 +251,17 @@ struct AdjArgs {
   const float4* vel;
   const uint32_t* id;
   const uint32_t* start;
-  const float4* vel_next;   // post-step state S_k+1 (its previous counts are this step's counts)
-  const uint32_t* id_next;
+  const float4* tvel;       // the step's new velocities by pre-step slot, .w = its count bits
   const SimDesc* sims;
   const float* thr;         // T*T cutoff thresholds of the forward
   AdjConst c;
-  int* cnt_by_id;
   const double4* lam_x_next;
   const double4* lam_v_next;
   double4* lam_x;
   double4* lam_v;
   double4* phi;
-  double* partial;          // [T*T + 1][stride] parameter partials, one column per block
+  double* partial;          // [T*T + 1][stride] parameter partials, one column per block,
+                            // added to (the caller zeroes them once per rollout)
   long stride;              // >= adj_pair_blocks(n)
 };
 

@@ -115,6 +115,24 @@ def test_adjoint_matches_finite_differences_small(steps, ckpt):
     check_against_fd(small_cfg(), steps, ckpt, 16, 0.1)
 
 
+def test_adjoint_dense_state_pairs_beyond_the_lds_list():
+    """A state dense enough that many particles have more in-range pairs than the pair
+    kernel's 64-entry LDS list (they take the windowed path) and others fewer."""
+    n = 3000
+    cfg = small_cfg(n=n)
+    cfg.update(canvas=[0.31] * 3)
+    st, ph = setup(cfg)
+    offs, _ = oracle.pairs(st["pos"], st["type"], ph, st["rbt"])
+    per = np.diff(offs)
+    assert (per > 64).sum() > n // 10 and (per <= 64).sum() > n // 10, (per.min(), per.max())
+    g = check_against_fd(cfg, 12, 5, 16, 0.1)
+    # and the windowed path is as deterministic as the listed one
+    tgt = target_hist(cfg, 12, 16, 0.1)
+    _, _, gt, gr = make_sim(st, ph).rolloutGrad(SimulationParams.from_physics(ph), 12, 5, 16, 0.1,
+                                                tgt)
+    assert np.concatenate([gt.ravel(), [gr]]).tobytes() == g.tobytes()
+
+
 def test_checkpoint_interval_and_repeat_bit_identical():
     cfg = small_cfg(n=2000)
     st, ph = setup(cfg)
