@@ -54,7 +54,8 @@ class cf_edge_opts(C.Structure):
 class cf_tuning(C.Structure):
     _fields_ = [("cell_divisor", C.c_int), ("force_variant", C.c_int),
                 ("lanes_per_particle", C.c_int), ("block_size", C.c_int),
-                ("steps_per_graph", C.c_int)]
+                ("steps_per_graph", C.c_int),
+                ("sort_variant", C.c_int)]
 
 
 class cf_preset(C.Structure):

@@ -170,7 +170,18 @@ struct Engine {
   long long edges_count = -1;  // -1: no edge list built
   int edges_maxc = 0;
   uint32_t* slot_of = nullptr;
-  cf_tuning tun{1, 0, 16, 256, 32};
+  // tile sort of the step: per-parity counters (the step with state buffer A counts into
+  // parity A and clears parity B for the next step), far list, simulation of each tile
+  uint32_t* tcnt[2] = {nullptr, nullptr};
+  uint32_t* etcnt[2] = {nullptr, nullptr};
+  uint32_t* farn = nullptr;  // two words
+  uint32_t* far_slot = nullptr;
+  uint32_t* ftn[2] = {nullptr, nullptr};  // far movers per tile
+  uint32_t* ftl = nullptr;
+  uint32_t* tile_sim = nullptr;
+  int ntiles = 0, ntiles_cap = 0, netiles = 0;
+  std::vector<uint32_t> htile_sim;
+  cf_tuning tun{1, 0, 16, 256, 32, 1};
   uint32_t* hsmall = nullptr;  // pinned readback words
   // Captured step graphs.  A graph bakes kernel arguments (buffer pointers, sizes, tuning),
   // so the cache is keyed on them and dropped whenever a buffer is reallocated.
@@ -203,6 +214,12 @@ struct Engine {
     dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.eoff);
     dfree(b.edges); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
     dfree(slot_of); dfree(tickets);
+    for (int i = 0; i < 2; ++i) {
+      dfree(tcnt[i]);
+      dfree(etcnt[i]);
+      dfree(ftn[i]);
+    }
+    dfree(farn); dfree(far_slot); dfree(tile_sim); dfree(ftl);
     for (auto& f : flags) dfree(f);
     if (hsmall) (void)hipHostFree(hsmall);
     hsmall = nullptr;
@@ -243,6 +260,10 @@ struct Engine {
     CFR(dalloc(&b.sims, nsims)); CFR(dalloc(&b.wave_sim, nwaves));
     CFR(dalloc(&b.pairs, (size_t)nsims * kMaxTypes * kMaxTypes));
     CFR(dalloc(&b.flag, 4)); CFR(dalloc(&tickets, 3)); CFR(dalloc(&slot_of, P));
+    netiles = (int)((P + kEdgeTile - 1) / kEdgeTile);
+    for (int i = 0; i < 2; ++i) CFR(dalloc(&etcnt[i], (size_t)netiles * kEdgeSub * kCtrStride));
+    CFR(dalloc(&farn, 2)); CFR(dalloc(&far_slot, P));
+    HIPR(hipMemsetAsync(far_slot, 0, P * sizeof(uint32_t), st));
     HIPR(hipMemsetAsync(b.ecount, 0, (P + 1) * sizeof(uint32_t), st));
     HIPR(hipMemsetAsync(b.tpos, 0, P * sizeof(float4), st));
     HIPR(hipMemsetAsync(b.tvel, 0, P * sizeof(float4), st));
@@ -273,6 +294,7 @@ struct Engine {
     hpairs.assign(sims.size() * kMaxTypes * kMaxTypes, PairConst{});
     std::vector<int> sig;
     long c = 0;
+    int nt = 0;
     for (size_t s = 0; s < sims.size(); ++s) {
       SimHost& h = sims[s];
       SimDesc& d = hdesc[s];
@@ -354,6 +376,13 @@ struct Engine {
       d.edge_rx = re[0];
       d.edge_ry = re[1];
       d.edge_rz = re[2];
+      // cell tiles: tr rows of one plane, about kTileTarget particles at this density
+      const double occ = (double)std::max(h.n, 1) / (double)d.ncells;
+      const int trmax = std::max(1, std::min(d.ny, kTileCells / d.nx));
+      d.tr = (int)std::max(1.0, std::min((double)trmax, std::floor(kTileTarget / (d.nx * occ))));
+      d.nyt = (d.ny + d.tr - 1) / d.tr;
+      d.t0 = nt;
+      nt += d.nz * d.nyt;
       c += (long)d.ncells + 1;
       sig.push_back(d.nx);
       sig.push_back(d.ny);
@@ -367,6 +396,39 @@ struct Engine {
       CFR(ensure_cells());
     }
     HIPR(hipStreamSynchronize(st));  // descriptors may still be read by queued work
+    if (nt > ntiles_cap) {
+      drop_graphs();
+      ++buf_gen;
+      for (int i = 0; i < 2; ++i) {
+        dfree(tcnt[i]);
+        CFR(dalloc(&tcnt[i], (size_t)nt * kCtrStride));
+        dfree(ftn[i]);
+        CFR(dalloc(&ftn[i], (size_t)nt * kCtrStride));
+      }
+      dfree(tile_sim);
+      CFR(dalloc(&tile_sim, nt));
+      dfree(ftl);
+      CFR(dalloc(&ftl, (size_t)nt * kFarCap));
+      HIPR(hipMemsetAsync(ftl, 0, (size_t)nt * kFarCap * sizeof(uint32_t), st));
+      ntiles_cap = nt;
+    }
+    ntiles = nt;
+    htile_sim.assign(nt, 0u);
+    for (size_t s2 = 0; s2 < hdesc.size(); ++s2)
+      for (int t = 0; t < hdesc[s2].nz * hdesc[s2].nyt; ++t) htile_sim[hdesc[s2].t0 + t] = (uint32_t)s2;
+    if (nt > 0)
+      HIPR(hipMemcpyAsync(tile_sim, htile_sim.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          st));
+    // no step is in flight here: both parities of the step counters start at zero
+    for (int i = 0; i < 2; ++i) {
+      if (nt > 0) {
+        HIPR(hipMemsetAsync(tcnt[i], 0, (size_t)nt * kCtrStride * sizeof(uint32_t), st));
+        HIPR(hipMemsetAsync(ftn[i], 0, (size_t)nt * kCtrStride * sizeof(uint32_t), st));
+      }
+      HIPR(hipMemsetAsync(etcnt[i], 0, (size_t)netiles * kEdgeSub * kCtrStride * sizeof(uint32_t),
+                          st));
+    }
+    HIPR(hipMemsetAsync(farn, 0, 2 * sizeof(uint32_t), st));
     HIPR(hipMemcpyAsync(b.sims, hdesc.data(), hdesc.size() * sizeof(SimDesc),
                         hipMemcpyHostToDevice, st));
     HIPR(hipMemcpyAsync(b.pairs, hpairs.data(), hpairs.size() * sizeof(PairConst),
@@ -553,6 +615,13 @@ struct Engine {
     fa.nwaves = nwaves;
     fa.edges = edges ? 1 : 0;
     fa.edge_cap = cap;
+    fa.tcount = tcnt[A];
+    fa.etcount = etcnt[A];
+    fa.far_n = farn + A;
+    fa.far_slot = far_slot;
+    fa.ftn = ftn[A];
+    fa.ftl = ftl;
+    fa.tile_sort = tun.sort_variant == 1 ? 1 : 0;
     return fa;
   }
 
@@ -582,20 +651,68 @@ struct Engine {
     return sa;
   }
 
-  // Kernel sequence of one step; `ev` (optional) receives an event after each kernel.
+  TileSortArgs tile_sort_args(bool edges) {
+    const int A = cur, B = 1 - cur;
+    TileSortArgs ta{};
+    ta.src_pos = b.tpos;
+    ta.src_vel = b.tvel;
+    ta.src_id = b.id[A];
+    ta.key = b.key;
+    ta.hist = b.hist;
+    ta.start_old = b.start[A];
+    ta.start_new = b.start[B];
+    ta.dst_pos = b.pos[B];
+    ta.dst_vel = b.vel[B];
+    ta.dst_id = b.id[B];
+    ta.tmp_id = b.tmp_id;
+    ta.tmp_slot = b.arrival;
+    ta.sims = b.sims;
+    ta.tile_sim = tile_sim;
+    ta.tcount = tcnt[A];
+    ta.tcount_next = tcnt[B];
+    ta.far_n = farn + A;
+    ta.far_n_next = farn + B;
+    ta.far_slot = far_slot;
+    ta.ftn = ftn[A];
+    ta.ftn_next = ftn[B];
+    ta.ftl = ftl;
+    ta.flag = b.flag;
+    ta.ntiles = ntiles;
+    if (edges) {
+      ta.ecount = b.ecount;
+      ta.etcount = etcnt[A];
+      ta.etcount_next = etcnt[B];
+      ta.estage = b.estage;
+      ta.edges = b.edges;
+      ta.edge_room = (long)edge_maxc_alloc * P;
+      ta.netiles = netiles;
+    }
+    ta.wave_sim = b.wave_sim;
+    ta.P = P;
+    return ta;
+  }
+
+  // Kernel sequence of one step: the force pass, then either the tile sort (which also
+  // writes the step's canonical edge list) or the three-pass counting sort; `ev`
+  // (optional) receives an event after each kernel.
   int step(bool fused_edges, int cap, hipEvent_t* ev = nullptr) {
     const int B = 1 - cur;
     ForceArgs fa = force_args(fused_edges, cap);
     HIPR(launch_force(fa, fused_edges, tun.block_size, tun.force_variant,
                       std::max(1, std::min(2, tun.cell_divisor)), tun.lanes_per_particle, st));
     if (ev) HIPR(hipEventRecord(ev[0], st));
-    HIPR(launch_scan(plan(fused_edges ? 1 : 0, b.start[B]), st));
-    if (ev) HIPR(hipEventRecord(ev[1], st));
-    SortArgs sa = step_sort_args(fused_edges);
-    HIPR(launch_fill(sa, st));
-    if (ev) HIPR(hipEventRecord(ev[2], st));
-    HIPR(launch_rank_scatter(sa, st));
-    if (ev) HIPR(hipEventRecord(ev[3], st));
+    if (tun.sort_variant == 1) {
+      HIPR(launch_tile_sort(tile_sort_args(fused_edges), st));
+      if (ev) HIPR(hipEventRecord(ev[1], st));
+    } else {
+      HIPR(launch_scan(plan(fused_edges ? 1 : 0, b.start[B]), st));
+      if (ev) HIPR(hipEventRecord(ev[1], st));
+      SortArgs sa = step_sort_args(fused_edges);
+      HIPR(launch_fill(sa, st));
+      if (ev) HIPR(hipEventRecord(ev[2], st));
+      HIPR(launch_rank_scatter(sa, st));
+      if (ev) HIPR(hipEventRecord(ev[3], st));
+    }
     cur = B;
     return CF_OK;
   }
@@ -640,8 +757,8 @@ struct Engine {
     const int span = std::max(2, (tun.steps_per_graph + 1) / 2 * 2);
     if (tun.steps_per_graph >= 2 && count >= span) {
       std::vector<long long> key = {buf_gen, cur, fused ? 1 : 0, cap, span, tun.cell_divisor,
-                                    tun.force_variant, tun.lanes_per_particle, tun.block_size, C,
-                                    P};
+                                    tun.force_variant, tun.lanes_per_particle, tun.block_size,
+                                    tun.sort_variant, C, P};
       hipGraphExec_t exec = nullptr;
       for (auto& g : graphs)
         if (g.key == key) exec = g.exec;
@@ -690,6 +807,7 @@ struct Engine {
     HIPR(hipStreamSynchronize(st));
     if (edges_built) edges_count = (long long)hsmall[1];
     if (hsmall[0] & 1u) return fail(CF_ERR_NONFINITE, "a position or velocity became non-finite");
+    if (hsmall[0] & 4u) return fail(CF_ERR_DEVICE, "internal: the step sort lost track of a particle");
     return CF_OK;
   }
 
@@ -903,19 +1021,24 @@ struct Engine {
     CFR(prepare(er));
     const bool fuse = fusable(er);
     const int cap = 2 * er.maxc;
-    const char* kn[] = {"force", "scan", "fill", "rank_scatter", "edges", "edge_scan", "edge_write"};
-    const int K = 7;
-    double acc[K] = {0, 0, 0, 0, 0, 0, 0};
-    int cnt[K] = {0, 0, 0, 0, 0, 0, 0};
+    // the step's kernels (ns of them), then the standalone edge pass's three
+    const bool tiles = tun.sort_variant == 1;
+    const int ns = tiles ? 2 : 4;
+    const char* kt[] = {"force", "sort", "edges", "edge_scan", "edge_write"};
+    const char* kc[] = {"force", "scan", "fill", "rank_scatter", "edges", "edge_scan", "edge_write"};
+    const char* const* kn = tiles ? kt : kc;
+    const int K = ns + 3;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[8];
     for (auto& e : ev) HIPR(hipEventCreate(&e));
     for (int k = 0; k < nsteps; ++k) {
       HIPR(hipEventRecord(ev[0], st));
       CFR(step(er.on && fuse && k > 0, cap, ev + 1));
-      int ne = 4;
+      int ne = ns;
       if (er.on && !fuse) {
-        CFR(edge_pass(cap, ev + 5));
-        ne = 7;
+        CFR(edge_pass(cap, ev + ns + 1));
+        ne = ns + 3;
       }
       HIPR(hipEventSynchronize(ev[ne]));
       for (int i = 0; i < ne; ++i) {
@@ -926,10 +1049,10 @@ struct Engine {
       }
     }
     if (er.on && fuse) {
-      HIPR(hipEventRecord(ev[4], st));
-      CFR(edge_pass(cap, ev + 5));
-      HIPR(hipEventSynchronize(ev[7]));
-      for (int i = 4; i < 7; ++i) {
+      HIPR(hipEventRecord(ev[ns], st));
+      CFR(edge_pass(cap, ev + ns + 1));
+      HIPR(hipEventSynchronize(ev[ns + 3]));
+      for (int i = ns; i < ns + 3; ++i) {
         float ms = 0;
         HIPR(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
         acc[i] += ms;
@@ -1308,14 +1431,16 @@ struct Engine {
     std::vector<cf_tuning> cands;
     for (int d = 1; d <= 2; ++d) {
       for (int g : {16, 8, 4, 32}) cands.push_back(cf_tuning{d, 0, g, orig.block_size,
-                                                             orig.steps_per_graph});
+                                                             orig.steps_per_graph, 1});
       for (int bs : {64, 128, 256}) cands.push_back(cf_tuning{d, 1, orig.lanes_per_particle, bs,
-                                                              orig.steps_per_graph});
+                                                              orig.steps_per_graph, 1});
       cands.push_back(cf_tuning{d, 2, orig.lanes_per_particle, orig.block_size,
-                                orig.steps_per_graph});
+                                orig.steps_per_graph, 1});
       for (int g : {16, 8, 4, 32})
-        if (g >= 4 * d) cands.push_back(cf_tuning{d, 3, g, orig.block_size, orig.steps_per_graph});
+        if (g >= 4 * d)
+          cands.push_back(cf_tuning{d, 3, g, orig.block_size, orig.steps_per_graph, 1});
     }
+    const size_t nforce = cands.size();  // then the best two again with the three-pass sort
     const int span = orig.steps_per_graph >= 2 ? (orig.steps_per_graph + 1) / 2 * 2 : 2;
     const int nsteps = std::max(span, (std::max(1, trial_steps) + span - 1) / span * span);
     StateBuf snap;
@@ -1327,7 +1452,21 @@ struct Engine {
       HIPR(hipMemcpyAsync(acc_snap, b.acc, P * sizeof(float4), hipMemcpyDeviceToDevice, st));
       double best = -1.0, min_probe = -1.0;
       cf_tuning bt = orig;
-      for (const cf_tuning& c : cands) {
+      for (size_t ci = 0;; ++ci) {
+        if (ci == nforce) {
+          std::vector<std::pair<double, cf_tuning>> timed;
+          for (const Trial& tr : *trials)
+            if (!tr.pruned) timed.push_back({tr.us, tr.t});
+          std::sort(timed.begin(), timed.end(),
+                    [](const auto& x, const auto& y) { return x.first < y.first; });
+          for (size_t k = 0; k < timed.size() && k < 2; ++k) {
+            cf_tuning c = timed[k].second;
+            c.sort_variant = 0;
+            cands.push_back(c);
+          }
+        }
+        if (ci >= cands.size()) break;
+        const cf_tuning c = cands[ci];
         CFR(restore(snap, orig, er));
         tun = c;
         CFR(prepare(er));
@@ -1410,9 +1549,10 @@ static int tune_call(Engine& e, const EdgeReq& er, int trial_steps, const char* 
       const cf_tuning& c = trials[i].t;
       std::snprintf(b, sizeof b,
                     "%s{\"cell_divisor\": %d, \"force_variant\": %d, \"lanes_per_particle\": %d, "
-                    "\"block_size\": %d, \"us_per_step\": %.3f, \"pruned\": %s}",
+                    "\"block_size\": %d, \"sort_variant\": %d, \"us_per_step\": %.3f, "
+                    "\"pruned\": %s}",
                     i ? ", " : "", c.cell_divisor, c.force_variant, c.lanes_per_particle,
-                    c.block_size, trials[i].us, trials[i].pruned ? "true" : "false");
+                    c.block_size, c.sort_variant, trials[i].us, trials[i].pruned ? "true" : "false");
       rep += b;
     }
     rep += "]}";
@@ -1867,6 +2007,8 @@ static int check_tuning(const cf_tuning* t) {
     return cf::fail(CF_ERR_ARG, "force_variant 3 needs lanes_per_particle >= 8 with "
                                 "cell_divisor 2");
   if (t->steps_per_graph < 1) return cf::fail(CF_ERR_ARG, "steps_per_graph must be >= 1");
+  if (t->sort_variant != 0 && t->sort_variant != 1)
+    return cf::fail(CF_ERR_ARG, "sort_variant must be 0 or 1");
   return CF_OK;
 }
 

@@ -15,6 +15,25 @@ constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 4096 elements per scan tile
 constexpr int kMaxEdgeCap = 64;        // 2 * CF_MAX_CONNECTIONS
 
+// Step sort by cell tiles (k_tile_sort).  A cell tile is `tr` consecutive rows of one
+// z-plane of one simulation.  A particle whose new cell lies in a tile was, before the
+// step, in the tile's window (the tile's rows and one row either side, in the tile's plane
+// and one plane either side, cyclically), unless it is a far mover: those carry kFarBit
+// in their key and are listed apart.
+constexpr int kTileCells = 2048;       // most cells in one tile (tr * nx)
+constexpr int kTileCap = 2048;         // members a tile sorts in LDS at once
+constexpr int kTileTarget = 192;       // expected members of a tile (chooses tr)
+constexpr int kEdgeTile = 1024;        // owner ids per edge tile
+constexpr uint32_t kFarBit = 0x80000000u;
+// Tile and edge-tile counters sit one per 128-byte line: a line takes every atomic of the
+// tiles it holds, one after another.  An edge tile's count is further split over kEdgeSub
+// lines (by the counting block's index), since its owners come from everywhere.
+constexpr int kCtrStride = 32;
+constexpr int kEdgeSub = 8;
+// Far movers are listed by destination tile (kFarCap per tile); a tile's excess goes to one
+// shared overflow list that every tile reads.
+constexpr int kFarCap = 256;
+
 // Per-type-pair constants of one simulation, T*T entries, row = own type.
 struct PairConst {
   float thr;       // smallest float t with sqrtf(t) >= effR (cutoff test: s + 1e-4f < thr)
@@ -46,6 +65,9 @@ struct SimDesc {
   float edge_d2;            // proximityDistance^2 in FP32 (ParticleSimulation.cu:660)
   int edge_maxc;            // min(maxConn, 32)
   int edge_rx, edge_ry, edge_rz;  // non-periodic edge stencil reach (standalone edge pass)
+  int t0;                   // first cell tile of this simulation
+  int tr;                   // rows per cell tile
+  int nyt;                  // cell tiles per z-plane, ceil(ny / tr)
 };
 
 // Decoupled look-back scan plan: one 64-bit ticket counter and one flag word per tile.
@@ -128,6 +150,17 @@ struct ForceArgs {
   int nwaves;
   int edges;        // fused edge candidates requested for this launch
   int edge_cap;
+  // 1: the tile sort follows (keys with kFarBit, tile counts, far lists); 0: the three-pass
+  // counting sort follows (arrival ranks)
+  int tile_sort;
+  // tile sort inputs (this step's counters; the sort clears the next step's)
+  uint32_t* tcount;     // members per cell tile (tile t at t * kCtrStride)
+  uint32_t* etcount;    // edges per edge tile (owner id / kEdgeTile) in kEdgeSub parts,
+                        // part j of tile u at (u * kEdgeSub + j) * kCtrStride
+  uint32_t* far_n;      // far movers in the overflow list
+  uint32_t* far_slot;   // their pre-step slots
+  uint32_t* ftn;        // far movers per tile (tile t at t * kCtrStride)
+  uint32_t* ftl;        // their pre-step slots (tile t at t * kFarCap)
 };
 
 struct EdgeArgs {
@@ -165,6 +198,46 @@ struct SortArgs {
   int2* __restrict__ edges;
   long P;
   int nwaves;
+};
+
+// One step's sort after the force pass (k_tile_sort): blocks [0, ntiles) each place the
+// particles of one cell tile, blocks [ntiles, ntiles + netiles) each write the canonical
+// edge list of one edge tile.
+struct TileSortArgs {
+  const float4* __restrict__ src_pos;   // new state in pre-step slot order
+  const float4* __restrict__ src_vel;
+  const uint32_t* __restrict__ src_id;
+  const uint32_t* __restrict__ key;     // new cell (| kFarBit) by pre-step slot
+  uint32_t* hist;                       // per-cell counts (read, then cleared)
+  const uint32_t* __restrict__ start_old;
+  uint32_t* start_new;
+  float4* dst_pos;
+  float4* dst_vel;
+  uint32_t* dst_id;
+  uint32_t* tmp_id;                     // scratch of the one-cell fallback: ids
+  uint32_t* tmp_slot;                   // and their pre-step slots
+  const SimDesc* __restrict__ sims;
+  const uint32_t* __restrict__ tile_sim;  // simulation of each cell tile
+  const uint32_t* tcount;
+  uint32_t* tcount_next;
+  const uint32_t* far_n;
+  uint32_t* far_n_next;
+  const uint32_t* far_slot;
+  const uint32_t* ftn;
+  uint32_t* ftn_next;
+  const uint32_t* ftl;
+  uint32_t* flag;                       // bit 1: a cell outgrew the LDS; bit 2: lost particle
+  int ntiles;
+  // edges (edge tiles; null edges: no edge tiles)
+  const uint32_t* ecount;
+  const uint32_t* etcount;
+  uint32_t* etcount_next;
+  const uint32_t* estage;               // [rank][owner id]
+  int2* edges;
+  long edge_room;                       // entries `edges` holds
+  const uint32_t* __restrict__ wave_sim;
+  int netiles;
+  long P;
 };
 
 struct BinArgs {
@@ -276,6 +349,7 @@ int adj_pair_blocks(int n);  // blocks (columns of AdjArgs::partial) of one back
 hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
                         int lanes, hipStream_t st);
 hipError_t launch_edges(const EdgeArgs& a, hipStream_t st);
+hipError_t launch_tile_sort(const TileSortArgs& a, hipStream_t st);
 hipError_t launch_edge_write(const SortArgs& a, hipStream_t st);
 hipError_t launch_scan(const ScanPlan& p, hipStream_t st);
 hipError_t launch_fill(const SortArgs& a, hipStream_t st);

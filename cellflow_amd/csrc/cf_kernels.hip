@@ -61,11 +61,13 @@ __device__ __forceinline__ float wrap_pos(float x, float L) {
 // same slots, then share an L2.
 constexpr int kXcds = 8;
 
-__device__ __forceinline__ int xcd_block() {
-  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+// The same order over the first nb blocks of a grid (b < nb).
+__device__ __forceinline__ int xcd_map(int b, int nb) {
   const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
   return x * q + min(x, r) + b / kXcds;
 }
+
+__device__ __forceinline__ int xcd_block() { return xcd_map((int)blockIdx.x, (int)gridDim.x); }
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
 #pragma unroll
@@ -118,9 +120,10 @@ __device__ __forceinline__ void edge_push(EdgeAcc& e, float d2, uint32_t jid, in
 
 // Stable sort of the kept candidates by squared distance (ties by index, which is the
 // order the reference's insertion sort leaves, ParticleSimulation.cu:235-243), truncated
-// to maxc (:246).  Writes estage[rank][p] and returns the kept count.
+// to maxc (:246).  Writes estage[rank][o] (o: the owner's index) and returns the kept count.
 __device__ __forceinline__ int edge_finish(const EdgeAcc& e, int cap, int maxc, const float* cd2,
-                                           const uint32_t* cid, uint32_t* estage, long P, int p) {
+                                           const uint32_t* cid, uint32_t* estage, long P, int p,
+                                           uint32_t o) {
   const int m = min(e.q, cap);
   for (int k = 0; k < m; ++k) {
     const float dk = cd2[(long)k * P + p];
@@ -131,9 +134,64 @@ __device__ __forceinline__ int edge_finish(const EdgeAcc& e, int cap, int maxc, 
       const uint32_t i2 = cid[(long)k2 * P + p];
       rank += (d2 < dk || (d2 == dk && i2 < ik)) ? 1 : 0;
     }
-    if (rank < maxc) estage[(long)rank * P + p] = ik;
+    if (rank < maxc) estage[(long)rank * P + o] = ik;
   }
   return min(m, maxc);
+}
+
+// New cell of an integrated particle (x, y, z) whose pre-step cell had rows/planes (cyo,
+// czo).  For the counting sort: its key and arrival rank.  For the tile sort: its key, with
+// kFarBit (and a place in a far list) when the pre-step cell lies outside the window of
+// the new cell's tile, and the per-cell and per-tile counts.  The window test is the one
+// k_tile_sort enumerates.
+__device__ __forceinline__ void commit_cell(const ForceArgs& a, const SimDesc& sd, int p, int cyo,
+                                            int czo, float x, float y, float z) {
+  const int xn = bin_axis(x, sd.ihx, sd.nx), yn = bin_axis(y, sd.ihy, sd.ny),
+            zn = bin_axis(z, sd.ihz, sd.nz);
+  const uint32_t c = (uint32_t)(sd.c0 + xn + sd.nx * (yn + sd.ny * zn));
+  if (!a.tile_sort) {  // the counting sort ranks by arrival
+    a.key[p] = c;
+    a.arrival[p] = atomicAdd(&a.hist[c], 1u);
+    return;
+  }
+  const int ty = yn / sd.tr, y0 = ty * sd.tr;
+  const int nrw = min(min(sd.tr, sd.ny - y0) + 2, sd.ny), npl = min(3, sd.nz);
+  const bool near = (cyo - y0 + 1 + sd.ny) % sd.ny < nrw && (czo - zn + 1 + sd.nz) % sd.nz < npl;
+  const uint32_t tile = (uint32_t)(sd.t0 + zn * sd.nyt + ty);
+  uint32_t k = c;
+  if (!near) {
+    k |= kFarBit;
+    const uint32_t j = atomicAdd(&a.ftn[tile * kCtrStride], 1u);
+    if (j < (uint32_t)kFarCap) {
+      a.ftl[(long)tile * kFarCap + j] = (uint32_t)p;
+    } else {
+      const uint32_t jo = atomicAdd(a.far_n, 1u);
+      if (jo < (uint32_t)a.P) a.far_slot[jo] = (uint32_t)p;
+    }
+  }
+  a.key[p] = k;
+  atomicAdd(&a.hist[c], 1u);
+  // one tile-count atomic per distinct tile among the wave's particles (they are
+  // consecutive slots, so nearly always one)
+  const int lane = threadIdx.x & (kWave - 1);
+  bool todo = true;
+  while (true) {
+    const unsigned long long m = __ballot(todo);
+    if (m == 0ull) break;
+    const int lead = __ffsll((long long)m) - 1;
+    const uint32_t lt = (uint32_t)__shfl((int)tile, lead, kWave);
+    const bool same = todo && tile == lt;
+    const unsigned long long ms = __ballot(same);
+    if (lane == lead) atomicAdd(&a.tcount[lt * kCtrStride], (uint32_t)__popcll(ms));
+    if (same) todo = false;
+  }
+}
+
+// Edge count of an owner into its edge tile's count (one of kEdgeSub parts).
+__device__ __forceinline__ void count_edges(const ForceArgs& a, uint32_t idi, uint32_t e) {
+  if (a.tile_sort && e > 0u)
+    atomicAdd(&a.etcount[((idi / kEdgeTile) * kEdgeSub + (blockIdx.x & (kEdgeSub - 1))) * kCtrStride],
+              e);
 }
 
 // One axis of the force stencil as up to three runs of cells: the run below cell 0
@@ -292,16 +350,14 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs a) {
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
   a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  const uint32_t c = (uint32_t)(sd.c0 + bin_axis(x, sd.ihx, sd.nx) +
-                                sd.nx * (bin_axis(y, sd.ihy, sd.ny) +
-                                         sd.ny * bin_axis(z, sd.ihz, sd.nz)));
-  a.key[p] = c;
-  a.arrival[p] = atomicAdd(&a.hist[c], 1u);
+  commit_cell(a, sd, p, cy, cz, x, y, z);
 
   if (EDGES) {
     if (do_edges) {
-      const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p);
+      const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage,
+                                a.P, p, idi);
       a.ecount[idi] = (uint32_t)e;
+      count_edges(a, idi, (uint32_t)e);
     }
   }
 }
@@ -586,16 +642,13 @@ __global__ void __launch_bounds__(64) k_force_lds(ForceArgs a) {
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(acc.cnt));
   a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  const uint32_t c = (uint32_t)(sd.c0 + bin_axis(x, sd.ihx, sd.nx) +
-                                sd.nx * (bin_axis(y, sd.ihy, sd.ny) +
-                                         sd.ny * bin_axis(z, sd.ihz, sd.nz)));
-  a.key[p] = c;
-  a.arrival[p] = atomicAdd(&a.hist[c], 1u);
+  commit_cell(a, sd, p, cy, cz, x, y, z);
   if (EDGES) {
     if (do_edges) {
       const int e = edge_finish(acc.ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id,
-                                a.estage, a.P, p);
+                                a.estage, a.P, p, idi);
       a.ecount[idi] = (uint32_t)e;
+      count_edges(a, idi, (uint32_t)e);
     }
   }
 }
@@ -901,11 +954,12 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
           const uint32_t i2 = el_i[wv][o][k2];
           rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
         }
-        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + p] = ik;
+        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + idi] = ik;
       }
     if (do_edges && g == 0) {
+      uint32_t e;
       if (fits) {
-        a.ecount[idi] = (uint32_t)min(min((int)n, cap), maxc);
+        e = (uint32_t)min(min((int)n, cap), maxc);
       } else {
         // more candidates than the LDS list holds: exact streaming selection by one lane
         EdgeAcc ea;
@@ -930,9 +984,10 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
             if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
           }
         }
-        a.ecount[idi] =
-            (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p);
+        e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
       }
+      a.ecount[idi] = e;
+      count_edges(a, idi, e);
     }
   }
 
@@ -957,11 +1012,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
   a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  const uint32_t c = (uint32_t)(sd.c0 + bin_axis(x, sd.ihx, sd.nx) +
-                                sd.nx * (bin_axis(y, sd.ihy, sd.ny) +
-                                         sd.ny * bin_axis(z, sd.ihz, sd.nz)));
-  a.key[p] = c;
-  a.arrival[p] = atomicAdd(&a.hist[c], 1u);
+  commit_cell(a, sd, p, cy, cz, x, y, z);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1267,11 +1318,12 @@ k_force_list(ForceArgs a) {
           const uint32_t i2 = el_i[wv][o][k2];
           rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
         }
-        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + p] = ik;
+        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + idi] = ik;
       }
     if (do_edges && g == 0) {
+      uint32_t e;
       if (fits) {
-        a.ecount[idi] = (uint32_t)min(min((int)n, cap), maxc);
+        e = (uint32_t)min(min((int)n, cap), maxc);
       } else {
         // more candidates than the LDS list holds: exact streaming selection by one lane,
         // over the stencil rows again
@@ -1299,9 +1351,10 @@ k_force_list(ForceArgs a) {
             }
           }
         }
-        a.ecount[idi] =
-            (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p);
+        e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
       }
+      a.ecount[idi] = e;
+      count_edges(a, idi, e);
     }
   }
 
@@ -1326,11 +1379,7 @@ k_force_list(ForceArgs a) {
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
   a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  const uint32_t cc = (uint32_t)(sd.c0 + bin_axis(x, sd.ihx, sd.nx) +
-                                 sd.nx * (bin_axis(y, sd.ihy, sd.ny) +
-                                          sd.ny * bin_axis(z, sd.ihz, sd.nz)));
-  a.key[p] = cc;
-  a.arrival[p] = atomicAdd(&a.hist[cc], 1u);
+  commit_cell(a, sd, p, c.cy, c.cz, x, y, z);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1376,8 +1425,268 @@ __global__ void __launch_bounds__(256) k_edges(EdgeArgs a) {
         edge_push(ea, s2, jid, a.edge_cap, a.ecand_d2, a.ecand_id, a.P, p);
       }
     }
-  const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p);
+  const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
+                            idi);
   a.ecount[idi] = (uint32_t)e;
+}
+
+// ------------------------------------------------------------------------------------
+// k_tile_sort: the (cell, index) sort of one step, and the step's canonical edge list.
+//
+// The force pass leaves, by pre-step slot, the new state and the new cell (key), and
+// counts per cell and per cell tile.  A cell tile's block needs no other block's result:
+// its first slot is its simulation's first slot plus the counts of the simulation's tiles
+// before it, its cells' starts follow from the per-cell counts, and its particles are
+// found by reading the keys of its window (and the far list).  Inside a cell they are
+// ranked by index, so the order is the canonical one whatever order the atomics arrived
+// in.  An edge tile writes its owners' edges at the counts of the edge tiles before it
+// plus a local scan.  Each block also clears its own counter for the next step (the next
+// step counts into the other parity).
+
+constexpr int kSortThreads = 256;
+
+// Sum over the block; every thread gets it.  `red` holds 4 words.
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* red) {
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, kWave);
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+// Exclusive scan over the block in thread order; `total` gets the sum.
+__device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* red, uint32_t& total) {
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+  const uint32_t incl = wave_inclusive_scan(v, lane);
+  __syncthreads();
+  if (lane == kWave - 1) red[wv] = incl;
+  __syncthreads();
+  uint32_t pre = 0;
+#pragma unroll
+  for (int w = 0; w < kSortThreads / kWave; ++w)
+    if (w < wv) pre += red[w];
+  total = red[0] + red[1] + red[2] + red[3];
+  return pre + incl - v;
+}
+
+__device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
+  const int tid = threadIdx.x;
+  if (tid < kEdgeSub) a.etcount_next[(u * kEdgeSub + tid) * kCtrStride] = 0u;
+  uint32_t part = 0, mine = 0;
+  for (int v = tid; v < (u + 1) * kEdgeSub; v += kSortThreads) {
+    const uint32_t x = a.etcount[v * kCtrStride];
+    if (v < u * kEdgeSub) part += x;
+    else mine += x;
+  }
+  const uint32_t own = block_sum_u32(mine, red);
+  const uint32_t base = block_sum_u32(part, red);
+  constexpr int kPer = kEdgeTile / kSortThreads;
+  const long id0 = (long)u * kEdgeTile + (long)tid * kPer;
+  uint32_t ec[kPer];
+  uint32_t run = 0;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    ec[i] = (id0 + i < a.P) ? a.ecount[id0 + i] : 0u;
+    run += ec[i];
+  }
+  uint32_t total;
+  uint32_t off = base + block_excl_u32(run, red, total);
+  if (total != own || (long)base + total > a.edge_room) {
+    if (tid == 0) atomicOr(a.flag, 4u);
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    if (ec[i]) {
+      const long id = id0 + i;
+      const int p0 = a.sims[a.wave_sim[id / kWave]].p0;
+      for (uint32_t k = 0; k < ec[i]; ++k)
+        a.edges[off + k] = make_int2((int)(id - p0), (int)(a.estage[(long)k * a.P + id] - p0));
+    }
+    off += ec[i];
+  }
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
+  __shared__ uint32_t cnt_s[kTileCells];
+  __shared__ uint32_t cofs[kTileCells + 1];
+  __shared__ uint32_t fill_s[kTileCells];
+  __shared__ uint32_t mslot[kTileCap];
+  __shared__ uint32_t mid[kTileCap];
+  __shared__ uint16_t mlc[kTileCap];
+  __shared__ uint32_t red[kSortThreads / kWave];
+  __shared__ int s_next;
+  const int tid = threadIdx.x;
+  if ((int)blockIdx.x >= a.ntiles) {
+    edge_tile(a, (int)blockIdx.x - a.ntiles, red);
+    return;
+  }
+  // consecutive tiles (consecutive slots) share an XCD, as in the force pass
+  const int t = xcd_map((int)blockIdx.x, a.ntiles);
+  const SimDesc& sd = a.sims[a.tile_sim[t]];
+  if (tid == 0) {
+    a.tcount_next[t * kCtrStride] = 0u;
+    a.ftn_next[t * kCtrStride] = 0u;
+    if (t == 0) *a.far_n_next = 0u;
+  }
+  const int lt = t - sd.t0, zt = lt / sd.nyt, y0 = (lt % sd.nyt) * sd.tr;
+  const int nrow = min(sd.tr, sd.ny - y0), nc = sd.nx * nrow;
+  const uint32_t cA = (uint32_t)(sd.c0 + sd.nx * (y0 + sd.ny * zt));
+
+  // first slot of the tile, then its cells' starts (counts cleared for the next step)
+  uint32_t part = 0;
+  for (int u = sd.t0 + tid; u < t; u += kSortThreads) part += a.tcount[u * kCtrStride];
+  const uint32_t base = (uint32_t)sd.p0 + block_sum_u32(part, red);
+  const int per = (nc + kSortThreads - 1) / kSortThreads;
+  const int i0 = min(tid * per, nc), i1 = min(i0 + per, nc);
+  uint32_t run = 0;
+  for (int i = i0; i < i1; ++i) {
+    const uint32_t v = a.hist[cA + i];
+    cnt_s[i] = v;
+    run += v;
+  }
+  uint32_t M;
+  uint32_t pre = block_excl_u32(run, red, M);
+  // The counts must agree (and the simulation's last tile must end at its last particle);
+  // if they do not, the tile writes nothing but clamped starts, and the call fails.
+  const uint32_t lim = (uint32_t)(sd.p0 + sd.n);
+  const bool last = lt == sd.nz * sd.nyt - 1;
+  const bool ok = M == a.tcount[t * kCtrStride] && base + M <= lim && (!last || base + M == lim);
+  for (int i = i0; i < i1; ++i) {
+    cofs[i] = pre;
+    a.start_new[cA + i] = min(base + pre, lim);
+    a.hist[cA + i] = 0u;
+    pre += cnt_s[i];
+  }
+  if (tid == 0) {
+    cofs[nc] = M;
+    if (!ok) atomicOr(a.flag, 4u);
+    if (last) {  // the simulation's pad cell, and the start of what follows it
+      a.start_new[sd.c0 + sd.ncells] = lim;
+      a.start_new[sd.c0 + sd.ncells + 1] =
+          (uint32_t)(sd.p0 + (sd.n + kSlotAlign - 1) / kSlotAlign * kSlotAlign);
+    }
+  }
+  __syncthreads();
+  if (!ok) return;
+
+  // Window: planes zt-1, zt, zt+1 and rows y0-1 .. y0+nrow, cyclic and distinct; each
+  // plane's rows are one or two runs of consecutive cells, so of consecutive slots.
+  const int np = min(3, sd.nz), nrw = min(nrow + 2, sd.ny);
+  const int ys = (y0 - 1 + sd.ny) % sd.ny, yrun0 = min(nrw, sd.ny - ys);
+  auto visit = [&](auto&& fn) {
+    for (int ip = 0; ip < np; ++ip) {
+      const int zp = (zt - 1 + ip + sd.nz) % sd.nz;
+      for (int h = 0; h < 2; ++h) {
+        const int ra = h ? 0 : ys, rn = h ? nrw - yrun0 : yrun0;
+        if (rn <= 0) continue;
+        const int cf = sd.c0 + sd.nx * (ra + sd.ny * zp);
+        const uint32_t q0 = a.start_old[cf], q1 = a.start_old[cf + sd.nx * rn];
+        for (uint32_t q = q0 + tid; q < q1; q += kSortThreads) {
+          const uint32_t k = a.key[q];
+          if (!(k & kFarBit)) fn(q, k);
+        }
+      }
+    }
+    const uint32_t nt = min(a.ftn[t * kCtrStride], (uint32_t)kFarCap);
+    for (uint32_t j = tid; j < nt; j += kSortThreads) {
+      const uint32_t q = a.ftl[(long)t * kFarCap + j];
+      fn(q, a.key[q] & ~kFarBit);
+    }
+    const uint32_t nf = min(*a.far_n, (uint32_t)a.P);
+    for (uint32_t j = tid; j < nf; j += kSortThreads) {
+      const uint32_t q = a.far_slot[j];
+      fn(q, a.key[q] & ~kFarBit);
+    }
+  };
+
+  // cells [ia, ib) at a time, as many as kTileCap members allow (all of them, normally)
+  int ia = 0;
+  while (ia < nc) {
+    if (tid == 0) {
+      int lo = ia + 1, hi = nc;
+      while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (cofs[m] - cofs[ia] <= (uint32_t)kTileCap) lo = m;
+        else hi = m - 1;
+      }
+      s_next = lo;
+    }
+    __syncthreads();
+    const int ib = s_next;
+    const uint32_t m0 = cofs[ia], mn = cofs[ib] - m0;
+    if (mn <= (uint32_t)kTileCap) {
+      for (int i = ia + tid; i < ib; i += kSortThreads) fill_s[i] = 0u;
+      for (uint32_t m = tid; m < mn; m += kSortThreads) mlc[m] = 0xFFFFu;
+      __syncthreads();
+      const uint32_t ca = cA + (uint32_t)ia, cb = cA + (uint32_t)ib;
+      visit([&](uint32_t q, uint32_t k) {
+        if (k < ca || k >= cb) return;
+        const uint32_t lc = k - cA;
+        const uint32_t at = atomicAdd(&fill_s[lc], 1u);
+        if (at >= cnt_s[lc]) {
+          atomicOr(a.flag, 4u);
+          return;
+        }
+        const uint32_t m = cofs[lc] - m0 + at;
+        mslot[m] = q;
+        mid[m] = a.src_id[q];
+        mlc[m] = (uint16_t)lc;
+      });
+      __syncthreads();
+      for (uint32_t m = tid; m < mn; m += kSortThreads) {
+        const uint32_t lc = mlc[m];
+        if (lc >= (uint32_t)nc || fill_s[lc] != cnt_s[lc]) {
+          atomicOr(a.flag, 4u);  // a cell lost a particle: leave its slots alone
+          continue;
+        }
+        const uint32_t s0 = cofs[lc] - m0, sn = cnt_s[lc], me = mid[m];
+        uint32_t rank = 0;
+        for (uint32_t e = 0; e < sn; ++e) rank += (mid[s0 + e] < me) ? 1u : 0u;
+        const uint32_t dst = base + cofs[lc] + rank, q = mslot[m];
+        a.dst_pos[dst] = a.src_pos[q];
+        a.dst_vel[dst] = a.src_vel[q];
+        a.dst_id[dst] = me;
+      }
+    } else {
+      // one cell (ib == ia + 1) holds more than the LDS takes: its ids and slots go to
+      // global scratch at the cell's own slot range, ranks by a pass over all of them
+      const uint32_t K = cnt_s[ia], g0 = base + cofs[ia], cg = cA + (uint32_t)ia;
+      if (tid == 0) {
+        fill_s[ia] = 0u;
+        atomicOr(a.flag, 2u);
+      }
+      __syncthreads();
+      visit([&](uint32_t q, uint32_t k) {
+        if (k != cg) return;
+        const uint32_t at = atomicAdd(&fill_s[ia], 1u);
+        if (at >= K) {
+          atomicOr(a.flag, 4u);
+          return;
+        }
+        a.tmp_id[g0 + at] = a.src_id[q];
+        a.tmp_slot[g0 + at] = q;
+      });
+      __syncthreads();
+      if (fill_s[ia] == K) {
+        for (uint32_t m = tid; m < K; m += kSortThreads) {
+          const uint32_t me = a.tmp_id[g0 + m];
+          uint32_t rank = 0;
+          for (uint32_t e = 0; e < K; ++e) rank += (a.tmp_id[g0 + e] < me) ? 1u : 0u;
+          const uint32_t q = a.tmp_slot[g0 + m];
+          a.dst_pos[g0 + rank] = a.src_pos[q];
+          a.dst_vel[g0 + rank] = a.src_vel[q];
+          a.dst_id[g0 + rank] = me;
+        }
+      } else if (tid == 0) {
+        atomicOr(a.flag, 4u);
+      }
+    }
+    __syncthreads();  // the LDS lists are reused by the next cells
+    ia = ib;
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -1506,7 +1815,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(ScanPlan plan) {
 
 
 __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
-  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int gw = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (gw >= a.nwaves) return;
   const SimDesc sd = a.sims[a.wave_sim[gw]];
   const int p = gw * kWave + (threadIdx.x & (kWave - 1));
@@ -1521,12 +1830,12 @@ __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
     const uint32_t off = a.eoff[me];
     const int e = (int)(a.eoff[me + 1] - off);
     for (int k = 0; k < e; ++k)
-      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
+      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + me] - sd.p0));
   }
 }
 
 __global__ void __launch_bounds__(256) k_rank_scatter(SortArgs a) {
-  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int gw = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (gw >= a.nwaves) return;
   const SimDesc sd = a.sims[a.wave_sim[gw]];
   const int p = gw * kWave + (threadIdx.x & (kWave - 1));
@@ -1566,7 +1875,7 @@ __global__ void __launch_bounds__(256) k_edge_write(SortArgs a) {
   const uint32_t off = a.eoff[me];
   const int e = (int)(a.eoff[me + 1] - off);
   for (int k = 0; k < e; ++k)
-    a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
+    a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + me] - sd.p0));
 }
 
 // ------------------------------------------------------------------------------------
@@ -1822,6 +2131,12 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
 
 hipError_t launch_edges(const EdgeArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_edges, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_sort(const TileSortArgs& a, hipStream_t st) {
+  const int g = a.ntiles + a.netiles;
+  if (g > 0) hipLaunchKernelGGL(k_tile_sort, dim3(g), dim3(kSortThreads), 0, st, a);
   return hipGetLastError();
 }
 

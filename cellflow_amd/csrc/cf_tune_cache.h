@@ -8,11 +8,12 @@
 //     "entries": {
 //       "gfx950/sims=0/n=16/rho=1/T=8/e=16": {"cell_divisor": 1, "force_variant": 0,
 //           "lanes_per_particle": 16, "block_size": 256, "steps_per_graph": 32,
-//           "us_per_step": 74.1},
+//           "sort_variant": 1, "us_per_step": 74.1},
 //       ...
 //     }
 //   }
-// Unknown top-level fields and unknown entry fields are ignored on read.
+// Unknown top-level fields and unknown entry fields are ignored on read; an entry without
+// "sort_variant" (written before it existed) reads as sort_variant 1.
 #pragma once
 
 #include <cerrno>
@@ -59,7 +60,7 @@ static inline bool valid(const cf_tuning& t) {
   return (t.cell_divisor == 1 || t.cell_divisor == 2) &&
          (t.block_size == 64 || t.block_size == 128 || t.block_size == 256) &&
          (t.force_variant >= 0 && t.force_variant <= 3) && lanes_ok && list_ok &&
-         t.steps_per_graph >= 1;
+         t.steps_per_graph >= 1 && (t.sort_variant == 0 || t.sort_variant == 1);
 }
 
 enum Status { kOk = 0, kMissing = 1, kBad = 2 };
@@ -113,6 +114,9 @@ static inline int read_file(const char* path, std::map<std::string, Entry>& out,
         !int_field(kv.second, "block_size", &e.tun.block_size) ||
         !int_field(kv.second, "steps_per_graph", &e.tun.steps_per_graph))
       continue;  // incomplete entries are skipped (and dropped on the next store)
+    e.tun.sort_variant = 1;
+    if (kv.second.get("sort_variant") && !int_field(kv.second, "sort_variant", &e.tun.sort_variant))
+      continue;
     if (!valid(e.tun)) continue;
     const JVal* us = kv.second.get("us_per_step");
     e.us = (us && us->kind == JVal::Num) ? us->num : 0.0;
@@ -137,9 +141,10 @@ static inline bool write_file(const char* path, const std::map<std::string, Entr
     std::fprintf(f,
                  "%s\n    \"%s\": {\"cell_divisor\": %d, \"force_variant\": %d, "
                  "\"lanes_per_particle\": %d, \"block_size\": %d, \"steps_per_graph\": %d, "
-                 "\"us_per_step\": %.4f}",
+                 "\"sort_variant\": %d, \"us_per_step\": %.4f}",
                  first ? "" : ",", escape(kv.first).c_str(), t.cell_divisor, t.force_variant,
-                 t.lanes_per_particle, t.block_size, t.steps_per_graph, kv.second.us);
+                 t.lanes_per_particle, t.block_size, t.steps_per_graph, t.sort_variant,
+                 kv.second.us);
     first = false;
   }
   std::fprintf(f, "%s}\n}\n", first ? "" : "\n  ");

@@ -374,9 +374,9 @@ class ParticleSimulation:
         return {k: ms[i] for i, k in enumerate(keys[: nk.value])}
 
     def setTuning(self, cell_divisor=1, force_variant=0, lanes_per_particle=16, block_size=256,
-                  steps_per_graph=32):
+                  steps_per_graph=32, sort_variant=1):
         t = _abi.cf_tuning(cell_divisor, force_variant, lanes_per_particle, block_size,
-                           steps_per_graph)
+                           steps_per_graph, sort_variant)
         check(lib().cf_set_tuning(self._h, C.byref(t)))
 
     def getTuning(self):
@@ -451,9 +451,9 @@ class Ensemble:
         return c
 
     def setTuning(self, cell_divisor=1, force_variant=0, lanes_per_particle=16, block_size=256,
-                  steps_per_graph=32):
+                  steps_per_graph=32, sort_variant=1):
         t = _abi.cf_tuning(cell_divisor, force_variant, lanes_per_particle, block_size,
-                           steps_per_graph)
+                           steps_per_graph, sort_variant)
         check(lib().cf_ensemble_set_tuning(self._h, C.byref(t)))
 
     def getTuning(self):
@@ -487,7 +487,7 @@ class Ensemble:
 def _tuning_dict(t) -> dict:
     return dict(cell_divisor=t.cell_divisor, force_variant=t.force_variant,
                 lanes_per_particle=t.lanes_per_particle, block_size=t.block_size,
-                steps_per_graph=t.steps_per_graph)
+                steps_per_graph=t.steps_per_graph, sort_variant=t.sort_variant)
 
 
 def _path(p):
@@ -513,7 +513,7 @@ def tuning_cache_lookup(path, key: str):
 def tuning_cache_store(path, key: str, tuning: dict, us_per_step: float) -> None:
     t = _abi.cf_tuning(tuning["cell_divisor"], tuning["force_variant"],
                        tuning["lanes_per_particle"], tuning["block_size"],
-                       tuning["steps_per_graph"])
+                       tuning["steps_per_graph"], tuning.get("sort_variant", 1))
     check(lib().cf_tuning_cache_store(os.fsencode(path), key.encode(), C.byref(t),
                                       float(us_per_step)))
 

@@ -90,6 +90,10 @@ typedef struct cf_tuning {
   int block_size;               /* threads per workgroup of variant 1: 64, 128, 256 */
   int steps_per_graph;          /* steps per captured HIP graph (rounded up to even);
                                    1 launches every kernel directly */
+  int sort_variant;             /* the step's (cell, index) sort: 1 (default) one pass over
+                                   cell tiles, each placing the particles its window holds;
+                                   0 three passes (scan, fill, rank), the faster one for
+                                   strongly clustered states */
 } cf_tuning;
 
 typedef struct cf_sim cf_sim;

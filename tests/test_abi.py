@@ -34,7 +34,7 @@ def test_struct_layouts():
     assert C.sizeof(_abi.cf_particle) == 44
     assert _abi.cf_particle.ptype.offset == 36 and _abi.cf_particle.pad.offset == 40
     assert C.sizeof(_abi.cf_params) == 14 * 4
-    assert C.sizeof(_abi.cf_tuning) == 5 * 4
+    assert C.sizeof(_abi.cf_tuning) == 6 * 4
 
 
 def test_reference_defaults():

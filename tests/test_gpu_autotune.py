@@ -27,11 +27,14 @@ def test_autotune_leaves_state_and_matches_direct_tuning(tmp_path, edges):
     res = sim.autotune(params, edges=edges, trial_steps=8)
     assert snapshot(sim) == before
     trials = res["trials"]
-    assert len(trials) == 23 and not res["cached"]
+    # 23 force-kernel candidates with the tile sort, then the best two with the counting sort
+    assert len(trials) == 25 and not res["cached"]
+    assert [t["sort_variant"] for t in trials] == [1] * 23 + [0] * 2
     timed = [t for t in trials if not t["pruned"]]
     best = min(timed, key=lambda t: t["us_per_step"])
     assert res["us_per_step"] == pytest.approx(best["us_per_step"], rel=1e-3)
-    assert all(res["tuning"][k] == best[k] for k in ("cell_divisor", "force_variant"))
+    assert all(res["tuning"][k] == best[k] for k in ("cell_divisor", "force_variant",
+                                                     "sort_variant"))
     assert sim.getTuning() == res["tuning"]
     print("chosen", res["tuning"], f"{res['us_per_step']:.2f} us/step")
 
@@ -76,7 +79,7 @@ def test_cache_hit_miss_and_force(tmp_path):
     expect = dict(r1["tuning"], steps_per_graph=8)  # the caller's graph length is kept
     assert r2["tuning"] == expect and b.getTuning() == expect
     r3 = b.autotune(params, trial_steps=4, cache=path, force=True)
-    assert not r3["cached"] and len(r3["trials"]) == 23
+    assert not r3["cached"] and len(r3["trials"]) == 25
 
 
 def test_ensemble_autotune_leaves_state():

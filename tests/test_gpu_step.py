@@ -56,10 +56,12 @@ def test_non_cubic_canvas(orc):
     check_step(sim.getParticleData(), ref, phys, st["vel"])
 
 
-def test_sort_order_matches_oracle(orc):
+@pytest.mark.parametrize("sort", [1, 0])
+def test_sort_order_matches_oracle(orc, sort):
     cfg = small_cfg(n=5000, side=0.6)
     st, phys, params = state_of(cfg)
     sim = make_sim(st, phys)
+    sim.setTuning(sort_variant=sort)
     sim.simulate(params, 3)
     dims, reach = sim.getGrid()
     pos, _, _ = sim.getState()
@@ -70,6 +72,67 @@ def test_sort_order_matches_oracle(orc):
     assert np.array_equal(cells, ref_cells[order])
     # every interacting pair lies within `reach` cells (periodic) of each other
     assert reach == 1
+
+
+def assert_sorted_like_oracle(orc, sim, phys):
+    dims, _ = sim.getGrid()
+    pos, _, _ = sim.getState()
+    cells, index = sim.getSortOrder()
+    ref_cells = orc.bin_cells(pos, dims, phys["canvas"])
+    order = orc.order(ref_cells)
+    assert np.array_equal(index, order)
+    assert np.array_equal(cells, ref_cells[order])
+
+
+def step_and_check(orc, st, phys, params, sort, steps=3):
+    """One step against the oracle, then the canonical order after each of `steps` steps."""
+    sim = make_sim(st, phys)
+    sim.setTuning(sort_variant=sort)
+    sim.simulate(params, 1)
+    ref = orc.step(st["pos"], st["vel"], st["type"], st["prev"], phys, st["table"], st["rbt"], mode=0)
+    assert np.array_equal(sim.getNeighborCounts(), ref["count"])
+    check_step(sim.getParticleData(), ref, phys, st["vel"])
+    assert_sorted_like_oracle(orc, sim, phys)
+    for _ in range(steps - 1):
+        sim.simulate(params, 1)
+        assert_sorted_like_oracle(orc, sim, phys)
+    sim.close()
+
+
+@pytest.mark.parametrize("sort", [1, 0])
+def test_far_movers(orc, sort):
+    """Velocities of several cells per step: most particles leave their tile's window and
+    are placed through the far list."""
+    cfg = small_cfg(n=3000, side=0.5)
+    st, phys, params = state_of(cfg)
+    rng = np.random.default_rng(11)
+    st["vel"] = rng.normal(0, 300.0, st["pos"].shape).astype(np.float32)  # ~0.3 per step
+    step_and_check(orc, st, phys, params, sort)
+
+
+@pytest.mark.parametrize("sort", [1, 0])
+def test_one_cell_beyond_the_lds(orc, sort):
+    """3000 particles in one cell: more than a tile sorts in LDS (the one-cell fallback)."""
+    cfg = small_cfg(n=3000, side=0.5)
+    st, phys, params = state_of(cfg)
+    rng = np.random.default_rng(12)
+    st["pos"] = (0.1 + rng.random(st["pos"].shape) * 0.02).astype(np.float32)
+    step_and_check(orc, st, phys, params, sort, steps=2)
+
+
+@pytest.mark.parametrize("sort", [1, 0])
+def test_tile_beyond_the_lds(orc, sort):
+    """6000 particles in one row of cells: one tile holds more than the LDS takes, no single
+    cell does (the tile is sorted a few cells at a time)."""
+    cfg = small_cfg(n=6000, side=0.5)
+    st, phys, params = state_of(cfg)
+    rng = np.random.default_rng(13)
+    pos = st["pos"].copy()
+    pos[:, 0] = rng.random(len(pos)) * 0.5
+    pos[:, 1] = 0.2 + rng.random(len(pos)) * 0.01
+    pos[:, 2] = 0.3 + rng.random(len(pos)) * 0.01
+    st["pos"] = pos.astype(np.float32)
+    step_and_check(orc, st, phys, params, sort)
 
 
 def test_teacher_forced_steps(orc):
@@ -276,18 +339,19 @@ VARIANTS = [(0, g, d, 128) for d in (1, 2) for g in (4, 8, 16, 32)] + \
     [(3, g, 1, 128) for g in (4, 8, 16, 32)] + [(3, g, 2, 128) for g in (8, 16, 32)]
 
 
+@pytest.mark.parametrize("sort", [1, 0])
 @pytest.mark.parametrize("variant,lanes,divisor,block", VARIANTS)
 @pytest.mark.parametrize("name", ["uniform_3k", "small_grid_generic", "dense_2k"])
-def test_variants_match_oracle(orc, variant, lanes, divisor, block, name):
-    """Every force-kernel variant, lane grouping and stencil reach gives the oracle's counts,
-    forces and edges."""
+def test_variants_match_oracle(orc, variant, lanes, divisor, block, name, sort):
+    """Every force-kernel variant, lane grouping and stencil reach, with either step sort,
+    gives the oracle's counts, forces and edges."""
     cfg = small_cfg(**CASES[name])
     st, phys, params = state_of(cfg)
     rng = np.random.default_rng(11)
     st["prev"] = rng.integers(0, 600, len(st["type"])).astype(np.int32)
     sim = make_sim(st, phys)
     sim.setTuning(cell_divisor=divisor, force_variant=variant, lanes_per_particle=lanes,
-                  block_size=block)
+                  block_size=block, sort_variant=sort)
     sim.simulate(params, 1, edges=(0.02, 6))
     ref = orc.step(st["pos"], st["vel"], st["type"], st["prev"], phys, st["table"], st["rbt"], mode=0)
     assert np.array_equal(sim.getNeighborCounts(), ref["count"])

@@ -14,9 +14,9 @@ from hypothesis import strategies as hs
 from cellflow_amd import CellFlowError, _abi, tuning_cache_lookup, tuning_cache_store
 
 T0 = dict(cell_divisor=1, force_variant=0, lanes_per_particle=16, block_size=256,
-          steps_per_graph=32)
+          steps_per_graph=32, sort_variant=1)
 T1 = dict(cell_divisor=2, force_variant=1, lanes_per_particle=4, block_size=64,
-          steps_per_graph=1)
+          steps_per_graph=1, sort_variant=0)
 
 
 def test_missing_file_is_empty_cache(tmp_path):
@@ -42,8 +42,10 @@ def test_reads_files_written_by_python_json(tmp_path):
     doc = {"version": 7, "note": ["x", {"y": None, "z": [1.5e3, -2, True, False]}],
            "format": "cellflow-tuning",
            "entries": {"aé中\U0001f600/\"q\"\\": dict(T1, us_per_step=3.25, extra={"k": []}),
-                       "b": dict(T0),  # no timing: reads as 0
+                       # no timing: reads as 0; no sort_variant (older files): reads as 1
+                       "b": {k: v for k, v in T0.items() if k != "sort_variant"},
                        "bad-values": dict(T0, block_size=96),
+                       "bad-sort": dict(T0, sort_variant=2),
                        "not-an-object": 5,
                        "incomplete": {"cell_divisor": 1}}}
     for style in (dict(indent=None), dict(indent=4), dict(ensure_ascii=False),
@@ -51,7 +53,7 @@ def test_reads_files_written_by_python_json(tmp_path):
         p.write_text(json.dumps(doc, **style), encoding="utf-8")
         assert tuning_cache_lookup(p, "aé中\U0001f600/\"q\"\\") == (T1, 3.25)
         assert tuning_cache_lookup(p, "b") == (T0, 0.0)
-        for k in ("bad-values", "not-an-object", "incomplete"):
+        for k in ("bad-values", "bad-sort", "not-an-object", "incomplete"):
             assert tuning_cache_lookup(p, k) is None
     # a store keeps the readable entries and drops the unusable ones
     tuning_cache_store(p, "c", T0, 1.0)
@@ -89,7 +91,7 @@ def test_store_rejects_invalid_tuning(tmp_path):
 _tunings = hs.fixed_dictionaries(dict(
     cell_divisor=hs.sampled_from([1, 2]), force_variant=hs.sampled_from([0, 1, 2]),
     lanes_per_particle=hs.sampled_from([4, 8, 16, 32]), block_size=hs.sampled_from([64, 128, 256]),
-    steps_per_graph=hs.integers(1, 1 << 20)))
+    steps_per_graph=hs.integers(1, 1 << 20), sort_variant=hs.sampled_from([0, 1])))
 
 
 @settings(max_examples=60, deadline=None)
