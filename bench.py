@@ -38,8 +38,9 @@ def force_kernel_name(t, edges=True):
     """The force kernel a tuning launches, as rocprofv3 names it."""
     e = "true" if edges else "false"
     r, g = t["cell_divisor"], t["lanes_per_particle"]
-    return {0: f"k_force_grp<{e}, {r}, {g}>", 1: f"k_force<{e}>", 2: f"k_force_lds<{e}, {r}>",
-            3: f"k_force_list<{e}, {r}, {g}>"}[t["force_variant"]]
+    tl = "true" if t.get("sort_variant", 1) == 1 else "false"
+    return {0: f"k_force_grp<{e}, {r}, {g}, {tl}>", 1: f"k_force<{e}>",
+            2: f"k_force_lds<{e}, {r}>", 3: f"k_force_list<{e}, {r}, {g}, {tl}>"}[t["force_variant"]]
 
 
 def read_traffic(path, kernel):

@@ -178,6 +178,7 @@ struct Engine {
   uint32_t* far_slot = nullptr;
   uint32_t* ftn[2] = {nullptr, nullptr};  // far movers per tile
   uint32_t* ftl = nullptr;
+
   uint32_t* tile_sim = nullptr;
   int ntiles = 0, ntiles_cap = 0, netiles = 0;
   std::vector<uint32_t> htile_sim;
@@ -615,13 +616,13 @@ struct Engine {
     fa.nwaves = nwaves;
     fa.edges = edges ? 1 : 0;
     fa.edge_cap = cap;
+    fa.tile_sort = tun.sort_variant == 1 ? 1 : 0;
     fa.tcount = tcnt[A];
     fa.etcount = etcnt[A];
     fa.far_n = farn + A;
     fa.far_slot = far_slot;
     fa.ftn = ftn[A];
     fa.ftl = ftl;
-    fa.tile_sort = tun.sort_variant == 1 ? 1 : 0;
     return fa;
   }
 

@@ -150,8 +150,8 @@ struct ForceArgs {
   int nwaves;
   int edges;        // fused edge candidates requested for this launch
   int edge_cap;
-  // 1: the tile sort follows (keys with kFarBit, tile counts, far lists); 0: the three-pass
-  // counting sort follows (arrival ranks)
+  // 1: the tile sort follows (keys with kFarBit, tile counts, far lists, estage by owner
+  // index); 0: the three-pass counting sort follows (arrival ranks, estage by slot)
   int tile_sort;
   // tile sort inputs (this step's counters; the sort clears the next step's)
   uint32_t* tcount;     // members per cell tile (tile t at t * kCtrStride)
@@ -232,7 +232,7 @@ struct TileSortArgs {
   const uint32_t* ecount;
   const uint32_t* etcount;
   uint32_t* etcount_next;
-  const uint32_t* estage;               // [rank][owner id]
+  const uint32_t* estage;               // [rank][owner index]
   int2* edges;
   long edge_room;                       // entries `edges` holds
   const uint32_t* __restrict__ wave_sim;

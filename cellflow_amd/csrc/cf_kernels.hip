@@ -120,7 +120,8 @@ __device__ __forceinline__ void edge_push(EdgeAcc& e, float d2, uint32_t jid, in
 
 // Stable sort of the kept candidates by squared distance (ties by index, which is the
 // order the reference's insertion sort leaves, ParticleSimulation.cu:235-243), truncated
-// to maxc (:246).  Writes estage[rank][o] (o: the owner's index) and returns the kept count.
+// to maxc (:246).  Writes estage[rank][o] and returns the kept count.  o is the owner's
+// slot, or its index when the tile sort (which writes edges in index order) follows.
 __device__ __forceinline__ int edge_finish(const EdgeAcc& e, int cap, int maxc, const float* cd2,
                                            const uint32_t* cid, uint32_t* estage, long P, int p,
                                            uint32_t o) {
@@ -144,12 +145,15 @@ __device__ __forceinline__ int edge_finish(const EdgeAcc& e, int cap, int maxc, 
 // kFarBit (and a place in a far list) when the pre-step cell lies outside the window of
 // the new cell's tile, and the per-cell and per-tile counts.  The window test is the one
 // k_tile_sort enumerates.
+// TILE selects the sort at compile time for the kernels the tuner picks (a runtime flag
+// there costs the stencil loop registers); the others pass a.tile_sort through.
+template <bool TILE>
 __device__ __forceinline__ void commit_cell(const ForceArgs& a, const SimDesc& sd, int p, int cyo,
                                             int czo, float x, float y, float z) {
   const int xn = bin_axis(x, sd.ihx, sd.nx), yn = bin_axis(y, sd.ihy, sd.ny),
             zn = bin_axis(z, sd.ihz, sd.nz);
   const uint32_t c = (uint32_t)(sd.c0 + xn + sd.nx * (yn + sd.ny * zn));
-  if (!a.tile_sort) {  // the counting sort ranks by arrival
+  if constexpr (!TILE) {  // the counting sort ranks by arrival
     a.key[p] = c;
     a.arrival[p] = atomicAdd(&a.hist[c], 1u);
     return;
@@ -187,9 +191,11 @@ __device__ __forceinline__ void commit_cell(const ForceArgs& a, const SimDesc& s
   }
 }
 
-// Edge count of an owner into its edge tile's count (one of kEdgeSub parts).
+// For the tile sort: an owner's edge count into its edge tile's count (one of kEdgeSub
+// parts).
+template <bool TILE>
 __device__ __forceinline__ void count_edges(const ForceArgs& a, uint32_t idi, uint32_t e) {
-  if (a.tile_sort && e > 0u)
+  if (TILE && e > 0u)
     atomicAdd(&a.etcount[((idi / kEdgeTile) * kEdgeSub + (blockIdx.x & (kEdgeSub - 1))) * kCtrStride],
               e);
 }
@@ -350,14 +356,15 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs a) {
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
   a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  commit_cell(a, sd, p, cy, cz, x, y, z);
+  if (a.tile_sort) commit_cell<true>(a, sd, p, cy, cz, x, y, z);
+  else commit_cell<false>(a, sd, p, cy, cz, x, y, z);
 
   if (EDGES) {
     if (do_edges) {
       const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage,
-                                a.P, p, idi);
+                                a.P, p, a.tile_sort ? idi : (uint32_t)p);
       a.ecount[idi] = (uint32_t)e;
-      count_edges(a, idi, (uint32_t)e);
+      if (a.tile_sort) count_edges<true>(a, idi, (uint32_t)e);
     }
   }
 }
@@ -642,13 +649,14 @@ __global__ void __launch_bounds__(64) k_force_lds(ForceArgs a) {
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(acc.cnt));
   a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  commit_cell(a, sd, p, cy, cz, x, y, z);
+  if (a.tile_sort) commit_cell<true>(a, sd, p, cy, cz, x, y, z);
+  else commit_cell<false>(a, sd, p, cy, cz, x, y, z);
   if (EDGES) {
     if (do_edges) {
       const int e = edge_finish(acc.ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id,
-                                a.estage, a.P, p, idi);
+                                a.estage, a.P, p, a.tile_sort ? idi : (uint32_t)p);
       a.ecount[idi] = (uint32_t)e;
-      count_edges(a, idi, (uint32_t)e);
+      if (a.tile_sort) count_edges<true>(a, idi, (uint32_t)e);
     }
   }
 }
@@ -705,7 +713,7 @@ struct GrpAcc {
   int cnt;
 };
 
-template <bool EDGES, int R, int G>
+template <bool EDGES, int R, int G, bool TILE>
 __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   constexpr int kRows = (2 * R + 1) * (2 * R + 1);
   constexpr int kRuns = 2 * kRows;
@@ -954,7 +962,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
           const uint32_t i2 = el_i[wv][o][k2];
           rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
         }
-        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + idi] = ik;
+        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + (TILE ? idi : (uint32_t)p)] = ik;
       }
     if (do_edges && g == 0) {
       uint32_t e;
@@ -984,10 +992,11 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
             if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
           }
         }
-        e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
+        e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
+                                  TILE ? idi : (uint32_t)p);
       }
       a.ecount[idi] = e;
-      count_edges(a, idi, e);
+      count_edges<TILE>(a, idi, e);
     }
   }
 
@@ -1012,7 +1021,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
   a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  commit_cell(a, sd, p, cy, cz, x, y, z);
+  commit_cell<TILE>(a, sd, p, cy, cz, x, y, z);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1148,7 +1157,7 @@ __device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, 
   }
 }
 
-template <bool EDGES, int R, int G>
+template <bool EDGES, int R, int G, bool TILE>
 // 27-cell stencil at G = 8 fits 7 waves per SIMD (72 VGPRs, under 22.8 KB of LDS per block)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 1 && G == 8 ? 7 : 1, 8)))
 k_force_list(ForceArgs a) {
@@ -1318,7 +1327,7 @@ k_force_list(ForceArgs a) {
           const uint32_t i2 = el_i[wv][o][k2];
           rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
         }
-        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + idi] = ik;
+        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + (TILE ? idi : (uint32_t)p)] = ik;
       }
     if (do_edges && g == 0) {
       uint32_t e;
@@ -1351,10 +1360,11 @@ k_force_list(ForceArgs a) {
             }
           }
         }
-        e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
+        e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
+                                  TILE ? idi : (uint32_t)p);
       }
       a.ecount[idi] = e;
-      count_edges(a, idi, e);
+      count_edges<TILE>(a, idi, e);
     }
   }
 
@@ -1379,7 +1389,7 @@ k_force_list(ForceArgs a) {
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
   a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  commit_cell(a, sd, p, c.cy, c.cz, x, y, z);
+  commit_cell<TILE>(a, sd, p, c.cy, c.cz, x, y, z);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1426,7 +1436,7 @@ __global__ void __launch_bounds__(256) k_edges(EdgeArgs a) {
       }
     }
   const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
-                            idi);
+                            (uint32_t)p);
   a.ecount[idi] = (uint32_t)e;
 }
 
@@ -1830,7 +1840,7 @@ __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
     const uint32_t off = a.eoff[me];
     const int e = (int)(a.eoff[me + 1] - off);
     for (int k = 0; k < e; ++k)
-      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + me] - sd.p0));
+      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
   }
 }
 
@@ -1875,7 +1885,7 @@ __global__ void __launch_bounds__(256) k_edge_write(SortArgs a) {
   const uint32_t off = a.eoff[me];
   const int e = (int)(a.eoff[me + 1] - off);
   for (int k = 0; k < e; ++k)
-    a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + me] - sd.p0));
+    a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
 }
 
 // ------------------------------------------------------------------------------------
@@ -2064,16 +2074,26 @@ template <int R, int G>
 static void launch_grp(const ForceArgs& a, bool edges, hipStream_t st) {
   const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
   const dim3 g((unsigned)((waves + 3) / 4)), b(256);
-  if (edges) hipLaunchKernelGGL((k_force_grp<true, R, G>), g, b, 0, st, a);
-  else hipLaunchKernelGGL((k_force_grp<false, R, G>), g, b, 0, st, a);
+  if (a.tile_sort) {
+    if (edges) hipLaunchKernelGGL((k_force_grp<true, R, G, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_force_grp<false, R, G, true>), g, b, 0, st, a);
+  } else {
+    if (edges) hipLaunchKernelGGL((k_force_grp<true, R, G, false>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_force_grp<false, R, G, false>), g, b, 0, st, a);
+  }
 }
 
 template <int R, int G>
 static void launch_list(const ForceArgs& a, bool edges, hipStream_t st) {
   const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
   const dim3 g((unsigned)((waves + 3) / 4)), b(256);
-  if (edges) hipLaunchKernelGGL((k_force_list<true, R, G>), g, b, 0, st, a);
-  else hipLaunchKernelGGL((k_force_list<false, R, G>), g, b, 0, st, a);
+  if (a.tile_sort) {
+    if (edges) hipLaunchKernelGGL((k_force_list<true, R, G, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_force_list<false, R, G, true>), g, b, 0, st, a);
+  } else {
+    if (edges) hipLaunchKernelGGL((k_force_list<true, R, G, false>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_force_list<false, R, G, false>), g, b, 0, st, a);
+  }
 }
 
 hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
