@@ -1527,6 +1527,7 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   __shared__ uint32_t mid[kTileCap];
   __shared__ uint16_t mlc[kTileCap];
   __shared__ uint32_t red[kSortThreads / kWave];
+  __shared__ uint32_t wq0[6], wq1[6];  // the window's slot runs
   __shared__ int s_next;
   const int tid = threadIdx.x;
   if ((int)blockIdx.x >= a.ntiles) {
@@ -1544,6 +1545,25 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   const int lt = t - sd.t0, zt = lt / sd.nyt, y0 = (lt % sd.nyt) * sd.tr;
   const int nrow = min(sd.tr, sd.ny - y0), nc = sd.nx * nrow;
   const uint32_t cA = (uint32_t)(sd.c0 + sd.nx * (y0 + sd.ny * zt));
+
+  // Window: planes zt-1, zt, zt+1 and rows y0-1 .. y0+nrow, cyclic and distinct; each
+  // plane's rows are one or two runs of consecutive cells, so of consecutive slots.  The
+  // runs' bounds are loaded first: they depend on nothing the counts below decide.
+  const int np = min(3, sd.nz), nrw = min(nrow + 2, sd.ny);
+  const int ys = (y0 - 1 + sd.ny) % sd.ny, yrun0 = min(nrw, sd.ny - ys);
+  if (tid < 6) {
+    const int ip = tid >> 1, h = tid & 1;
+    const int ra = h ? 0 : ys, rn = h ? nrw - yrun0 : yrun0;
+    uint32_t q0 = 0u, q1 = 0u;
+    if (ip < np && rn > 0) {
+      const int zp = (zt - 1 + ip + sd.nz) % sd.nz;
+      const int cf = sd.c0 + sd.nx * (ra + sd.ny * zp);
+      q0 = a.start_old[cf];
+      q1 = a.start_old[cf + sd.nx * rn];
+    }
+    wq0[tid] = q0;
+    wq1[tid] = q1;
+  }
 
   // first slot of the tile, then its cells' starts (counts cleared for the next step)
   uint32_t part = 0;
@@ -1582,33 +1602,25 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   __syncthreads();
   if (!ok) return;
 
-  // Window: planes zt-1, zt, zt+1 and rows y0-1 .. y0+nrow, cyclic and distinct; each
-  // plane's rows are one or two runs of consecutive cells, so of consecutive slots.
-  const int np = min(3, sd.nz), nrw = min(nrow + 2, sd.ny);
-  const int ys = (y0 - 1 + sd.ny) % sd.ny, yrun0 = min(nrw, sd.ny - ys);
+  // fn(q, key, id) for every slot of the window and of the far lists (the index is loaded
+  // with the key: most slots are not members, but a dependent load per member costs more)
   auto visit = [&](auto&& fn) {
-    for (int ip = 0; ip < np; ++ip) {
-      const int zp = (zt - 1 + ip + sd.nz) % sd.nz;
-      for (int h = 0; h < 2; ++h) {
-        const int ra = h ? 0 : ys, rn = h ? nrw - yrun0 : yrun0;
-        if (rn <= 0) continue;
-        const int cf = sd.c0 + sd.nx * (ra + sd.ny * zp);
-        const uint32_t q0 = a.start_old[cf], q1 = a.start_old[cf + sd.nx * rn];
-        for (uint32_t q = q0 + tid; q < q1; q += kSortThreads) {
-          const uint32_t k = a.key[q];
-          if (!(k & kFarBit)) fn(q, k);
-        }
+    for (int sgm = 0; sgm < 6; ++sgm) {
+      const uint32_t q0 = wq0[sgm], q1 = wq1[sgm];
+      for (uint32_t q = q0 + tid; q < q1; q += kSortThreads) {
+        const uint32_t k = a.key[q], id = a.src_id[q];
+        if (!(k & kFarBit)) fn(q, k, id);
       }
     }
     const uint32_t nt = min(a.ftn[t * kCtrStride], (uint32_t)kFarCap);
     for (uint32_t j = tid; j < nt; j += kSortThreads) {
       const uint32_t q = a.ftl[(long)t * kFarCap + j];
-      fn(q, a.key[q] & ~kFarBit);
+      fn(q, a.key[q] & ~kFarBit, a.src_id[q]);
     }
     const uint32_t nf = min(*a.far_n, (uint32_t)a.P);
     for (uint32_t j = tid; j < nf; j += kSortThreads) {
       const uint32_t q = a.far_slot[j];
-      fn(q, a.key[q] & ~kFarBit);
+      fn(q, a.key[q] & ~kFarBit, a.src_id[q]);
     }
   };
 
@@ -1632,7 +1644,7 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
       for (uint32_t m = tid; m < mn; m += kSortThreads) mlc[m] = 0xFFFFu;
       __syncthreads();
       const uint32_t ca = cA + (uint32_t)ia, cb = cA + (uint32_t)ib;
-      visit([&](uint32_t q, uint32_t k) {
+      visit([&](uint32_t q, uint32_t k, uint32_t id) {
         if (k < ca || k >= cb) return;
         const uint32_t lc = k - cA;
         const uint32_t at = atomicAdd(&fill_s[lc], 1u);
@@ -1642,7 +1654,7 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
         }
         const uint32_t m = cofs[lc] - m0 + at;
         mslot[m] = q;
-        mid[m] = a.src_id[q];
+        mid[m] = id;
         mlc[m] = (uint16_t)lc;
       });
       __syncthreads();
@@ -1652,12 +1664,14 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
           atomicOr(a.flag, 4u);  // a cell lost a particle: leave its slots alone
           continue;
         }
+        const uint32_t q = mslot[m];
+        const float4 sp = a.src_pos[q], sv = a.src_vel[q];  // in flight during the ranking
         const uint32_t s0 = cofs[lc] - m0, sn = cnt_s[lc], me = mid[m];
         uint32_t rank = 0;
         for (uint32_t e = 0; e < sn; ++e) rank += (mid[s0 + e] < me) ? 1u : 0u;
-        const uint32_t dst = base + cofs[lc] + rank, q = mslot[m];
-        a.dst_pos[dst] = a.src_pos[q];
-        a.dst_vel[dst] = a.src_vel[q];
+        const uint32_t dst = base + cofs[lc] + rank;
+        a.dst_pos[dst] = sp;
+        a.dst_vel[dst] = sv;
         a.dst_id[dst] = me;
       }
     } else {
@@ -1669,14 +1683,14 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
         atomicOr(a.flag, 2u);
       }
       __syncthreads();
-      visit([&](uint32_t q, uint32_t k) {
+      visit([&](uint32_t q, uint32_t k, uint32_t id) {
         if (k != cg) return;
         const uint32_t at = atomicAdd(&fill_s[ia], 1u);
         if (at >= K) {
           atomicOr(a.flag, 4u);
           return;
         }
-        a.tmp_id[g0 + at] = a.src_id[q];
+        a.tmp_id[g0 + at] = id;
         a.tmp_slot[g0 + at] = q;
       });
       __syncthreads();
