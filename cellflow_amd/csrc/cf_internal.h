@@ -22,7 +22,7 @@ constexpr int kMaxEdgeCap = 64;        // 2 * CF_MAX_CONNECTIONS
 // in their key and are listed apart.
 constexpr int kTileCells = 2048;       // most cells in one tile (tr * nx)
 constexpr int kTileCap = 2048;         // members a tile sorts in LDS at once
-constexpr int kTileTarget = 192;       // expected members of a tile (chooses tr)
+constexpr int kTileTarget = 384;       // expected members of a tile (chooses tr)
 constexpr int kEdgeTile = 1024;        // owner ids per edge tile
 constexpr uint32_t kFarBit = 0x80000000u;
 // Tile and edge-tile counters sit one per 128-byte line: a line takes every atomic of the
