@@ -118,14 +118,15 @@ constexpr int kAdjBlock = 256;                 // threads per block
 constexpr int kAdjPer = kAdjBlock / kAdjG;     // particles per block
 constexpr int kAdjVals = 4 + kMaxTypes;        // lx, ly, lz, ratio, table row
 
-// Visit the slots of the stencil cells c = first, first + step, ... of the (2r+1)^3 cube
-// around (cx, cy, cz) (every cell once on axes with fewer than 2r+1 cells), in a fixed
-// order.  Cell coordinates come from float reciprocals: (c + 1/2) / m is at least 1/(2m)
-// away from an integer and m <= 25, far beyond the float error.
+// Visit the stencil cells c = first, first + step, ... of the (2r+1)^3 cube around
+// (cx, cy, cz) (every cell once on axes with fewer than 2r+1 cells), in a fixed order:
+// for_cell_runs passes each cell's slot range, for_cells_split each of its slots.  Cell
+// coordinates come from float reciprocals: (c + 1/2) / m is at least 1/(2m) away from an
+// integer and m <= 25, far beyond the float error.
 template <class Fn>
-__device__ __forceinline__ void for_cells_split(const SimDesc& sd, const uint32_t* start, int cx,
-                                                int cy, int cz, int r, int first, int step,
-                                                Fn&& fn) {
+__device__ __forceinline__ void for_cell_runs(const SimDesc& sd, const uint32_t* start, int cx,
+                                              int cy, int cz, int r, int first, int step,
+                                              Fn&& fn) {
   const bool gx = sd.nx < 2 * r + 1, gy = sd.ny < 2 * r + 1, gz = sd.nz < 2 * r + 1;
   const int nxo = gx ? sd.nx : 2 * r + 1, nyo = gy ? sd.ny : 2 * r + 1,
             nzo = gz ? sd.nz : 2 * r + 1;
@@ -140,10 +141,18 @@ __device__ __forceinline__ void for_cells_split(const SimDesc& sd, const uint32_
     if (x < 0) x += sd.nx; else if (x >= sd.nx) x -= sd.nx;
     if (y < 0) y += sd.ny; else if (y >= sd.ny) y -= sd.ny;
     if (z < 0) z += sd.nz; else if (z >= sd.nz) z -= sd.nz;
-    const int c = sd.c0 + x + sd.nx * (y + sd.ny * z);
-    const uint32_t q1 = start[c + 1];
-    for (uint32_t q = start[c]; q < q1; ++q) fn(q);
+    const int cc = sd.c0 + x + sd.nx * (y + sd.ny * z);
+    fn(start[cc], start[cc + 1]);
   }
+}
+
+template <class Fn>
+__device__ __forceinline__ void for_cells_split(const SimDesc& sd, const uint32_t* start, int cx,
+                                                int cy, int cz, int r, int first, int step,
+                                                Fn&& fn) {
+  for_cell_runs(sd, start, cx, cy, cz, r, first, step, [&](uint32_t q0, uint32_t q1) {
+    for (uint32_t q = q0; q < q1; ++q) fn(q);
+  });
 }
 
 // Histogram partials, G lanes per particle.  The group's lanes add each pair's four spline
@@ -259,29 +268,91 @@ __device__ __forceinline__ uint32_t group_excl_scan(uint32_t v, int g) {
   return x - v;
 }
 
-// One backward step's pair terms, G lanes per particle.  Few of a particle's stencil
-// candidates are in range (about 1 in 15 on a uniform state), so evaluating the FP64 terms
-// where they are found would run the FP64 body on nearly every step of the walk for some
-// lane of the wave.  Instead the walk only takes the forward's FP32 decision and appends
-// the in-range slots to the particle's list in LDS; the list is put in slot order (rank
-// by counting, slots are distinct) and its entries are evaluated densely, entry e by lane
-// e mod G.  A particle with more than kAdjHits pairs takes them in windows of kAdjHits in
-// lane-major walk order (lane g's pairs after those of lanes below g), one more walk per
-// window.
+constexpr int kAdjHits = 64;
+
+// The pair kernel's particle context and its FP32 in-range decision, shared by the walk
+// kernel and the pair kernel.
+struct AdjOwn {
+  float4 pi;
+  int p, ti, cx, cy, cz;
+};
+
+__device__ __forceinline__ AdjOwn adj_own(const AdjArgs& a, const SimDesc& sd, int k, bool active) {
+  AdjOwn c;
+  c.p = sd.p0 + k;
+  c.pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  c.ti = -1;
+  c.cx = c.cy = c.cz = 0;
+  if (active) {
+    c.pi = a.pos[c.p];
+    c.ti = (int)__float_as_uint(c.pi.w);
+    c.cx = bin_axis_a(c.pi.x, sd.ihx, sd.nx);
+    c.cy = bin_axis_a(c.pi.y, sd.ihy, sd.ny);
+    c.cz = bin_axis_a(c.pi.z, sd.ihz, sd.nz);
+  }
+  return c;
+}
+
+// the forward's FP32 decision (ParticleSimulation.cu:92-112 with the engine's op order)
+__device__ __forceinline__ bool adj_in_range(const AdjArgs& a, const SimDesc& sd, const AdjOwn& c,
+                                             uint32_t q) {
+  if ((int)q == c.p) return false;
+  const float4 pj = a.pos[q];
+  const int tj = (int)__float_as_uint(pj.w);
+  const float fx = min_image_f(pj.x - c.pi.x, sd.Lx), fy = min_image_f(pj.y - c.pi.y, sd.Ly),
+              fz = min_image_f(pj.z - c.pi.z, sd.Lz);
+  const float s2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+  return s2f + 1e-4f < a.thr[c.ti * a.c.T + tj];
+}
+
+// The walk: G lanes per particle take the FP32 decision over the stencil and list the
+// in-range slots in LDS, which are then put in slot order (rank by counting, slots are
+// distinct) and written to hits[k][0..); nhits[k] is the particle's pair count, also when
+// it exceeds kAdjHits (the pair kernel then walks again).  No FP64 here, so the walk runs
+// at the occupancy its few registers allow.
+__global__ void __launch_bounds__(kAdjBlock) k_adj_walk(AdjArgs a) {
+  constexpr int G = kAdjG;
+  __shared__ uint32_t hit[kAdjPer][kAdjHits];
+  __shared__ uint32_t nhit[kAdjPer];
+  const SimDesc sd = a.sims[0];
+  const int o = threadIdx.x / G, g = threadIdx.x % G;
+  const int k = blockIdx.x * kAdjPer + o;
+  const bool active = k < sd.n;
+  const AdjOwn c = adj_own(a, sd, k, active);
+  if (g == 0) nhit[o] = 0u;
+  __syncthreads();
+  if (active)
+    for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
+      if (!adj_in_range(a, sd, c, q)) return;
+      const uint32_t at = atomicAdd(&nhit[o], 1u);
+      if (at < (uint32_t)kAdjHits) hit[o][at] = q;
+    });
+  __syncthreads();
+  if (!active) return;
+  const uint32_t nh = nhit[o];
+  if (g == 0) a.nhits[k] = nh;
+  if (nh <= (uint32_t)kAdjHits)
+    for (uint32_t e = g; e < nh; e += G) {
+      const uint32_t q = hit[o][e];
+      uint32_t rank = 0;
+      for (uint32_t e2 = 0; e2 < nh; ++e2) rank += (hit[o][e2] < q) ? 1u : 0u;
+      a.hits[(long)k * kAdjHits + rank] = q;
+    }
+}
+
+// One backward step's pair terms, G lanes per particle, over the walk's slot-ordered lists:
+// entry e by lane e mod G.  A particle with more than kAdjHits pairs walks again and takes
+// them in windows of kAdjHits in lane-major walk order (lane g's pairs after those of the
+// lanes below g), one more walk per window.
 // Per particle: lane sums reduced in lane order; per block: the table and ratio partials
 // of its particles summed in particle order and added to one column of `partial` (row r at
 // partial[r * stride + block]), which so sums the steps of the rollout in the order they are
-// reversed.  Every order is fixed by the data, so the results are
-// deterministic.
-constexpr int kAdjHits = 64;
-
+// reversed.  Every order is fixed by the data, so the results are deterministic.
 __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
   constexpr int G = kAdjG;
   __shared__ double red[kAdjBlock][kAdjVals];
   __shared__ int btype[kAdjPer];
-  __shared__ uint32_t hit[kAdjPer][kAdjHits];
   __shared__ uint32_t hsorted[kAdjPer][kAdjHits];
-  __shared__ uint32_t nhit[kAdjPer];
   const SimDesc sd = a.sims[0];
   const int o = threadIdx.x / G, g = threadIdx.x % G;
   const int k = blockIdx.x * kAdjPer + o;
@@ -291,31 +362,16 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
   double gfv[kMaxTypes];
 #pragma unroll
   for (int t = 0; t < kMaxTypes; ++t) gfv[t] = 0.0;
-  const int p = sd.p0 + k;
-  if (g == 0) nhit[o] = 0u;
-  float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  uint32_t idi = 0;
-  int ti = -1, cx = 0, cy = 0, cz = 0;
+  const AdjOwn c = adj_own(a, sd, k, active);
+  const int p = c.p, ti = c.ti;
+  const float4 pi = c.pi;
+  uint32_t idi = 0, nh = 0;
   double4 phi_i = make_double4(0.0, 0.0, 0.0, 0.0);
   if (active) {
-    pi = a.pos[p];
     idi = a.id[p];
-    ti = (int)__float_as_uint(pi.w);
-    cx = bin_axis_a(pi.x, sd.ihx, sd.nx);
-    cy = bin_axis_a(pi.y, sd.ihy, sd.ny);
-    cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
     phi_i = a.phi[p];
+    nh = a.nhits[k];
   }
-  // the forward's FP32 decision (ParticleSimulation.cu:92-112 with the engine's op order)
-  auto in_range = [&](uint32_t q) {
-    if ((int)q == p) return false;
-    const float4 pj = a.pos[q];
-    const int tj = (int)__float_as_uint(pj.w);
-    const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
-                fz = min_image_f(pj.z - pi.z, sd.Lz);
-    const float s2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-    return s2f + 1e-4f < a.thr[ti * T + tj];
-  };
   // the FP64 terms of one in-range pair
   auto pair = [&](uint32_t q) {
     const double R = a.c.radius, rho = a.c.ratio, rep = a.c.rep, att = a.c.att, kk = a.c.k;
@@ -346,43 +402,33 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
     for (int t = 0; t < kMaxTypes; ++t) gfv[t] += (t == tj) ? gd : 0.0;
     grat += fij * (-gp * r / eff) * dR * dpi;
   };
-  __syncthreads();  // list counters cleared
-  uint32_t nmine = 0u;  // this lane's pairs
-  if (active)
-    for_cells_split(sd, a.start, cx, cy, cz, sd.reach, g, G, [&](uint32_t q) {
-      if (!in_range(q)) return;
-      ++nmine;
-      const uint32_t at = atomicAdd(&nhit[o], 1u);
-      if (at < (uint32_t)kAdjHits) hit[o][at] = q;
+  const bool listed = nh <= (uint32_t)kAdjHits;
+  // a particle beyond the list: this lane's pairs, and where they start in lane-major order
+  uint32_t nmine = 0u;
+  if (active && !listed)
+    for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
+      if (adj_in_range(a, sd, c, q)) ++nmine;
     });
   const uint32_t first = group_excl_scan<G>(nmine, g);
-  __syncthreads();
-  const uint32_t nh = active ? nhit[o] : 0u;
-  const bool listed = nh <= (uint32_t)kAdjHits;
-  if (listed)
-    for (uint32_t e = g; e < nh; e += G) {
-      const uint32_t q = hit[o][e];
-      uint32_t rank = 0;
-      for (uint32_t e2 = 0; e2 < nh; ++e2) rank += (hit[o][e2] < q) ? 1u : 0u;
-      hsorted[o][rank] = q;
-    }
-  __syncthreads();
-  // a group's lanes share one wave: LDS writes and reads between them are ordered by
-  // wavefront fences, so the windows need no block barrier
-  for (uint32_t w0 = 0; w0 < nh; w0 += (uint32_t)kAdjHits) {
-    if (!listed) {
+  if (listed) {
+#pragma nounroll
+    for (uint32_t e = g; e < nh; e += G) pair(a.hits[(long)k * kAdjHits + e]);
+  } else {
+    // a group's lanes share one wave: LDS writes and reads between them are ordered by
+    // wavefront fences, so the windows need no block barrier
+    for (uint32_t w0 = 0; w0 < nh; w0 += (uint32_t)kAdjHits) {
       uint32_t idx = first;
-      for_cells_split(sd, a.start, cx, cy, cz, sd.reach, g, G, [&](uint32_t q) {
-        if (!in_range(q)) return;
+      for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
+        if (!adj_in_range(a, sd, c, q)) return;
         if (idx >= w0 && idx < w0 + (uint32_t)kAdjHits) hsorted[o][idx - w0] = q;
         ++idx;
       });
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    }
-    const uint32_t nb = min(nh - w0, (uint32_t)kAdjHits);
+      const uint32_t nb = min(nh - w0, (uint32_t)kAdjHits);
 #pragma nounroll
-    for (uint32_t e = g; e < nb; e += G) pair(hsorted[o][e]);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // window reused by the next
+      for (uint32_t e = g; e < nb; e += G) pair(hsorted[o][e]);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // window reused by the next
+    }
   }
   double* mine = red[threadIdx.x];
   mine[0] = lx;
@@ -440,6 +486,7 @@ hipError_t launch_hist_grad(const HistArgs& a, int n, hipStream_t st) {
 hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st) {
   const int g = (n + 127) / 128;
   hipLaunchKernelGGL(k_adj_phi, dim3(g), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(k_adj_walk, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
   hipLaunchKernelGGL(k_adj_pair_g, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
   return hipGetLastError();
 }

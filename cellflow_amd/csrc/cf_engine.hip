@@ -1178,6 +1178,7 @@ struct Engine {
     std::vector<float4*> tv(ckpt, nullptr);  // each segment step's new velocities + counts
     StateBuf fin;
     double4 *lx[2] = {nullptr, nullptr}, *lv[2] = {nullptr, nullptr}, *phi = nullptr;
+    uint32_t *hits = nullptr, *nhits = nullptr;
     double *partial = nullptr, *hd = nullptr, *gh = nullptr, *gacc = nullptr;
     float* thr = nullptr;
     int rc = CF_OK;
@@ -1192,6 +1193,8 @@ struct Engine {
         dfree(lv[i]);
       }
       dfree(phi);
+      dfree(hits);
+      dfree(nhits);
       dfree(partial);
       dfree(hd);
       dfree(gh);
@@ -1208,6 +1211,8 @@ struct Engine {
         CFR(dalloc(&lv[i], P));
       }
       CFR(dalloc(&phi, P));
+      CFR(dalloc(&hits, (size_t)std::max(n, 1) * 64));
+      CFR(dalloc(&nhits, std::max(n, 1)));
       const int rows = std::max(bins, T * T + 1);
       CFR(dalloc(&partial, (size_t)rows * stride));
       CFR(dalloc(&hd, bins));
@@ -1273,6 +1278,8 @@ struct Engine {
       for (int t = 0; t < T; ++t) aa.c.rbt[t] = h.rbt[t];
       for (int t = 0; t < T * T; ++t) aa.c.table[t] = h.table[t];
       aa.phi = phi;
+      aa.hits = hits;
+      aa.nhits = nhits;
       aa.partial = partial;
       aa.stride = nblk;
       int L = 0;  // lx[L] holds dL/dx of the state after the step being reversed

@@ -333,6 +333,8 @@ struct AdjArgs {
   double4* lam_x;
   double4* lam_v;
   double4* phi;
+  uint32_t* hits;           // [n][64] in-range slots of each particle, slot order (walk)
+  uint32_t* nhits;          // [n] their number (more than 64: the pair kernel walks again)
   double* partial;          // [T*T + 1][stride] parameter partials, one column per block,
                             // added to (the caller zeroes them once per rollout)
   long stride;              // >= adj_pair_blocks(n)
