@@ -69,6 +69,14 @@ __device__ __forceinline__ int xcd_map(int b, int nb) {
 
 __device__ __forceinline__ int xcd_block() { return xcd_map((int)blockIdx.x, (int)gridDim.x); }
 
+// Orders a wave's LDS writes before later LDS reads by other lanes of the same wave (a
+// wave's LDS operations complete in issue order).  The force kernels keep every LDS table
+// per wave, so their waves need no workgroup barrier and never wait for each other.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
@@ -246,7 +254,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs a) {
   const SimDesc sd = a.sims[s];
   if (wave_ok)
     for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = a.pairs[sd.tab + t];
-  __syncthreads();
+  wave_lds_sync();
   if (!wave_ok) return;
   const int p = gw * kWave + lane;
   if (p - sd.p0 >= sd.n) return;
@@ -751,7 +759,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   const int nzo = gen && sd.nz < 2 * R + 1 ? sd.nz : 2 * R + 1;
   const int nrows = nyo * nzo;
   if (EDGES && g == 0) el_n[wv][o] = 0u;
-  __syncthreads();  // pair table ready
+  wave_lds_sync();  // pair table ready
 
   const PairConst* row = &tab[wv][ti * sd.T];
   // ---- run table ------------------------------------------------------------------
@@ -826,7 +834,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
       runs[wv][o][2 * r + 1] = e1;
     }
   }
-  __syncthreads();
+  wave_lds_sync();
 
   // ---- candidates -----------------------------------------------------------------
   const bool do_edges = EDGES && a.edges && sd.edge_on && active;
@@ -936,8 +944,8 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   if (active && 1e-4f < row[ti].thr) cnt -= 1;  // the particle itself
 
   if (EDGES) {
-    // barriers stay outside lane-dependent branches
-    __syncthreads();
+    // the wave's candidate lists are complete
+    wave_lds_sync();
     const uint32_t n = do_edges ? el_n[wv][o] : 0u;
     const bool fits = n <= (uint32_t)kEdgeList;
     const int cap = a.edge_cap, maxc = sd.edge_maxc;
@@ -949,7 +957,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
         for (uint32_t k2 = 0; k2 < n; ++k2) below += (el_i[wv][o][k2] < ik) ? 1u : 0u;
         el_d[wv][o][k].y = (below < (uint32_t)cap) ? 1.0f : 0.0f;
       }
-    __syncthreads();
+    wave_lds_sync();
     // rank of each member of S by (distance, index); keep the first maxc
     if (fits)
       for (uint32_t k = g; k < n; k += G) {
@@ -1225,7 +1233,7 @@ k_force_list(ForceArgs a) {
   }
   const uint32_t first = group_excl_scan<G>(mine, g);
   const uint32_t total = (uint32_t)dpp_group_sum_i<G>((int)mine);
-  __syncthreads();  // pair table ready
+  wave_lds_sync();  // pair table ready
   const PairConst* row = &tab[wv][ti * sd.T];
 
   // ---- candidates, kList at a time -------------------------------------------------
@@ -1303,8 +1311,8 @@ k_force_list(ForceArgs a) {
   if (active && 1e-4f < row[ti].thr) cnt -= 1;  // the particle itself
 
   if (EDGES) {
-    // barriers stay outside lane-dependent branches
-    __syncthreads();
+    // the wave's candidate lists are complete
+    wave_lds_sync();
     const uint32_t n = do_edges ? el_n[wv][o] : 0u;
     const bool fits = n <= (uint32_t)kListEdges;
     const int cap = a.edge_cap, maxc = sd.edge_maxc;
@@ -1315,7 +1323,7 @@ k_force_list(ForceArgs a) {
         for (uint32_t k2 = 0; k2 < n; ++k2) below += (el_i[wv][o][k2] < ik) ? 1u : 0u;
         el_d[wv][o][k].y = (below < (uint32_t)cap) ? 1.0f : 0.0f;
       }
-    __syncthreads();
+    wave_lds_sync();
     if (fits)
       for (uint32_t k = g; k < n; k += G) {
         const float2 dk = el_d[wv][o][k];
@@ -1454,6 +1462,8 @@ __global__ void __launch_bounds__(256) k_edges(EdgeArgs a) {
 // step counts into the other parity).
 
 constexpr int kSortThreads = 256;
+constexpr int kVisitBatch = 8;  // window slots per thread with their loads in flight together
+constexpr int kMoveBatch = 4;   // members per thread moved together
 
 // Sum over the block; every thread gets it.  `red` holds 4 words.
 __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* red) {
@@ -1565,6 +1575,10 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
     wq1[tid] = q1;
   }
 
+  // the far lists' lengths do not depend on anything below either
+  const uint32_t nfar_t = min(a.ftn[t * kCtrStride], (uint32_t)kFarCap);
+  const uint32_t nfar = min(*a.far_n, (uint32_t)a.P);
+
   // first slot of the tile, then its cells' starts (counts cleared for the next step)
   uint32_t part = 0;
   for (int u = sd.t0 + tid; u < t; u += kSortThreads) part += a.tcount[u * kCtrStride];
@@ -1603,22 +1617,43 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   if (!ok) return;
 
   // fn(q, key, id) for every slot of the window and of the far lists (the index is loaded
-  // with the key: most slots are not members, but a dependent load per member costs more)
+  // with the key: most slots are not members, but a dependent load per member costs more).
+  // The window's six runs are taken as one sequence, kVisitBatch slots per thread with all
+  // their loads in flight before any is used; fn's order does not matter (ranks are by
+  // index).
   auto visit = [&](auto&& fn) {
-    for (int sgm = 0; sgm < 6; ++sgm) {
-      const uint32_t q0 = wq0[sgm], q1 = wq1[sgm];
-      for (uint32_t q = q0 + tid; q < q1; q += kSortThreads) {
-        const uint32_t k = a.key[q], id = a.src_id[q];
-        if (!(k & kFarBit)) fn(q, k, id);
+    uint32_t wp[7];
+    wp[0] = 0u;
+#pragma unroll
+    for (int g = 0; g < 6; ++g) wp[g + 1] = wp[g] + (wq1[g] - wq0[g]);
+    const uint32_t nw = wp[6];
+    for (uint32_t b = 0; b < nw; b += kVisitBatch * kSortThreads) {
+      uint32_t qv[kVisitBatch], kv[kVisitBatch], iv[kVisitBatch];
+#pragma unroll
+      for (int u = 0; u < kVisitBatch; ++u) {
+        const uint32_t i = b + (uint32_t)(u * kSortThreads + tid);
+        qv[u] = 0u;
+        kv[u] = kFarBit;  // nothing to visit
+        iv[u] = 0u;
+        if (i < nw) {
+          uint32_t q = wq0[0] + i;
+#pragma unroll
+          for (int g = 1; g < 6; ++g)
+            if (i >= wp[g]) q = wq0[g] + (i - wp[g]);  // an empty run gives way to the next
+          qv[u] = q;
+          kv[u] = a.key[q];
+          iv[u] = a.src_id[q];
+        }
       }
+#pragma unroll
+      for (int u = 0; u < kVisitBatch; ++u)
+        if (!(kv[u] & kFarBit)) fn(qv[u], kv[u], iv[u]);
     }
-    const uint32_t nt = min(a.ftn[t * kCtrStride], (uint32_t)kFarCap);
-    for (uint32_t j = tid; j < nt; j += kSortThreads) {
+    for (uint32_t j = tid; j < nfar_t; j += kSortThreads) {
       const uint32_t q = a.ftl[(long)t * kFarCap + j];
       fn(q, a.key[q] & ~kFarBit, a.src_id[q]);
     }
-    const uint32_t nf = min(*a.far_n, (uint32_t)a.P);
-    for (uint32_t j = tid; j < nf; j += kSortThreads) {
+    for (uint32_t j = tid; j < nfar; j += kSortThreads) {
       const uint32_t q = a.far_slot[j];
       fn(q, a.key[q] & ~kFarBit, a.src_id[q]);
     }
@@ -1658,21 +1693,41 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
         mlc[m] = (uint16_t)lc;
       });
       __syncthreads();
-      for (uint32_t m = tid; m < mn; m += kSortThreads) {
-        const uint32_t lc = mlc[m];
-        if (lc >= (uint32_t)nc || fill_s[lc] != cnt_s[lc]) {
-          atomicOr(a.flag, 4u);  // a cell lost a particle: leave its slots alone
-          continue;
+      // kMoveBatch members per thread: their state loads are in flight during the ranking
+      for (uint32_t mb = 0; mb < mn; mb += kMoveBatch * kSortThreads) {
+        float4 sp[kMoveBatch], sv[kMoveBatch];
+        uint32_t lcv[kMoveBatch];
+#pragma unroll
+        for (int u = 0; u < kMoveBatch; ++u) {
+          const uint32_t m = mb + (uint32_t)(u * kSortThreads + tid);
+          lcv[u] = 0xFFFFu;
+          sp[u] = sv[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if (m < mn) {
+            const uint32_t lc = mlc[m];
+            if (lc >= (uint32_t)nc || fill_s[lc] != cnt_s[lc]) {
+              atomicOr(a.flag, 4u);  // a cell lost a particle: leave its slots alone
+            } else {
+              const uint32_t q = mslot[m];
+              sp[u] = a.src_pos[q];
+              sv[u] = a.src_vel[q];
+              lcv[u] = lc;
+            }
+          }
         }
-        const uint32_t q = mslot[m];
-        const float4 sp = a.src_pos[q], sv = a.src_vel[q];  // in flight during the ranking
-        const uint32_t s0 = cofs[lc] - m0, sn = cnt_s[lc], me = mid[m];
-        uint32_t rank = 0;
-        for (uint32_t e = 0; e < sn; ++e) rank += (mid[s0 + e] < me) ? 1u : 0u;
-        const uint32_t dst = base + cofs[lc] + rank;
-        a.dst_pos[dst] = sp;
-        a.dst_vel[dst] = sv;
-        a.dst_id[dst] = me;
+#pragma unroll
+        for (int u = 0; u < kMoveBatch; ++u) {
+          const uint32_t lc = lcv[u];
+          if (lc != 0xFFFFu) {
+            const uint32_t m = mb + (uint32_t)(u * kSortThreads + tid);
+            const uint32_t s0 = cofs[lc] - m0, sn = cnt_s[lc], me = mid[m];
+            uint32_t rank = 0;
+            for (uint32_t e = 0; e < sn; ++e) rank += (mid[s0 + e] < me) ? 1u : 0u;
+            const uint32_t dst = base + cofs[lc] + rank;
+            a.dst_pos[dst] = sp[u];
+            a.dst_vel[dst] = sv[u];
+            a.dst_id[dst] = me;
+          }
+        }
       }
     } else {
       // one cell (ib == ia + 1) holds more than the LDS takes: its ids and slots go to
