@@ -1258,6 +1258,8 @@ k_force_list(ForceArgs a) {
       const uint32_t q = lslot[wv][o][k];
       const uint32_t code = lcode[wv][o][k];
       const float4 pj = a.pos[q];
+      // the index travels with the position: an edge candidate then waits for no load
+      const uint32_t jq = (EDGES && do_edges) ? a.id[q] : 0u;
       float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
       bool raw;
       if (!c.gen) {
@@ -1291,7 +1293,7 @@ k_force_list(ForceArgs a) {
       if (EDGES) {
         // edges are non-periodic (ParticleSimulation.cu:222-227): unshifted pairs only
         if (do_edges && raw && tj == ti && s2 < sd.edge_d2) {
-          const uint32_t jid = a.id[q];
+          const uint32_t jid = jq;
           if (jid > idi) {
             const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
             if (kk < (uint32_t)kListEdges) {
