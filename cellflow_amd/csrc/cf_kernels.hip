@@ -1496,6 +1496,14 @@ __device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* red, ui
 __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
   const int tid = threadIdx.x;
   if (tid < kEdgeSub) a.etcount_next[(u * kEdgeSub + tid) * kCtrStride] = 0u;
+  // the owners' edge counts and their simulation's first slot (a thread's kPer owners share
+  // one 64-slot group) depend on nothing below: loaded first
+  constexpr int kPer = kEdgeTile / kSortThreads;
+  const long id0 = (long)u * kEdgeTile + (long)tid * kPer;
+  uint32_t ec[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) ec[i] = (id0 + i < a.P) ? a.ecount[id0 + i] : 0u;
+  const int p0 = (id0 < a.P) ? a.sims[a.wave_sim[id0 / kWave]].p0 : 0;
   uint32_t part = 0, mine = 0;
   for (int v = tid; v < (u + 1) * kEdgeSub; v += kSortThreads) {
     const uint32_t x = a.etcount[v * kCtrStride];
@@ -1504,15 +1512,9 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
   }
   const uint32_t own = block_sum_u32(mine, red);
   const uint32_t base = block_sum_u32(part, red);
-  constexpr int kPer = kEdgeTile / kSortThreads;
-  const long id0 = (long)u * kEdgeTile + (long)tid * kPer;
-  uint32_t ec[kPer];
   uint32_t run = 0;
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    ec[i] = (id0 + i < a.P) ? a.ecount[id0 + i] : 0u;
-    run += ec[i];
-  }
+  for (int i = 0; i < kPer; ++i) run += ec[i];
   uint32_t total;
   uint32_t off = base + block_excl_u32(run, red, total);
   if (total != own || (long)base + total > a.edge_room) {
@@ -1523,7 +1525,6 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
   for (int i = 0; i < kPer; ++i) {
     if (ec[i]) {
       const long id = id0 + i;
-      const int p0 = a.sims[a.wave_sim[id / kWave]].p0;
       for (uint32_t k = 0; k < ec[i]; ++k)
         a.edges[off + k] = make_int2((int)(id - p0), (int)(a.estage[(long)k * a.P + id] - p0));
     }
@@ -1664,17 +1665,20 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   // cells [ia, ib) at a time, as many as kTileCap members allow (all of them, normally)
   int ia = 0;
   while (ia < nc) {
-    if (tid == 0) {
-      int lo = ia + 1, hi = nc;
-      while (lo < hi) {
-        const int m = (lo + hi + 1) >> 1;
-        if (cofs[m] - cofs[ia] <= (uint32_t)kTileCap) lo = m;
-        else hi = m - 1;
+    int ib = nc;  // every member fits (the normal case): no search
+    if (M - cofs[ia] > (uint32_t)kTileCap) {
+      if (tid == 0) {
+        int lo = ia + 1, hi = nc;
+        while (lo < hi) {
+          const int m = (lo + hi + 1) >> 1;
+          if (cofs[m] - cofs[ia] <= (uint32_t)kTileCap) lo = m;
+          else hi = m - 1;
+        }
+        s_next = lo;
       }
-      s_next = lo;
+      __syncthreads();
+      ib = s_next;
     }
-    __syncthreads();
-    const int ib = s_next;
     const uint32_t m0 = cofs[ia], mn = cofs[ib] - m0;
     if (mn <= (uint32_t)kTileCap) {
       for (int i = ia + tid; i < ib; i += kSortThreads) fill_s[i] = 0u;
