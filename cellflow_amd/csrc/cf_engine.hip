@@ -690,6 +690,7 @@ struct Engine {
     }
     ta.wave_sim = b.wave_sim;
     ta.P = P;
+    ta.one_sim = sims.size() == 1 ? 1 : 0;
     return ta;
   }
 

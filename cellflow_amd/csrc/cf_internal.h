@@ -238,6 +238,7 @@ struct TileSortArgs {
   const uint32_t* __restrict__ wave_sim;
   int netiles;
   long P;
+  int one_sim;                          // one simulation: tile_sim is all zero
 };
 
 struct BinArgs {

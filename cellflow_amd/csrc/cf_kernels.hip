@@ -1467,30 +1467,37 @@ constexpr int kSortThreads = 256;
 constexpr int kVisitBatch = 8;  // window slots per thread with their loads in flight together
 constexpr int kMoveBatch = 4;   // members per thread moved together
 
-// Sum over the block; every thread gets it.  `red` holds 4 words.
-__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* red) {
+// Sums of `v` and `u` over the block, and the exclusive scan of `w` in thread order with
+// its total, behind one pair of barriers.  `red` holds 12 words.
+__device__ __forceinline__ void block_sums_excl_u32(uint32_t v, uint32_t u, uint32_t w,
+                                                    uint32_t* red, uint32_t& vsum,
+                                                    uint32_t& usum, uint32_t& wpre,
+                                                    uint32_t& wtotal) {
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+  constexpr int kW = kSortThreads / kWave;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, kWave);
+  for (int o = 32; o > 0; o >>= 1) {
+    v += (uint32_t)__shfl_xor((int)v, o, kWave);
+    u += (uint32_t)__shfl_xor((int)u, o, kWave);
+  }
+  const uint32_t incl = wave_inclusive_scan(w, lane);
   __syncthreads();
-  if (lane == 0) red[wv] = v;
+  if (lane == kWave - 1) {
+    red[wv] = v;
+    red[kW + wv] = u;
+    red[2 * kW + wv] = incl;
+  }
   __syncthreads();
-  return red[0] + red[1] + red[2] + red[3];
-}
-
-// Exclusive scan over the block in thread order; `total` gets the sum.
-__device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* red, uint32_t& total) {
-  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
-  const uint32_t incl = wave_inclusive_scan(v, lane);
-  __syncthreads();
-  if (lane == kWave - 1) red[wv] = incl;
-  __syncthreads();
-  uint32_t pre = 0;
+  vsum = usum = wtotal = 0u;
+  uint32_t pre = 0u;
 #pragma unroll
-  for (int w = 0; w < kSortThreads / kWave; ++w)
-    if (w < wv) pre += red[w];
-  total = red[0] + red[1] + red[2] + red[3];
-  return pre + incl - v;
+  for (int k = 0; k < kW; ++k) {
+    vsum += red[k];
+    usum += red[kW + k];
+    wtotal += red[2 * kW + k];
+    if (k < wv) pre += red[2 * kW + k];
+  }
+  wpre = pre + incl - w;
 }
 
 __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
@@ -1510,13 +1517,12 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
     if (v < u * kEdgeSub) part += x;
     else mine += x;
   }
-  const uint32_t own = block_sum_u32(mine, red);
-  const uint32_t base = block_sum_u32(part, red);
   uint32_t run = 0;
 #pragma unroll
   for (int i = 0; i < kPer; ++i) run += ec[i];
-  uint32_t total;
-  uint32_t off = base + block_excl_u32(run, red, total);
+  uint32_t own, base, pre, total;
+  block_sums_excl_u32(mine, part, run, red, own, base, pre, total);
+  uint32_t off = base + pre;
   if (total != own || (long)base + total > a.edge_room) {
     if (tid == 0) atomicOr(a.flag, 4u);
     return;
@@ -1539,7 +1545,7 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   __shared__ uint32_t mslot[kTileCap];
   __shared__ uint32_t mid[kTileCap];
   __shared__ uint16_t mlc[kTileCap];
-  __shared__ uint32_t red[kSortThreads / kWave];
+  __shared__ uint32_t red[3 * (kSortThreads / kWave)];
   __shared__ uint32_t wq0[6], wq1[6];  // the window's slot runs
   __shared__ int s_next;
   const int tid = threadIdx.x;
@@ -1549,7 +1555,7 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   }
   // consecutive tiles (consecutive slots) share an XCD, as in the force pass
   const int t = xcd_map((int)blockIdx.x, a.ntiles);
-  const SimDesc& sd = a.sims[a.tile_sim[t]];
+  const SimDesc& sd = a.sims[a.one_sim ? 0 : (int)a.tile_sim[t]];
   if (tid == 0) {
     a.tcount_next[t * kCtrStride] = 0u;
     a.ftn_next[t * kCtrStride] = 0u;
@@ -1585,7 +1591,6 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   // first slot of the tile, then its cells' starts (counts cleared for the next step)
   uint32_t part = 0;
   for (int u = sd.t0 + tid; u < t; u += kSortThreads) part += a.tcount[u * kCtrStride];
-  const uint32_t base = (uint32_t)sd.p0 + block_sum_u32(part, red);
   const int per = (nc + kSortThreads - 1) / kSortThreads;
   const int i0 = min(tid * per, nc), i1 = min(i0 + per, nc);
   uint32_t run = 0;
@@ -1594,8 +1599,9 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
     cnt_s[i] = v;
     run += v;
   }
-  uint32_t M;
-  uint32_t pre = block_excl_u32(run, red, M);
+  uint32_t before, unused, pre, M;
+  block_sums_excl_u32(part, 0u, run, red, before, unused, pre, M);
+  const uint32_t base = (uint32_t)sd.p0 + before;
   // The counts must agree (and the simulation's last tile must end at its last particle);
   // if they do not, the tile writes nothing but clamped starts, and the call fails.
   const uint32_t lim = (uint32_t)(sd.p0 + sd.n);
