@@ -24,7 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
-STEP_BYTES = 60.0         # compulsory state traffic per particle-step
+# compulsory state traffic per particle-step: read pos 12 + vel 12 + type 4 + previous
+# count 4, write pos 12 + vel 12 + count 4
+STEP_BYTES = 60.0
 EDGE_BYTES = 20.0         # + per particle for the edge pass
 EDGE_PAIR_BYTES = 8.0     # + per emitted edge (an int32 pair)
 METRIC = "particle-steps/s @100k; batched-ensemble & adjoint speedup vs plain step"
@@ -235,8 +237,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this engine "
-              "runs one simulation per GPU; see DESIGN.md for how it would shard)",
-              file=sys.stderr)
+              "runs each simulation or ensemble on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
