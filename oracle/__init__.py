@@ -2,7 +2,7 @@
 
 ctypes bindings to ``oracle/_build/liboracle.so`` (built from ``oracle/oracle.cpp`` by
 ``oracle/Makefile``).  Only the tests, the smoke check and the benchmark's CPU-baseline
-leg import this module, and only as the checker.  The engine never calls it.
+leg import this module, and only to compare the engine against.  The engine never calls it.
 
 Each wrapper names the reference lines its C function restates (paths relative to the
 reference root):

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle.cpp) before the engine is compared against it.
 
 The reference ships no tests or golden vectors, so each oracle function is checked against
 what the mathematics and the reference's own semantics fix, chosen so that a plausible
