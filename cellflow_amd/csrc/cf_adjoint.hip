@@ -84,12 +84,10 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ 
 
 // ---- one backward step -----------------------------------------------------------------
 
-// phi_i = dt m_i (lv'_i + dt lx'_i),  lv_i = fr (lv'_i + dt lx'_i); phi is stored by the
-// pre-step slot, where the pair kernel finds both ends of a pair without an index lookup
-__global__ void k_adj_phi(AdjArgs a) {
-  const SimDesc sd = a.sims[0];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sd.n) return;
+// phi_i = dt m_i (lv'_i + dt lx'_i),  lv_i = fr (lv'_i + dt lx'_i) of particle k; phi is
+// stored by the pre-step slot, where the pair kernel finds both ends of a pair without an
+// index lookup.  The walk kernel computes it on the side (it needs nothing the walk makes).
+__device__ __forceinline__ void adj_phi(const AdjArgs& a, const SimDesc& sd, int k) {
   const int p = sd.p0 + k;
   const uint32_t id = a.id[p];
   const int prev = __float_as_int(a.vel[p].w);
@@ -320,6 +318,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_walk(AdjArgs a) {
   const bool active = k < sd.n;
   const AdjOwn c = adj_own(a, sd, k, active);
   if (g == 0) nhit[o] = 0u;
+  if (active && g == G - 1) adj_phi(a, sd, k);  // a lane that leads no list work below
   __syncthreads();
   if (active)
     for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
@@ -484,8 +483,6 @@ hipError_t launch_hist_grad(const HistArgs& a, int n, hipStream_t st) {
 }
 
 hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st) {
-  const int g = (n + 127) / 128;
-  hipLaunchKernelGGL(k_adj_phi, dim3(g), dim3(128), 0, st, a);
   hipLaunchKernelGGL(k_adj_walk, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
   hipLaunchKernelGGL(k_adj_pair_g, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
   return hipGetLastError();
