@@ -1032,36 +1032,52 @@ struct Engine {
     const int K = ns + 3;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
-    hipEvent_t ev[8];
-    for (auto& e : ev) HIPR(hipEventCreate(&e));
-    for (int k = 0; k < nsteps; ++k) {
-      HIPR(hipEventRecord(ev[0], st));
-      CFR(step(er.on && fuse && k > 0, cap, ev + 1));
-      int ne = ns;
-      if (er.on && !fuse) {
-        CFR(edge_pass(cap, ev + ns + 1));
-        ne = ns + 3;
+    // Every step is queued before any is waited for, each with its own events, so the GPU
+    // runs the kernels back to back: an interval holds one kernel and the event marker
+    // after it, never a host launch.  The first step's intervals are left out when there
+    // are others (its first event may reach the GPU before the kernel behind it is queued).
+    const int per = K + 1;
+    std::vector<hipEvent_t> ev((size_t)(nsteps + 1) * per, nullptr);
+    int rc = CF_OK;
+    for (auto& e : ev)
+      if (rc == CF_OK && hipEventCreate(&e) != hipSuccess)
+        rc = fail(CF_ERR_DEVICE, "hipEventCreate failed");
+    std::vector<int> nev(nsteps, 0);
+    for (int k = 0; k < nsteps && rc == CF_OK; ++k) {
+      hipEvent_t* e = &ev[(size_t)k * per];
+      if (hipEventRecord(e[0], st) != hipSuccess) {
+        rc = fail(CF_ERR_DEVICE, "hipEventRecord failed");
+        break;
       }
-      HIPR(hipEventSynchronize(ev[ne]));
-      for (int i = 0; i < ne; ++i) {
-        float ms = 0;
-        HIPR(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-        acc[i] += ms;
-        cnt[i]++;
-      }
-    }
-    if (er.on && fuse) {
-      HIPR(hipEventRecord(ev[ns], st));
-      CFR(edge_pass(cap, ev + ns + 1));
-      HIPR(hipEventSynchronize(ev[ns + 3]));
-      for (int i = ns; i < ns + 3; ++i) {
-        float ms = 0;
-        HIPR(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-        acc[i] += ms;
-        cnt[i]++;
+      rc = step(er.on && fuse && k > 0, cap, e + 1);
+      nev[k] = ns;
+      if (rc == CF_OK && er.on && !fuse) {
+        rc = edge_pass(cap, e + ns + 1);
+        nev[k] = ns + 3;
       }
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    hipEvent_t* fe = &ev[(size_t)nsteps * per];  // the final standalone edge pass (fused case)
+    if (rc == CF_OK && er.on && fuse) {
+      if (hipEventRecord(fe[ns], st) != hipSuccess) rc = fail(CF_ERR_DEVICE, "hipEventRecord failed");
+      if (rc == CF_OK) rc = edge_pass(cap, fe + ns + 1);
+    }
+    if (rc == CF_OK && hipStreamSynchronize(st) != hipSuccess)
+      rc = fail(CF_ERR_DEVICE, "the profiled steps failed");
+    auto add = [&](const hipEvent_t* e, int i) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, e[i], e[i + 1]) != hipSuccess)
+        return fail(CF_ERR_DEVICE, "hipEventElapsedTime failed");
+      acc[i] += ms;
+      cnt[i]++;
+      return CF_OK;
+    };
+    for (int k = (nsteps > 1 ? 1 : 0); k < nsteps && rc == CF_OK; ++k)
+      for (int i = 0; i < nev[k] && rc == CF_OK; ++i) rc = add(&ev[(size_t)k * per], i);
+    if (rc == CF_OK && er.on && fuse)
+      for (int i = ns; i < ns + 3 && rc == CF_OK; ++i) rc = add(fe, i);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (rc != CF_OK) return rc;
     int w = 0;
     std::string nm;
     for (int i = 0; i < K && w < max_k; ++i) {

@@ -379,3 +379,21 @@ def test_both_sorts_bit_identical(variant, lanes):
                     sim.getNeighborCounts().tobytes()))
         sim.close()
     assert out[0] == out[1]
+
+
+def test_profile_steps_times_each_kernel_and_steps_the_state():
+    """profileSteps times the step's kernels with HIP events and advances the state exactly
+    as simulate does (same kernels, launched one by one instead of from a graph)."""
+    st, phys, params = state_of(small_cfg(n=4000, side=0.5))
+    a, b = make_sim(st, phys), make_sim(st, phys)
+    t = a.profileSteps(params, 6, edges=(0.02, 8))
+    assert list(t) == ["force", "sort", "edges", "edge_scan", "edge_write"]
+    assert all(v > 0.0 for v in t.values())
+    b.simulate(params, 6, edges=(0.02, 8))
+    assert a.getParticleData().tobytes() == b.getParticleData().tobytes()
+    assert np.array_equal(a.getEdges(), b.getEdges())
+    c = make_sim(st, phys)
+    c.setTuning(sort_variant=0)
+    t0 = c.profileSteps(params, 3)
+    assert list(t0) == ["force", "scan", "fill", "rank_scatter"]
+    assert all(v > 0.0 for v in t0.values())
