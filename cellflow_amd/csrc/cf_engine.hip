@@ -204,6 +204,7 @@ struct Engine {
   void release() {
     if (st) (void)hipStreamSynchronize(st);
     drop_graphs();
+    free_adj();
     for (int i = 0; i < 2; ++i) {
       dfree(b.pos[i]);
       dfree(b.vel[i]);
@@ -1118,6 +1119,75 @@ struct Engine {
     dfree(s.start);
   }
 
+  // The gradient rollout's buffers, kept from one call to the next of the same shape
+  // (slots, cells, particles, segments, checkpoint interval, bins, types), so an
+  // optimisation loop that calls it repeatedly allocates once.  Freed with the engine, when
+  // the shape changes and after a failed call.
+  struct AdjBufs {
+    std::vector<StateBuf> ck, seg;
+    std::vector<float4*> tv;
+    StateBuf fin;
+    double4* lx[2] = {nullptr, nullptr};
+    double4* lv[2] = {nullptr, nullptr};
+    double4* phi = nullptr;
+    uint32_t *hits = nullptr, *nhits = nullptr;
+    double *partial = nullptr, *hd = nullptr, *gh = nullptr, *gacc = nullptr;
+    float* thr = nullptr;
+    std::vector<long> shape;  // empty: nothing allocated
+  } adj;
+
+  void free_adj() {
+    for (auto& s : adj.ck) free_state(s);
+    for (auto& s : adj.seg) free_state(s);
+    for (auto& t : adj.tv) dfree(t);
+    free_state(adj.fin);
+    for (int i = 0; i < 2; ++i) {
+      dfree(adj.lx[i]);
+      dfree(adj.lv[i]);
+    }
+    dfree(adj.phi);
+    dfree(adj.hits);
+    dfree(adj.nhits);
+    dfree(adj.partial);
+    dfree(adj.hd);
+    dfree(adj.gh);
+    dfree(adj.gacc);
+    dfree(adj.thr);
+    adj.ck.clear();
+    adj.seg.clear();
+    adj.tv.clear();
+    adj.shape.clear();
+  }
+
+  // shape = {P, C, n, segments, checkpoint interval, bins, T, partial rows, partial stride}
+  int alloc_adj(const std::vector<long>& shape) {
+    if (adj.shape == shape) return CF_OK;
+    free_adj();
+    const long n = shape[2], nseg = shape[3], ckpt = shape[4], bins = shape[5], T = shape[6],
+               rows = shape[7], stride = shape[8];
+    adj.ck.resize((size_t)std::max(nseg, 1L));
+    adj.seg.resize((size_t)ckpt + 1);
+    adj.tv.assign((size_t)ckpt, nullptr);
+    for (auto& s : adj.ck) CFR(alloc_state(s));
+    for (size_t i = 1; i < adj.seg.size(); ++i) CFR(alloc_state(adj.seg[i]));
+    for (auto& t : adj.tv) CFR(dalloc(&t, P));
+    CFR(alloc_state(adj.fin));
+    for (int i = 0; i < 2; ++i) {
+      CFR(dalloc(&adj.lx[i], P));
+      CFR(dalloc(&adj.lv[i], P));
+    }
+    CFR(dalloc(&adj.phi, P));
+    CFR(dalloc(&adj.hits, (size_t)std::max(n, 1L) * 64));
+    CFR(dalloc(&adj.nhits, (size_t)std::max(n, 1L)));
+    CFR(dalloc(&adj.partial, (size_t)(rows * stride)));
+    CFR(dalloc(&adj.hd, (size_t)bins));
+    CFR(dalloc(&adj.gh, (size_t)bins));
+    CFR(dalloc(&adj.gacc, (size_t)(T * T + 1)));
+    CFR(dalloc(&adj.thr, (size_t)(T * T)));
+    adj.shape = shape;
+    return CF_OK;
+  }
+
   int save_state(StateBuf& s) {
     HIPR(hipMemcpyAsync(s.pos, b.pos[cur], P * sizeof(float4), hipMemcpyDeviceToDevice, st));
     HIPR(hipMemcpyAsync(s.vel, b.vel[cur], P * sizeof(float4), hipMemcpyDeviceToDevice, st));
@@ -1191,51 +1261,22 @@ struct Engine {
     const int n = h.n, T = h.T;
     const long stride = ((long)n + 127) / 128 * 128;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
-    std::vector<StateBuf> ck(std::max(nseg, 1)), seg(ckpt + 1);
-    std::vector<float4*> tv(ckpt, nullptr);  // each segment step's new velocities + counts
-    StateBuf fin;
-    double4 *lx[2] = {nullptr, nullptr}, *lv[2] = {nullptr, nullptr}, *phi = nullptr;
-    uint32_t *hits = nullptr, *nhits = nullptr;
-    double *partial = nullptr, *hd = nullptr, *gh = nullptr, *gacc = nullptr;
-    float* thr = nullptr;
+    const int rows = std::max(bins, T * T + 1);
     int rc = CF_OK;
     const DevBufs own = b;  // the engine's state buffers (the recompute re-points them)
-    auto cleanup = [&]() {
-      for (auto& s : ck) free_state(s);
-      for (auto& s : seg) free_state(s);
-      for (auto& t : tv) dfree(t);
-      free_state(fin);
-      for (int i = 0; i < 2; ++i) {
-        dfree(lx[i]);
-        dfree(lv[i]);
-      }
-      dfree(phi);
-      dfree(hits);
-      dfree(nhits);
-      dfree(partial);
-      dfree(hd);
-      dfree(gh);
-      dfree(gacc);
-      dfree(thr);
-    };
     auto body = [&]() -> int {
-      for (auto& s : ck) CFR(alloc_state(s));
-      for (size_t i = 1; i < seg.size(); ++i) CFR(alloc_state(seg[i]));  // seg[0] is ck[sg]
-      for (auto& t : tv) CFR(dalloc(&t, P));
-      CFR(alloc_state(fin));
-      for (int i = 0; i < 2; ++i) {
-        CFR(dalloc(&lx[i], P));
-        CFR(dalloc(&lv[i], P));
-      }
-      CFR(dalloc(&phi, P));
-      CFR(dalloc(&hits, (size_t)std::max(n, 1) * 64));
-      CFR(dalloc(&nhits, std::max(n, 1)));
-      const int rows = std::max(bins, T * T + 1);
-      CFR(dalloc(&partial, (size_t)rows * stride));
-      CFR(dalloc(&hd, bins));
-      CFR(dalloc(&gh, bins));
-      CFR(dalloc(&gacc, T * T + 1));
-      CFR(dalloc(&thr, T * T));
+      CFR(alloc_adj({P, C, (long)n, (long)nseg, (long)ckpt, (long)bins, (long)T, (long)rows,
+                     stride}));
+      std::vector<StateBuf>& ck = adj.ck;
+      std::vector<StateBuf>& seg = adj.seg;  // seg[0] is never allocated: it is ck[sg]
+      std::vector<float4*>& tv = adj.tv;     // each segment step's new velocities + counts
+      StateBuf& fin = adj.fin;
+      double4** lx = adj.lx;
+      double4** lv = adj.lv;
+      double4* phi = adj.phi;
+      uint32_t *hits = adj.hits, *nhits = adj.nhits;
+      double *partial = adj.partial, *hd = adj.hd, *gh = adj.gh, *gacc = adj.gacc;
+      float* thr = adj.thr;
       std::vector<float> hthr(T * T);
       for (int i = 0; i < T * T; ++i) hthr[i] = hpairs[hdesc[0].tab + i].thr;
       HIPR(hipMemcpyAsync(thr, hthr.data(), T * T * sizeof(float), hipMemcpyHostToDevice, st));
@@ -1340,7 +1381,7 @@ struct Engine {
     rc = body();
     (void)hipStreamSynchronize(st);
     b = own;  // the engine's own state buffers, whatever the recompute pointed them at
-    cleanup();
+    if (rc != CF_OK) free_adj();
     return rc;
   }
 

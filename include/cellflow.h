@@ -207,7 +207,9 @@ int cf_pair_histogram(cf_sim* sim, int bins, float r_max, double* hist);
  * and returns the loss L = sum_b (h_b - target_b)^2 of the final histogram and its gradient
  * with respect to the num_types^2 force-table entries (row-major) and ratio_with_lfo, by
  * a hand-written adjoint with state checkpoints every `checkpoint_every` steps.  The
- * rollout's discrete decisions (which pairs interact, neighbour counts) are held fixed. */
+ * rollout's discrete decisions (which pairs interact, neighbour counts) are held fixed.
+ * The call's device buffers (checkpoints, one segment's states, adjoint state) stay
+ * allocated for the next call of the same shape and are freed with the handle. */
 int cf_rollout_grad(cf_sim* sim, const cf_params* params, int nsteps, int checkpoint_every,
                     int bins, float r_max, const double* target_hist, double* loss,
                     double* hist, double* grad_table, double* grad_ratio);
