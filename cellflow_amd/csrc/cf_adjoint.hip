@@ -266,7 +266,7 @@ __device__ __forceinline__ uint32_t group_excl_scan(uint32_t v, int g) {
   return x - v;
 }
 
-constexpr int kAdjHits = 64;
+constexpr int kAdjHits = kPairList;  // also the stride of the force kernel's lists
 
 // The pair kernel's particle context and its FP32 in-range decision, shared by the walk
 // kernel and the pair kernel.
@@ -482,7 +482,19 @@ hipError_t launch_hist_grad(const HistArgs& a, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st) {
+// The density-factor terms alone, for steps whose lists the recompute's force kernel wrote.
+__global__ void __launch_bounds__(128) k_adj_phi(AdjArgs a) {
+  const SimDesc sd = a.sims[0];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < sd.n) adj_phi(a, sd, k);
+}
+
+hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st, bool walk) {
+  if (!walk) {
+    hipLaunchKernelGGL(k_adj_phi, dim3((n + 127) / 128), dim3(128), 0, st, a);
+    hipLaunchKernelGGL(k_adj_pair_g, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_adj_walk, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
   hipLaunchKernelGGL(k_adj_pair_g, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
   return hipGetLastError();

@@ -181,6 +181,10 @@ struct Engine {
 
   uint32_t* tile_sim = nullptr;
   int ntiles = 0, ntiles_cap = 0, netiles = 0;
+  // set only while the gradient rollout steps: where the force kernel lists the step's
+  // in-range partners (ForceArgs::hits)
+  uint32_t* fhits = nullptr;
+  uint32_t* fnhits = nullptr;
   std::vector<uint32_t> htile_sim;
   cf_tuning tun{1, 0, 16, 256, 32, 1};
   uint32_t* hsmall = nullptr;  // pinned readback words
@@ -624,6 +628,8 @@ struct Engine {
     fa.far_slot = far_slot;
     fa.ftn = ftn[A];
     fa.ftl = ftl;
+    fa.hits = fhits;
+    fa.nhits = fnhits;
     return fa;
   }
 
@@ -1130,7 +1136,9 @@ struct Engine {
     double4* lx[2] = {nullptr, nullptr};
     double4* lv[2] = {nullptr, nullptr};
     double4* phi = nullptr;
-    uint32_t *hits = nullptr, *nhits = nullptr;
+    // in-range lists of each segment step (the force kernel writes them during the
+    // recompute with variant 0; with other variants the walk fills the first)
+    std::vector<uint32_t*> hits, nhits;
     double *partial = nullptr, *hd = nullptr, *gh = nullptr, *gacc = nullptr;
     float* thr = nullptr;
     std::vector<long> shape;  // empty: nothing allocated
@@ -1146,8 +1154,10 @@ struct Engine {
       dfree(adj.lv[i]);
     }
     dfree(adj.phi);
-    dfree(adj.hits);
-    dfree(adj.nhits);
+    for (auto& x : adj.hits) dfree(x);
+    for (auto& x : adj.nhits) dfree(x);
+    adj.hits.clear();
+    adj.nhits.clear();
     dfree(adj.partial);
     dfree(adj.hd);
     dfree(adj.gh);
@@ -1177,8 +1187,12 @@ struct Engine {
       CFR(dalloc(&adj.lv[i], P));
     }
     CFR(dalloc(&adj.phi, P));
-    CFR(dalloc(&adj.hits, (size_t)std::max(n, 1L) * 64));
-    CFR(dalloc(&adj.nhits, (size_t)std::max(n, 1L)));
+    adj.hits.assign((size_t)ckpt, nullptr);
+    adj.nhits.assign((size_t)ckpt, nullptr);
+    for (long i = 0; i < ckpt; ++i) {
+      CFR(dalloc(&adj.hits[i], (size_t)std::max(n, 1L) * kPairList));
+      CFR(dalloc(&adj.nhits[i], (size_t)std::max(n, 1L)));
+    }
     CFR(dalloc(&adj.partial, (size_t)(rows * stride)));
     CFR(dalloc(&adj.hd, (size_t)bins));
     CFR(dalloc(&adj.gh, (size_t)bins));
@@ -1274,7 +1288,13 @@ struct Engine {
       double4** lx = adj.lx;
       double4** lv = adj.lv;
       double4* phi = adj.phi;
-      uint32_t *hits = adj.hits, *nhits = adj.nhits;
+      // variant 0 lists each step's in-range partners during the recompute; the other
+      // variants leave that to the adjoint's walk kernel
+      const bool lists = tun.force_variant == 0;
+      auto listing = [&](int i) {
+        fhits = lists ? adj.hits[i] : nullptr;
+        fnhits = lists ? adj.nhits[i] : nullptr;
+      };
       double *partial = adj.partial, *hd = adj.hd, *gh = adj.gh, *gacc = adj.gacc;
       float* thr = adj.thr;
       std::vector<float> hthr(T * T);
@@ -1298,9 +1318,11 @@ struct Engine {
         if (s >= last0) {
           point(1 - cur, seg[s - last0 + 1]);
           b.tvel = tv[s - last0];
+          listing(s - last0);
         }
         CFR(step(false, 0));
       }
+      fhits = fnhits = nullptr;
       CFR(save_state(fin));
       b = own;
       // loss of the final state
@@ -1336,8 +1358,6 @@ struct Engine {
       for (int t = 0; t < T; ++t) aa.c.rbt[t] = h.rbt[t];
       for (int t = 0; t < T * T; ++t) aa.c.table[t] = h.table[t];
       aa.phi = phi;
-      aa.hits = hits;
-      aa.nhits = nhits;
       aa.partial = partial;
       aa.stride = nblk;
       int L = 0;  // lx[L] holds dL/dx of the state after the step being reversed
@@ -1348,8 +1368,10 @@ struct Engine {
           for (int s = k0; s < k1; ++s) {
             point(1 - cur, seg[s - k0 + 1]);
             b.tvel = tv[s - k0];
+            listing(s - k0);
             CFR(step(false, 0));
           }
+          fhits = fnhits = nullptr;
           b = own;
         }
         for (int s = k1 - 1; s >= k0; --s) {
@@ -1363,7 +1385,9 @@ struct Engine {
           aa.lam_v_next = lv[L];
           aa.lam_x = lx[1 - L];
           aa.lam_v = lv[1 - L];
-          HIPR(launch_adj_step(aa, n, st));
+          aa.hits = adj.hits[lists ? s - k0 : 0];
+          aa.nhits = adj.nhits[lists ? s - k0 : 0];
+          HIPR(launch_adj_step(aa, n, st, !lists));
           L = 1 - L;
         }
       }
@@ -1381,6 +1405,7 @@ struct Engine {
     rc = body();
     (void)hipStreamSynchronize(st);
     b = own;  // the engine's own state buffers, whatever the recompute pointed them at
+    fhits = fnhits = nullptr;
     if (rc != CF_OK) free_adj();
     return rc;
   }

@@ -33,6 +33,8 @@ constexpr int kEdgeSub = 8;
 // Far movers are listed by destination tile (kFarCap per tile); a tile's excess goes to one
 // shared overflow list that every tile reads.
 constexpr int kFarCap = 256;
+// In-range partners listed per particle for the adjoint's pair kernel (more: it walks again).
+constexpr int kPairList = 64;
 
 // Per-type-pair constants of one simulation, T*T entries, row = own type.
 struct PairConst {
@@ -161,6 +163,11 @@ struct ForceArgs {
   uint32_t* far_slot;   // their pre-step slots
   uint32_t* ftn;        // far movers per tile (tile t at t * kCtrStride)
   uint32_t* ftl;        // their pre-step slots (tile t at t * kFarCap)
+  // for the gradient rollout (variant 0, no edges; null otherwise): each particle's
+  // in-range partners (pre-step slots, in slot order, at most kPairList of them, at
+  // hits[k * kPairList]) and their number nhits[k], k = slot - first slot
+  uint32_t* hits;
+  uint32_t* nhits;
 };
 
 struct EdgeArgs {
@@ -345,7 +352,10 @@ hipError_t launch_hist(const HistArgs& a, int n, hipStream_t st);
 hipError_t launch_hist_grad(const HistArgs& a, int n, hipStream_t st);
 hipError_t launch_reduce_rows(const double* in, long stride, int n, int rows, double scale,
                               double* out, int accumulate, hipStream_t st);
-hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st);
+// One backward step: with `walk`, the FP32 walk lists each particle's in-range partners
+// (and computes the density-factor terms); without, the lists in AdjArgs::hits came from
+// the recompute's force kernel.
+hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st, bool walk);
 int adj_pair_blocks(int n);  // blocks (columns of AdjArgs::partial) of one backward step
 
 // host-side launchers (cf_kernels.hip)

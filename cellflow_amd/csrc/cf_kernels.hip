@@ -721,7 +721,9 @@ struct GrpAcc {
   int cnt;
 };
 
-template <bool EDGES, int R, int G, bool TILE>
+// HITS (the gradient rollout's recompute, no edges): also list each particle's in-range
+// partners in slot order (ForceArgs::hits), the lists the adjoint's pair kernel reads.
+template <bool EDGES, int R, int G, bool TILE, bool HITS = false>
 __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   constexpr int kRows = (2 * R + 1) * (2 * R + 1);
   constexpr int kRuns = 2 * kRows;
@@ -731,6 +733,8 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
   __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
   __shared__ uint32_t el_n[4][kOwn];
+  __shared__ uint32_t hl[HITS ? 4 : 1][HITS ? kOwn : 1][HITS ? kPairList : 1];
+  __shared__ uint32_t hn[HITS ? 4 : 1][HITS ? kOwn : 1];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int gw = xcd_block() * (blockDim.x >> 6) + wv;  // wave index in units of kOwn slots
@@ -759,6 +763,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   const int nzo = gen && sd.nz < 2 * R + 1 ? sd.nz : 2 * R + 1;
   const int nrows = nyo * nzo;
   if (EDGES && g == 0) el_n[wv][o] = 0u;
+  if (HITS && g == 0) hn[wv][o] = 0u;
   wave_lds_sync();  // pair table ready
 
   const PairConst* row = &tab[wv][ti * sd.T];
@@ -864,6 +869,10 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
         const PairConst pc = row[tj];
         const float sp = s2 + 1e-4f;
         const bool in = sp < pc.thr;
+        if (HITS && in && q != (uint32_t)p) {
+          const uint32_t at = atomicAdd(&hn[wv][o], 1u);
+          if (at < (uint32_t)kPairList) hl[wv][o][at] = q;
+        }
         if (in) {
           cnt++;
           const float inv = __builtin_amdgcn_rsqf(sp);
@@ -912,6 +921,10 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
         const uint32_t tj = __float_as_uint(pj.w);
         const PairConst pc = row[tj];
         const float sp = s2 + 1e-4f;
+        if (HITS && sp < pc.thr && q != (uint32_t)p) {
+          const uint32_t at = atomicAdd(&hn[wv][o], 1u);
+          if (at < (uint32_t)kPairList) hl[wv][o][at] = q;
+        }
         if (sp < pc.thr) {
           cnt++;
           const float inv = __builtin_amdgcn_rsqf(sp);
@@ -942,6 +955,22 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   fz = group_sum_f<G>(fz);
   cnt = group_sum_i<G>(cnt);
   if (active && 1e-4f < row[ti].thr) cnt -= 1;  // the particle itself
+
+  if (HITS) {
+    // the wave's lists are complete: each in slot order (rank by counting; slots are
+    // distinct), all of a particle's partners or, beyond kPairList, only their number
+    wave_lds_sync();
+    const uint32_t nh = active ? hn[wv][o] : 0u;
+    const long k = (long)(p - sd.p0);
+    if (nh <= (uint32_t)kPairList)
+      for (uint32_t e = g; e < nh; e += G) {
+        const uint32_t q = hl[wv][o][e];
+        uint32_t rank = 0;
+        for (uint32_t e2 = 0; e2 < nh; ++e2) rank += (hl[wv][o][e2] < q) ? 1u : 0u;
+        a.hits[k * kPairList + rank] = q;
+      }
+    if (active && g == 0) a.nhits[k] = nh;
+  }
 
   if (EDGES) {
     // the wave's candidate lists are complete
@@ -2155,6 +2184,11 @@ template <int R, int G>
 static void launch_grp(const ForceArgs& a, bool edges, hipStream_t st) {
   const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
   const dim3 g((unsigned)((waves + 3) / 4)), b(256);
+  if (a.hits) {  // the gradient rollout's lists (never with edges)
+    if (a.tile_sort) hipLaunchKernelGGL((k_force_grp<false, R, G, true, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_force_grp<false, R, G, false, true>), g, b, 0, st, a);
+    return;
+  }
   if (a.tile_sort) {
     if (edges) hipLaunchKernelGGL((k_force_grp<true, R, G, true>), g, b, 0, st, a);
     else hipLaunchKernelGGL((k_force_grp<false, R, G, true>), g, b, 0, st, a);
@@ -2179,6 +2213,7 @@ static void launch_list(const ForceArgs& a, bool edges, hipStream_t st) {
 
 hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
                         int lanes, hipStream_t st) {
+  if (a.hits && (variant != 0 || edges)) return hipErrorInvalidValue;  // lists: variant 0 only
   if (variant == 0) {
     if (reach == 1) {
       if (lanes == 4) launch_grp<1, 4>(a, edges, st);

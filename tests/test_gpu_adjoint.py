@@ -53,9 +53,11 @@ def target_hist(cfg, steps, bins, r_max):
     return sim.pairHistogram(bins, r_max)
 
 
-def engine_states(st, ph, steps):
+def engine_states(st, ph, steps, tuning=None):
     """FP32 positions and previous counts of every pre-step state 0..steps-1."""
     sim = make_sim(st, ph)
+    if tuning:
+        sim.setTuning(**tuning)
     prm = SimulationParams.from_physics(ph)
     pos = []
     for _ in range(steps):
@@ -64,14 +66,16 @@ def engine_states(st, ph, steps):
     return pos
 
 
-def check_against_fd(cfg, steps, ckpt, bins, r_max):
+def check_against_fd(cfg, steps, ckpt, bins, r_max, tuning=None):
     st, ph = setup(cfg)
     tgt = target_hist(cfg, steps, bins, r_max)
     sim = make_sim(st, ph)
+    if tuning:
+        sim.setTuning(**tuning)
     loss, h, gt, gr = sim.rolloutGrad(SimulationParams.from_physics(ph), steps, ckpt, bins, r_max,
                                       tgt)
     # oracle: FP64 rollout with the engine trajectory's decisions
-    pos = engine_states(st, ph, steps)
+    pos = engine_states(st, ph, steps, tuning)
     lists = [oracle.pairs(p, st["type"], ph, st["rbt"]) for p in pos]
     fr = oracle.FrozenRollout(st["pos"], st["vel"], st["type"], st["prev"], ph, st["rbt"], lists,
                               bins, r_max, tgt)
@@ -113,6 +117,15 @@ def test_histogram_matches_oracle():
 @pytest.mark.parametrize("steps,ckpt", [(40, 10), (37, 7)])
 def test_adjoint_matches_finite_differences_small(steps, ckpt):
     check_against_fd(small_cfg(), steps, ckpt, 16, 0.1)
+
+
+@pytest.mark.parametrize("variant,lanes", [(3, 8), (0, 4)])
+def test_adjoint_other_force_kernels(variant, lanes):
+    """Variant 0 lists each step's in-range partners in its force kernel during the
+    recompute; the other variants leave that to the adjoint's own walk.  Both paths, and
+    another lane count, against finite differences."""
+    check_against_fd(small_cfg(), 30, 10, 16, 0.1,
+                     dict(force_variant=variant, lanes_per_particle=lanes))
 
 
 def test_adjoint_dense_state_pairs_beyond_the_lds_list():
