@@ -111,8 +111,8 @@ __device__ __forceinline__ void adj_phi(const AdjArgs& a, const SimDesc& sd, int
 // over G lanes (cell index c = g, g+G, ...) and the lanes' partial sums are reduced in a
 // fixed order, so results are deterministic.
 
-constexpr int kAdjG = 16;                      // lanes per particle
-constexpr int kAdjBlock = 256;                 // threads per block
+constexpr int kAdjG = 8;                       // lanes per particle
+constexpr int kAdjBlock = 128;                 // threads per block
 constexpr int kAdjPer = kAdjBlock / kAdjG;     // particles per block
 constexpr int kAdjVals = 4 + kMaxTypes;        // lx, ly, lz, ratio, table row
 
@@ -438,12 +438,12 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
   for (int t = 0; t < kMaxTypes; ++t) mine[4 + t] = gfv[t];
   if (g == 0) btype[o] = active ? ti : -1;
   __syncthreads();
-  // lane g of each group sums value g over the group's lanes (only lane g touches column g
-  // of the group's first row)
-  if (g < kAdjVals) {
+  // lane g of each group sums values g, g + G, ... over the group's lanes (only lane v mod G
+  // touches column v of the group's first row)
+  for (int v = g; v < kAdjVals; v += G) {
     double acc = 0.0;
-    for (int l = 0; l < G; ++l) acc += red[o * G + l][g];
-    red[o * G][g] = acc;
+    for (int l = 0; l < G; ++l) acc += red[o * G + l][v];
+    red[o * G][v] = acc;
   }
   __syncthreads();
   if (active && g == 0) {
