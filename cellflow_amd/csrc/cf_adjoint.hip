@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_walk(AdjArgs a) {
   const bool active = k < sd.n;
   const AdjOwn c = adj_own(a, sd, k, active);
   if (g == 0) nhit[o] = 0u;
-  if (active && g == G - 1) adj_phi(a, sd, k);  // a lane that leads no list work below
+  if (active && g == G - 1) adj_phi(a, sd, k);  // not the lead lane (g == 0) of the list
   __syncthreads();
   if (active)
     for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
