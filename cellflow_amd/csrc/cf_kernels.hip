@@ -168,7 +168,14 @@ __device__ __forceinline__ void commit_cell(const ForceArgs& a, const SimDesc& s
   }
   const int ty = yn / sd.tr, y0 = ty * sd.tr;
   const int nrw = min(min(sd.tr, sd.ny - y0) + 2, sd.ny), npl = min(3, sd.nz);
-  const bool near = (cyo - y0 + 1 + sd.ny) % sd.ny < nrw && (czo - zn + 1 + sd.nz) % sd.nz < npl;
+  // the old row and plane relative to the window's first, cyclically (each difference lies
+  // within one period either way, so one correction replaces the modulo)
+  int dyw = cyo - y0 + 1, dzw = czo - zn + 1;
+  if (dyw < 0) dyw += sd.ny;
+  else if (dyw >= sd.ny) dyw -= sd.ny;
+  if (dzw < 0) dzw += sd.nz;
+  else if (dzw >= sd.nz) dzw -= sd.nz;
+  const bool near = dyw < nrw && dzw < npl;
   const uint32_t tile = (uint32_t)(sd.t0 + zn * sd.nyt + ty);
   uint32_t k = c;
   if (!near) {
@@ -1133,7 +1140,14 @@ __device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, 
                                          int r, uint32_t& s0, uint32_t& l0, uint32_t& k0,
                                          uint32_t& s1, uint32_t& l1, uint32_t& k1) {
   s0 = l0 = k0 = s1 = l1 = k1 = 0u;
-  const int iy = r % c.nyo, iz = r / c.nyo;
+  int iy, iz;
+  if (c.gen) {  // uniform: the simulation's grid decides
+    iy = r % c.nyo;
+    iz = r / c.nyo;
+  } else {  // nyo == 2R+1: division by a constant
+    iy = r % (2 * R + 1);
+    iz = r / (2 * R + 1);
+  }
   int y, z, dy = 0, dz = 0;
   uint32_t cyc = 0u, czc = 0u;
   if (c.gen && sd.ny < 2 * R + 1) {
