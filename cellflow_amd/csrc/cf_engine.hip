@@ -168,6 +168,9 @@ struct Engine {
   int edge_cap_alloc = 0;   // candidate rows allocated (2*maxc)
   int edge_maxc_alloc = 0;
   long long edges_count = -1;  // -1: no edge list built
+  int32_t* tris = nullptr;     // triangle mesh: 3 local indices per triangle
+  long long tris_cap = 0;      // triangles `tris` holds
+  long long tri_count = -1;    // -1: no mesh built
   int edges_maxc = 0;
   uint32_t* slot_of = nullptr;
   // tile sort of the step: per-parity counters (the step with state buffer A counts into
@@ -219,7 +222,9 @@ struct Engine {
     dfree(b.hist); dfree(b.tmp_id); dfree(b.acc); dfree(b.pad);
     dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.eoff);
     dfree(b.edges); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
-    dfree(slot_of); dfree(tickets);
+    dfree(slot_of); dfree(tickets); dfree(tris);
+    tris_cap = 0;
+    tri_count = -1;
     for (int i = 0; i < 2; ++i) {
       dfree(tcnt[i]);
       dfree(etcnt[i]);
@@ -586,9 +591,16 @@ struct Engine {
     if (maxc > edge_maxc_alloc) {
       HIPR(hipStreamSynchronize(st));
       dfree(b.estage);
-      dfree(b.edges);
       CFR(dalloc(&b.estage, (size_t)maxc * P));
-      CFR(dalloc(&b.edges, (size_t)maxc * P));
+      // the last graph stays readable when a mesh with a longer list grows the buffers
+      decltype(b.edges) grown = nullptr;
+      CFR(dalloc(&grown, (size_t)maxc * P));
+      if (edges_count > 0 && b.edges)
+        HIPR(hipMemcpyAsync(grown, b.edges, (size_t)edges_count * sizeof(*grown),
+                            hipMemcpyDeviceToDevice, st));
+      HIPR(hipStreamSynchronize(st));
+      dfree(b.edges);
+      b.edges = grown;
       edge_maxc_alloc = maxc;
     }
     return CF_OK;
@@ -987,6 +999,114 @@ struct Engine {
                           hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
     return CF_OK;
+  }
+
+  // Triangle mesh of the current state (generateTriangleMesh, ParticleSimulation.cu:689-754)
+  // in canonical order: owner index ascending, then the owner's fan order.
+  int tri_mesh(float distance, int max_conn) {
+    tri_count = -1;
+    const int maxc = std::min(max_conn, 32);  // :303
+    if (maxc <= 0) {
+      tri_count = 0;
+      return CF_OK;
+    }
+    CFR(ensure_edge_bufs(maxc));
+    EdgeReq none;
+    CFR(prepare(none));
+    const SimDesc& d = hdesc[0];
+    const float hm = 1.2f;                             // :722, a 20% larger radius
+    const float d2 = distance * distance * (hm * hm);  // :723, FP32 left to right
+    TriArgs ta{};
+    ta.pos = b.pos[cur];
+    ta.id = b.id[cur];
+    ta.start = b.start[cur];
+    ta.sims = b.sims;
+    ta.wave_sim = b.wave_sim;
+    ta.cand_d2 = b.ecand_d2;
+    ta.cand_id = b.ecand_id;
+    ta.sorted = b.estage;
+    ta.nsel = b.tmp_id;
+    ta.count = b.ecount;
+    ta.off = b.eoff;
+    ta.P = P;
+    ta.nwaves = nwaves;
+    ta.maxc = maxc;
+    ta.d2 = d2;
+    // cells within reach of sqrt(d2) on each axis (non-periodic, as :314-317)
+    const double reach = std::sqrt((double)d2) * (1.0 + 1.0 / 1024.0);
+    const float ih[3] = {d.ihx, d.ihy, d.ihz};
+    int r[3];
+    for (int a = 0; a < 3; ++a) r[a] = (int)std::max(1.0, std::min(std::ceil(reach * ih[a]), 1024.0));
+    ta.rx = r[0];
+    ta.ry = r[1];
+    ta.rz = r[2];
+    HIPR(launch_tri_count(ta, st));
+    HIPR(launch_scan(plan(2, nullptr), st));
+    HIPR(hipMemcpyAsync(hsmall + 2, b.eoff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    const long long total = (long long)hsmall[2];
+    if (total > tris_cap) {
+      dfree(tris);
+      tris_cap = 0;
+      CFR(dalloc(&tris, (size_t)(3 * total)));
+      tris_cap = total;
+    }
+    ta.tris = tris;
+    if (total > 0) HIPR(launch_tri_write(ta, st));
+    HIPR(hipStreamSynchronize(st));
+    tri_count = total;
+    return CF_OK;
+  }
+
+  int get_triangles(int32_t* out, long long capacity, long long* count) {
+    if (tri_count < 0) return fail(CF_ERR_STATE, "no triangle mesh has been built");
+    *count = tri_count;
+    if (capacity < tri_count)
+      return fail(CF_ERR_CAPACITY, "triangle buffer holds %lld, %lld needed", capacity, tri_count);
+    if (tri_count > 0)
+      HIPR(hipMemcpyAsync(out, tris, (size_t)(3 * tri_count) * sizeof(int32_t),
+                          hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    return CF_OK;
+  }
+
+  int triangle_vertices(const cf_color* colors, int ncolors, float* out, long long cap_floats,
+                        long long* vcount) {
+    if (tri_count < 0) return fail(CF_ERR_STATE, "no triangle mesh has been built");
+    *vcount = 3 * tri_count;
+    const long long need = 27 * tri_count;
+    if (cap_floats < need)
+      return fail(CF_ERR_CAPACITY, "vertex buffer holds %lld floats, %lld needed", cap_floats, need);
+    if (tri_count == 0) return CF_OK;
+    float* dcol = nullptr;
+    float* dout = nullptr;
+    int rc = dalloc(&dcol, 3 * (size_t)std::max(1, ncolors));
+    if (rc == CF_OK) rc = dalloc(&dout, (size_t)need);
+    if (rc == CF_OK && ncolors > 0 &&
+        hipMemcpyAsync(dcol, colors, 3 * (size_t)ncolors * sizeof(float), hipMemcpyHostToDevice,
+                       st) != hipSuccess)
+      rc = fail(CF_ERR_DEVICE, "colour upload failed");
+    if (rc == CF_OK && launch_slot_of(b.id[cur], slot_of, b.sims, 0, sims[0].n, st) != hipSuccess)
+      rc = fail(CF_ERR_DEVICE, "index map failed");
+    if (rc == CF_OK) {
+      TriVertArgs va{};
+      va.tris = tris;
+      va.pos = b.pos[cur];
+      va.slot_of = slot_of;
+      va.colors = dcol;
+      va.out = dout;
+      va.ntris = tri_count;
+      va.ncolors = ncolors;
+      va.T = sims[0].T;
+      if (launch_tri_vertices(va, st) != hipSuccess) rc = fail(CF_ERR_DEVICE, "vertex kernel failed");
+    }
+    if (rc == CF_OK && hipMemcpyAsync(out, dout, (size_t)need * sizeof(float),
+                                      hipMemcpyDeviceToHost, st) != hipSuccess)
+      rc = fail(CF_ERR_DEVICE, "vertex download failed");
+    (void)hipStreamSynchronize(st);
+    dfree(dcol);
+    dfree(dout);
+    return rc;
   }
 
   int line_vertices(const cf_color* colors, int ncolors, float* out, long long cap_floats,
@@ -2044,6 +2164,30 @@ int cf_get_edges(cf_sim* sim, int32_t* pairs, long long capacity, long long* cou
   CFR(check(sim));
   if (!count || (!pairs && capacity > 0)) return cf::fail(CF_ERR_ARG, "null argument");
   return sim->e->get_edges(pairs, capacity, count);
+}
+
+int cf_build_triangles(cf_sim* sim, float distance, int max_connections) {
+  CFR(check(sim));
+  if (!(distance >= 0.0f) || !std::isfinite(distance))
+    return cf::fail(CF_ERR_ARG, "distance must be finite and >= 0");
+  cf::Engine& e = *sim->e;
+  if (!e.sims[0].have_prm)
+    return cf::fail(CF_ERR_STATE, "call cf_simulate (or set parameters) before building a mesh");
+  return e.tri_mesh(distance, max_connections);
+}
+
+int cf_get_triangles(cf_sim* sim, int32_t* tris, long long capacity, long long* count) {
+  CFR(check(sim));
+  if (!count || (!tris && capacity > 0)) return cf::fail(CF_ERR_ARG, "null argument");
+  return sim->e->get_triangles(tris, capacity, count);
+}
+
+int cf_get_triangle_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float* out,
+                             long long capacity_floats, long long* vertex_count) {
+  CFR(check(sim));
+  if (!vertex_count || (!out && capacity_floats > 0) || (!colors && ncolors > 0))
+    return cf::fail(CF_ERR_ARG, "null argument");
+  return sim->e->triangle_vertices(colors, ncolors, out, capacity_floats, vertex_count);
 }
 
 int cf_get_line_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float* out,

@@ -302,6 +302,42 @@ struct LineArgs {
   int p0;
 };
 
+// Triangle mesh of the current state (generateTriangleMeshKernel): per owner slot, the
+// kept same-type neighbours and the owner's triangle count, then the canonical triangles.
+struct TriArgs {
+  const float4* __restrict__ pos;
+  const uint32_t* __restrict__ id;
+  const uint32_t* __restrict__ start;
+  const SimDesc* __restrict__ sims;
+  const uint32_t* __restrict__ wave_sim;
+  float* cand_d2;        // [maxc][P] scratch of the index selection
+  uint32_t* cand_id;
+  uint32_t* sorted;      // [maxc][P] each owner's kept neighbours, ascending
+  uint32_t* nsel;        // [P] their number, by owner slot
+  uint32_t* count;       // [P] triangles of each owner, by owner index
+  const uint32_t* off;   // exclusive scan of count
+  int32_t* tris;         // 3 indices per triangle
+  long P;
+  int nwaves;
+  int maxc;              // min(maxConnectionsPerParticle, 32)
+  float d2;              // squared distance threshold, hysteresis included
+  int rx, ry, rz;        // non-periodic stencil reach in cells
+};
+
+struct TriVertArgs {
+  const int32_t* __restrict__ tris;
+  const float4* __restrict__ pos;
+  const uint32_t* __restrict__ slot_of;   // index -> slot
+  const float* __restrict__ colors;       // ncolors x 3
+  float* out;                             // 27 floats per triangle
+  long ntris;
+  int ncolors;
+  int T;
+};
+hipError_t launch_tri_count(const TriArgs& a, hipStream_t st);
+hipError_t launch_tri_write(const TriArgs& a, hipStream_t st);
+hipError_t launch_tri_vertices(const TriVertArgs& a, hipStream_t st);
+
 // ---- differentiable rollout (cf_adjoint.hip) ----
 constexpr int kMaxBins = 64;
 

@@ -2190,6 +2190,134 @@ __global__ void k_line_vertices(LineArgs a) {
 }
 
 // ------------------------------------------------------------------------------------
+// Triangle mesh (generateTriangleMeshKernel, ParticleSimulation.cu:279-424), in canonical
+// order instead of the reference's atomicAdd order (:353).  Per owner i: its neighbours are
+// the first maxc particles j != i of its type, in index order, whose non-periodic squared
+// distance is below d2 (:306-324), that is the maxc smallest such indices; its triangles are
+// (i, a, b) for the pairs a < b of that list with i < a, in list order, at most 64
+// (:334-348).  The owners are visited by slot over a non-periodic stencil.
+
+__global__ void __launch_bounds__(256) k_tri_count(TriArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (gw >= a.nwaves) return;
+  const SimDesc sd = a.sims[a.wave_sim[gw]];
+  const int p = gw * kWave + lane;
+  if (p - sd.p0 >= sd.n) return;
+  const float4 pi = a.pos[p];
+  const uint32_t idi = a.id[p];
+  const uint32_t ti = __float_as_uint(pi.w);
+  const int cx = bin_axis(pi.x, sd.ihx, sd.nx);
+  const int cy = bin_axis(pi.y, sd.ihy, sd.ny);
+  const int cz = bin_axis(pi.z, sd.ihz, sd.nz);
+  const int xlo = max(cx - a.rx, 0), xhi = min(cx + a.rx, sd.nx - 1);
+  const int ylo = max(cy - a.ry, 0), yhi = min(cy + a.ry, sd.ny - 1);
+  const int zlo = max(cz - a.rz, 0), zhi = min(cz + a.rz, sd.nz - 1);
+  EdgeAcc ea;
+  ea.q = 0;
+  ea.maxid = 0;
+  ea.maxk = 0;
+  for (int z = zlo; z <= zhi; ++z)
+    for (int y = ylo; y <= yhi; ++y) {
+      const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+      const int q0 = (int)a.start[rowc + xlo];
+      const int q1 = (int)a.start[rowc + xhi + 1];
+      for (int q = q0; q < q1; ++q) {
+        if (q == p) continue;                             // :307
+        const float4 pj = a.pos[q];
+        if (__float_as_uint(pj.w) != ti) continue;       // :311
+        const float dx = pj.x - pi.x;                     // :314-317
+        const float dy = pj.y - pi.y;
+        const float dz = pj.z - pi.z;
+        const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (!(s2 < a.d2)) continue;                       // :319
+        edge_push(ea, s2, a.id[q], a.maxc, a.cand_d2, a.cand_id, a.P, p);
+      }
+    }
+  // the kept indices in ascending order (distinct, so rank by counting), and the owner's
+  // triangles: pairs among the kept indices above its own
+  const int m = min(ea.q, a.maxc);
+  int above = 0;
+  for (int k = 0; k < m; ++k) {
+    const uint32_t ik = a.cand_id[(long)k * a.P + p];
+    int rank = 0;
+    for (int k2 = 0; k2 < m; ++k2) rank += (a.cand_id[(long)k2 * a.P + p] < ik) ? 1 : 0;
+    a.sorted[(long)rank * a.P + p] = ik;
+    above += (ik > idi) ? 1 : 0;
+  }
+  a.nsel[p] = (uint32_t)m;
+  a.count[idi] = (uint32_t)min(64, above * (above - 1) / 2);
+}
+
+__global__ void __launch_bounds__(256) k_tri_write(TriArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (gw >= a.nwaves) return;
+  const SimDesc sd = a.sims[a.wave_sim[gw]];
+  const int p = gw * kWave + lane;
+  if (p - sd.p0 >= sd.n) return;
+  const uint32_t idi = a.id[p];
+  const uint32_t nt = a.count[idi];
+  if (nt == 0u) return;
+  const int m = (int)a.nsel[p];
+  int s0 = 0;  // the first kept index above the owner's
+  while (s0 < m && a.sorted[(long)s0 * a.P + p] <= idi) ++s0;
+  long w = (long)a.off[idi];
+  uint32_t t = 0;
+  for (int i1 = s0; i1 < m && t < nt; ++i1) {
+    const uint32_t n1 = a.sorted[(long)i1 * a.P + p];
+    for (int i2 = i1 + 1; i2 < m && t < nt; ++i2, ++t, ++w) {
+      a.tris[3 * w] = (int32_t)idi - sd.p0;
+      a.tris[3 * w + 1] = (int32_t)n1 - sd.p0;
+      a.tris[3 * w + 2] = (int32_t)a.sorted[(long)i2 * a.P + p] - sd.p0;
+    }
+  }
+}
+
+// The reference's vertex layout (:362-418): per triangle three vertices of position,
+// flat normal (normalised cross product of the two edges from the first vertex, left as
+// is when its length is at most 1e-4) and the owner type's colour.
+__global__ void k_tri_vertices(TriVertArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.ntris) return;
+  const float4 v1 = a.pos[a.slot_of[a.tris[3 * t]]];  // local indices, like slot_of's
+  const float4 v2 = a.pos[a.slot_of[a.tris[3 * t + 1]]];
+  const float4 v3 = a.pos[a.slot_of[a.tris[3 * t + 2]]];
+  const float e1x = v2.x - v1.x, e1y = v2.y - v1.y, e1z = v2.z - v1.z;
+  const float e2x = v3.x - v1.x, e2y = v3.y - v1.y, e2z = v3.z - v1.z;
+  float nx = e1y * e2z - e1z * e2y;
+  float ny = e1z * e2x - e1x * e2z;
+  float nz = e1x * e2y - e1y * e2x;
+  const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+  if (len > 0.0001f) {
+    nx /= len;
+    ny /= len;
+    nz /= len;
+  }
+  const uint32_t ci = __float_as_uint(v1.w) % (uint32_t)a.T;
+  float r = 0.0f, g = 0.0f, b = 0.0f;
+  if ((int)ci < a.ncolors) {
+    r = a.colors[3 * ci];
+    g = a.colors[3 * ci + 1];
+    b = a.colors[3 * ci + 2];
+  }
+  float* o = a.out + t * 27;
+  const float4 v[3] = {v1, v2, v3};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o[9 * k + 0] = v[k].x;
+    o[9 * k + 1] = v[k].y;
+    o[9 * k + 2] = v[k].z;
+    o[9 * k + 3] = nx;
+    o[9 * k + 4] = ny;
+    o[9 * k + 5] = nz;
+    o[9 * k + 6] = r;
+    o[9 * k + 7] = g;
+    o[9 * k + 8] = b;
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // host-side launchers
 
 static inline int grid_for(int nwaves, int block) { return (nwaves * kWave + block - 1) / block; }
@@ -2307,6 +2435,22 @@ hipError_t launch_rank_scatter(const SortArgs& a, hipStream_t st) {
 
 hipError_t launch_edge_write(const SortArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_edge_write, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_count(const TriArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_tri_count, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_write(const TriArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_tri_write, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tri_vertices(const TriVertArgs& a, hipStream_t st) {
+  const long g = (a.ntris + 255) / 256;
+  if (g > 0) hipLaunchKernelGGL(k_tri_vertices, dim3((unsigned)g), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -308,6 +308,37 @@ class ParticleSimulation:
         check(lib().cf_get_edges(self._h, _ip(out), cnt.value, C.byref(cnt)))
         return out[: cnt.value]
 
+    def generateTriangleMesh(self, proximityDistance: float, maxConnectionsPerParticle: int):
+        """generateTriangleMesh (ParticleSimulation.cu:689-754) on the current state; returns
+        the (T, 3) int32 triangles in canonical (owner, fan) order."""
+        check(lib().cf_build_triangles(self._h, float(proximityDistance),
+                                       int(maxConnectionsPerParticle)))
+        return self.getTriangles()
+
+    def getTriangles(self) -> np.ndarray:
+        cnt = C.c_longlong()
+        rc = lib().cf_get_triangles(self._h, None, 0, C.byref(cnt))
+        if rc not in (_abi.CF_OK, _abi.CF_ERR_CAPACITY):
+            check(rc)
+        out = np.zeros((max(cnt.value, 1), 3), np.int32)
+        check(lib().cf_get_triangles(self._h, _ip(out), cnt.value, C.byref(cnt)))
+        return out[: cnt.value]
+
+    def getTriangleVertices(self, colors) -> np.ndarray:
+        """The reference's triangle-vertex layout (ParticleSimulation.cu:362-418): (3T, 9)
+        rows of position, flat normal and colour."""
+        col = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
+        vc = C.c_longlong()
+        rc = lib().cf_get_triangle_vertices(self._h, col.ctypes.data_as(C.POINTER(_abi.cf_color)),
+                                            len(col), None, 0, C.byref(vc))
+        if rc not in (_abi.CF_OK, _abi.CF_ERR_CAPACITY):
+            check(rc)
+        out = np.zeros(max(9 * vc.value, 1), np.float32)
+        check(lib().cf_get_triangle_vertices(self._h,
+                                             col.ctypes.data_as(C.POINTER(_abi.cf_color)),
+                                             len(col), _fp(out), 9 * vc.value, C.byref(vc)))
+        return out[: 9 * vc.value].reshape(-1, 9)
+
     def getLineVertices(self, colors) -> np.ndarray:
         """The reference's GL line-vertex layout (ParticleSimulation.cu:253-275)."""
         col = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)

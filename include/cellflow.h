@@ -184,6 +184,23 @@ int cf_get_edges(cf_sim* sim, int32_t* pairs, long long capacity, long long* cou
 int cf_get_line_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float* out,
                          long long capacity_floats, long long* vertex_count);
 
+/* generateTriangleMesh, .cu:689-754 (its kernel :279-424), on the current state: for each
+ * owner i, the first min(max_connections, 32) particles of its type in index order within
+ * 1.2 * distance (non-periodic), and the fan triangles (i, a, b) over the pairs a < b of
+ * that list with i < a, at most 64 per owner.  Triangles are kept in canonical order
+ * (owner ascending, then fan order) instead of the reference's atomicAdd order (:353),
+ * and read with cf_get_triangles / cf_get_triangle_vertices (the GL VBO is replaced). */
+int cf_build_triangles(cf_sim* sim, float distance, int max_connections);
+/* The last built mesh: `tris` holds capacity triangles of three int32 indices.  Returns
+ * CF_ERR_CAPACITY with *count set to the needed number when capacity is too small. */
+int cf_get_triangles(cf_sim* sim, int32_t* tris, long long capacity, long long* count);
+/* The reference's triangle-vertex layout (.cu:362-418): per triangle three vertices of
+ * (x, y, z, nx, ny, nz, r, g, b), the flat normal normalised when longer than 1e-4,
+ * colour = colors[ptype_owner % num_types]; *vertex_count = 3 * triangles (the value
+ * generateTriangleMesh returns through outVertexCount). */
+int cf_get_triangle_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float* out,
+                             long long capacity_floats, long long* vertex_count);
+
 /* ---- introspection (parity checks) -------------------------------------------------- */
 
 /* Grid of the current binning: cells per axis and the reach of the force stencil. */

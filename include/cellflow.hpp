@@ -235,6 +235,21 @@ class ParticleSimulation {
                                vertices.data(), (long long)vertices.size(), &vc));
     outVertexCount = (int)vc;
   }
+  // generateTriangleMesh, .cu:689-754, into a host vertex buffer: 3 vertices per triangle
+  // of (x, y, z, nx, ny, nz, r, g, b), colour = particleColors[ptype_owner % numTypes].
+  void generateTriangleMesh(std::vector<float>& vertices, int& outVertexCount,
+                            float proximityDistance, int maxConnectionsPerParticle,
+                            const std::vector<ParticleColor>& particleColors) {
+    check(cf_build_triangles(h_, proximityDistance, maxConnectionsPerParticle));
+    long long vc = 0;
+    const int rc = cf_get_triangle_vertices(h_, particleColors.data(),
+                                            (int)particleColors.size(), nullptr, 0, &vc);
+    if (rc != CF_OK && rc != CF_ERR_CAPACITY) check(rc);
+    vertices.resize((size_t)vc * 9);
+    check(cf_get_triangle_vertices(h_, particleColors.data(), (int)particleColors.size(),
+                                   vertices.data(), (long long)vertices.size(), &vc));
+    outVertexCount = (int)vc;
+  }
   // The last proximity graph as canonical (i, j) pairs.
   std::vector<int32_t> getEdges() {
     long long n = 0;

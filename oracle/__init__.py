@@ -10,6 +10,8 @@ reference root):
 * :func:`step`          cuda-native/src/ParticleSimulation.cu:69-166 (Jacobi form)
 * :func:`rollout`       ParticleSimulation.cu:541-556 (count ping-pong over steps)
 * :func:`edges`         ParticleSimulation.cu:187-246
+* :func:`triangles`     ParticleSimulation.cu:279-348 with the driver's threshold :722-724
+* :func:`triangle_vertices`  ParticleSimulation.cu:350-418
 * :func:`force_table`   ParticleSimulation.cu:521-531
 * :func:`move`          ParticleSimulation.cu:169-185
 * :func:`bin_cells`, :func:`order`   the engine's grid rule (no reference counterpart)
@@ -63,6 +65,12 @@ def lib():
         L.orc_order.argtypes = [C.c_int, P(C.c_uint32), P(C.c_uint32)]
         L.orc_edges.argtypes = [C.c_int, P(C.c_float), P(C.c_uint32), C.c_float, C.c_int, C.c_int,
                                 P(C.c_int32), C.c_longlong, P(C.c_longlong), C.c_int]
+        L.orc_triangles.argtypes = [C.c_int, P(C.c_float), P(C.c_uint32), C.c_float, C.c_int,
+                                    C.c_int, P(C.c_int32), C.c_longlong, P(C.c_longlong), C.c_int]
+        L.orc_triangle_vertices.argtypes = [C.c_longlong, P(C.c_int32), P(C.c_float),
+                                            P(C.c_uint32), C.c_int, P(C.c_float), C.c_int,
+                                            P(C.c_float)]
+        L.orc_triangle_vertices.restype = None
         L.orc_force_table.argtypes = [P(C.c_float), C.c_int, C.c_float, C.c_float, C.c_float,
                                       P(C.c_float)]
         L.orc_force_table.restype = None
@@ -172,6 +180,34 @@ def edges(pos, ptype, dist, max_conn, mode=1, nthreads=0):
     if rc:
         raise ValueError(f"orc_edges failed rc={rc}")
     return out[: cnt.value].copy()
+
+
+def triangles(pos, ptype, dist, max_conn, mode=1, nthreads=0):
+    """Canonical triangle mesh, shape (T, 3) int32: owner ascending, then fan order."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    ptype = np.ascontiguousarray(ptype, np.uint32)
+    n = len(ptype)
+    cap = max(1, n * 64)
+    out = np.zeros((cap, 3), np.int32)
+    cnt = C.c_longlong(0)
+    rc = lib().orc_triangles(n, _p(pos, C.c_float), _p(ptype, C.c_uint32), float(dist),
+                             int(max_conn), mode, _p(out, C.c_int32), cap, C.byref(cnt), nthreads)
+    if rc:
+        raise ValueError(f"orc_triangles failed rc={rc}")
+    return out[: cnt.value].copy()
+
+
+def triangle_vertices(tris, pos, ptype, num_types, colors):
+    """(3T, 9) float32 vertex rows (position, flat normal, owner colour) of `tris`."""
+    tris = np.ascontiguousarray(tris, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    ptype = np.ascontiguousarray(ptype, np.uint32)
+    col = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
+    out = np.zeros((max(len(tris), 1) * 3, 9), np.float32)
+    lib().orc_triangle_vertices(len(tris), _p(tris, C.c_int32), _p(pos, C.c_float),
+                                _p(ptype, C.c_uint32), int(num_types), _p(col, C.c_float),
+                                len(col), _p(out, C.c_float))
+    return out[: 3 * len(tris)]
 
 
 def force_table(raw, rng, bias, offset):

@@ -405,6 +405,114 @@ int orc_edges(int n, const float* pos, const uint32_t* type, float dist, int max
   return 0;
 }
 
+// Triangle mesh (generateTriangleMeshKernel, ParticleSimulation.cu:279-348, with the
+// driver's threshold :722-724).  Output: triangles (i, a, b) in canonical order (owner i
+// ascending, then the owner's fan order).  mode 0 is the reference's own scan (j = 0.. while
+// found < maxC, :306); mode 1 uses a non-periodic grid (same result).  Returns 0, or 3 when
+// `capacity` triangles are too few (count_out then holds the need).
+int orc_triangles(int n, const float* pos, const uint32_t* type, float dist, int max_conn,
+                  int mode, int32_t* tris_out, long long capacity, long long* count_out,
+                  int nthreads) {
+  set_threads(nthreads);
+  const float hm = 1.2f;
+  const float d2 = dist * dist * (hm * hm);     // :723
+  const int maxc = std::min(max_conn, 32);      // :303
+  std::vector<std::vector<int>> tri(n);         // per owner: a, b of each triangle
+  if (maxc <= 0 || n <= 1) {
+    *count_out = 0;
+    return 0;
+  }
+  auto d2of = [&](int i, int j) {               // :314-317
+    float dx = pos[3 * j] - pos[3 * i];
+    float dy = pos[3 * j + 1] - pos[3 * i + 1];
+    float dz = pos[3 * j + 2] - pos[3 * i + 2];
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  };
+  Grid g;
+  if (mode == 1) {
+    float Lb[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) Lb[a] = std::max(Lb[a], pos[3 * i + a]);
+    for (int a = 0; a < 3; ++a) Lb[a] = std::nextafter(Lb[a], 1e30f) * 1.001f + 1e-30f;
+    g = build_grid(n, pos, Lb, std::max(std::sqrt((double)d2), 1e-6));
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    std::vector<int> near;  // the neighbour list of :301-324, ascending index
+    if (mode == 0) {
+      for (int j = 0; j < n && (int)near.size() < maxc; ++j) {
+        if (j == i || type[j] != type[i]) continue;
+        if (d2of(i, j) < d2) near.push_back(j);
+      }
+    } else {
+      for_each_near(g, pos, i, 1, false, [&](int j) {
+        if (j == i || type[j] != type[i]) return;
+        if (d2of(i, j) < d2) near.push_back(j);
+      });
+      std::sort(near.begin(), near.end());
+      if ((int)near.size() > maxc) near.resize(maxc);
+    }
+    // :334-348 fan triangles, at most 64
+    std::vector<int>& t = tri[i];
+    for (size_t a = 0; a < near.size() && t.size() < 128; ++a)
+      for (size_t b = a + 1; b < near.size() && t.size() < 128; ++b)
+        if (i < near[a] && i < near[b]) {
+          t.push_back(near[a]);
+          t.push_back(near[b]);
+        }
+  }
+  long long total = 0;
+  for (int i = 0; i < n; ++i) total += (long long)tri[i].size() / 2;
+  *count_out = total;
+  if (total > capacity) return 3;
+  long long w = 0;
+  for (int i = 0; i < n; ++i)
+    for (size_t k = 0; k < tri[i].size(); k += 2, ++w) {
+      tris_out[3 * w] = i;
+      tris_out[3 * w + 1] = tri[i][k];
+      tris_out[3 * w + 2] = tri[i][k + 1];
+    }
+  return 0;
+}
+
+// Vertices of triangles (:362-418): per vertex x, y, z, nx, ny, nz, r, g, b with the flat
+// normal of the triangle (cross product of the edges from its first vertex, normalised when
+// longer than 1e-4) and the colour of the first vertex's type, colors[ptype % T].  FP32.
+void orc_triangle_vertices(long long ntri, const int32_t* tris, const float* pos,
+                           const uint32_t* type, int T, const float* colors, int ncolors,
+                           float* out) {
+  for (long long t = 0; t < ntri; ++t) {
+    const float* v[3] = {pos + 3 * tris[3 * t], pos + 3 * tris[3 * t + 1], pos + 3 * tris[3 * t + 2]};
+    const float e1x = v[1][0] - v[0][0], e1y = v[1][1] - v[0][1], e1z = v[1][2] - v[0][2];
+    const float e2x = v[2][0] - v[0][0], e2y = v[2][1] - v[0][1], e2z = v[2][2] - v[0][2];
+    float nx = e1y * e2z - e1z * e2y;
+    float ny = e1z * e2x - e1x * e2z;
+    float nz = e1x * e2y - e1y * e2x;
+    const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+    if (len > 0.0001f) {
+      nx /= len;
+      ny /= len;
+      nz /= len;
+    }
+    const uint32_t c = type[tris[3 * t]] % (uint32_t)T;
+    float col[3] = {0.0f, 0.0f, 0.0f};
+    if ((int)c < ncolors)
+      for (int k = 0; k < 3; ++k) col[k] = colors[3 * c + k];
+    for (int k = 0; k < 3; ++k) {
+      float* o = out + 27 * t + 9 * k;
+      o[0] = v[k][0];
+      o[1] = v[k][1];
+      o[2] = v[k][2];
+      o[3] = nx;
+      o[4] = ny;
+      o[5] = nz;
+      o[6] = col[0];
+      o[7] = col[1];
+      o[8] = col[2];
+    }
+  }
+}
+
 // updateForceTable (ParticleSimulation.cu:521-531): clamp(tanh(raw*offset)*range+bias, -1, 1).
 // The operands are float, so the float overloads of tanh/fmax/fmin apply; all in FP32.
 void orc_force_table(const float* raw, int count, float range, float bias, float offset, float* out) {

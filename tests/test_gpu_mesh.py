@@ -1,0 +1,109 @@
+"""GPU parity of the triangle mesh (generateTriangleMesh, ParticleSimulation.cu:689-754,
+and the triangle-vertex layout of ParticleSimulation.cu:362-418) against the oracle.
+
+Triangles are index work and must match the oracle's canonical list (owner ascending, then
+fan order) bit for bit.  Vertex positions and colours are copies and must be exact; the
+flat normals are the same FP32 expression in both and are held to 1e-6 absolute.
+"""
+
+import numpy as np
+import pytest
+
+from cellflow_amd import CellFlowError, ParticleSimulation
+from helpers import make_sim, small_cfg, state_of
+
+pytestmark = pytest.mark.gpu
+
+NORMAL_ATOL = 1e-6
+
+
+def clustered(n=5000, side=1.0, T=2, blobs=4, sd=0.006, seed=3):
+    cfg = small_cfg(n=n, side=side, T=T, seed=seed)
+    cfg["init"] = dict(kind="blobs", count=blobs, sd=sd)
+    return state_of(cfg)
+
+
+def check_vertices(orc, sim, tris, T):
+    got = sim.getParticleData()
+    rng = np.random.default_rng(T)
+    colors = rng.random((T, 3)).astype(np.float32)
+    v = sim.getTriangleVertices(colors)
+    ref = orc.triangle_vertices(tris, got["pos"], got["ptype"], T, colors)
+    assert v.shape == (3 * len(tris), 9)
+    assert np.array_equal(v[:, :3], ref[:, :3])
+    assert np.array_equal(v[:, 6:], ref[:, 6:])
+    np.testing.assert_allclose(v[:, 3:6], ref[:, 3:6], rtol=0, atol=NORMAL_ATOL)
+
+
+@pytest.mark.parametrize("maxc", [1, 4, 16, 32, 40])
+def test_mesh_dense_blobs_matches_oracle(orc, maxc):
+    """Dense blobs: most owners have far more same-type neighbours than the list holds, so
+    the index-order truncation and the 64-triangle cap both matter (maxConn 1 keeps one
+    neighbour per owner, hence no triangle)."""
+    st, phys, params = clustered()
+    sim = make_sim(st, phys)
+    sim.simulate(params, 0)  # the blobs as seeded (a step spreads them)
+    got = sim.getParticleData()
+    for dist in (0.004, 0.012, 0.15):  # 0.15 * 1.2 spans several cells
+        t = sim.generateTriangleMesh(dist, maxc)
+        ref = orc.triangles(got["pos"], got["ptype"], dist, maxc, mode=1)
+        assert np.array_equal(t, ref), f"dist={dist} maxc={maxc}: {len(t)} vs {len(ref)}"
+        assert np.array_equal(sim.getTriangles(), t)
+        assert (len(t) > 0) == (maxc > 1)
+        if dist == 0.004:
+            check_vertices(orc, sim, t, 2)
+
+
+@pytest.mark.parametrize("name,over", [
+    ("uniform", dict(n=4000, side=0.5, T=6)),
+    ("ten_types", dict(n=3000, side=0.4, T=10)),
+    ("tiny_grid", dict(n=400, side=0.08, T=3)),
+])
+def test_mesh_uniform_matches_oracle(orc, name, over):
+    st, phys, params = state_of(small_cfg(**over))
+    sim = make_sim(st, phys)
+    sim.simulate(params, 2, edges=(0.03, 5))  # a graph beside the mesh changes neither
+    got = sim.getParticleData()
+    t = sim.generateTriangleMesh(0.03, 8)
+    ref = orc.triangles(got["pos"], got["ptype"], 0.03, 8, mode=1)
+    assert np.array_equal(t, ref)
+    assert len(t) > 0
+    assert np.array_equal(sim.getEdges(), orc.edges(got["pos"], got["ptype"], 0.03, 5, mode=0))
+    check_vertices(orc, sim, t, over["T"])
+
+
+def test_mesh_non_cubic_canvas(orc):
+    st, phys, params = state_of(small_cfg(n=3000, side=0.5, canvas=[0.8, 0.3, 0.45]))
+    sim = make_sim(st, phys)
+    sim.simulate(params, 1)
+    got = sim.getParticleData()
+    t = sim.generateTriangleMesh(0.04, 6)
+    assert np.array_equal(t, orc.triangles(got["pos"], got["ptype"], 0.04, 6, mode=1))
+
+
+def test_mesh_rebuild_shrinks_and_zero_distance(orc):
+    st, phys, params = state_of(small_cfg(n=3000, side=0.5))
+    sim = make_sim(st, phys)
+    sim.simulate(params, 0)
+    big = sim.generateTriangleMesh(0.05, 12)
+    small = sim.generateTriangleMesh(0.02, 4)
+    assert 0 < len(small) < len(big)
+    assert np.array_equal(small, orc.triangles(st["pos"], st["type"], 0.02, 4, mode=1))
+    assert len(sim.generateTriangleMesh(0.0, 8)) == 0
+    assert len(sim.generateTriangleMesh(0.05, 0)) == 0
+    assert sim.getTriangleVertices(np.ones((6, 3), np.float32)).shape == (0, 9)
+
+
+def test_mesh_errors():
+    sim = ParticleSimulation(100, 2)
+    with pytest.raises(CellFlowError):
+        sim.generateTriangleMesh(0.02, 4)  # no parameters yet
+    with pytest.raises(CellFlowError):
+        sim.getTriangles()  # nothing built yet
+    st, phys, params = state_of(small_cfg(n=500, side=0.3, T=2))
+    sim = make_sim(st, phys)
+    sim.simulate(params, 0)
+    with pytest.raises(CellFlowError):
+        sim.generateTriangleMesh(float("nan"), 4)
+    with pytest.raises(CellFlowError):
+        sim.generateTriangleMesh(-1.0, 4)

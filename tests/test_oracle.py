@@ -273,6 +273,88 @@ def test_edges_nonpositive_maxconn(orc):
     assert len(orc.edges(pos, np.zeros(2, np.uint32), 0.02, -3, mode=0)) == 0
 
 
+# ---- triangle mesh (ParticleSimulation.cu:279-418, threshold :722-724) -------------------
+
+
+def test_triangles_square_by_hand(orc):
+    """Four same-type particles on a square of side 0.01 (diagonal 0.0141, all within
+    1.2 * 0.02) and one far away: owner 0 fans over {1, 2, 3}, owner 1 over {2, 3}."""
+    pos = np.array([[0, 0, 0], [0.01, 0, 0], [0, 0.01, 0], [0.01, 0.01, 0], [0.5, 0.5, 0.5]],
+                   np.float32)
+    for mode in (0, 1):
+        t = orc.triangles(pos, np.zeros(5, np.uint32), 0.02, 8, mode=mode)
+        assert t.tolist() == [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]
+    # other types are never neighbours; the hysteresis widens the distance by 1.2
+    ty = np.array([0, 1, 0, 0, 0], np.uint32)
+    assert orc.triangles(pos, ty, 0.02, 8, mode=0).tolist() == [[0, 2, 3]]
+    # reach 0.0095 (below the side): nothing; reach 0.0105 (sides, not diagonals): owner 0
+    # has {1, 2}, owners 1 and 2 have one neighbour above them at most
+    assert len(orc.triangles(pos, np.zeros(5, np.uint32), 0.0095 / 1.2, 8, mode=0)) == 0
+    assert orc.triangles(pos, np.zeros(5, np.uint32), 0.0105 / 1.2, 8, mode=0).tolist() == \
+        [[0, 1, 2]]
+
+
+def test_triangles_lower_neighbours_fill_the_list(orc):
+    """Six particles all within reach: owner 3's list holds the first three indices {0, 1, 2}
+    (all below it) at maxConn 3, so it owns no triangle though 4 and 5 are near."""
+    rng = np.random.default_rng(1)
+    pos = (0.5 + 0.002 * rng.random((6, 3))).astype(np.float32)
+    t = orc.triangles(pos, np.zeros(6, np.uint32), 0.05, 3, mode=0)
+    owners = t[:, 0].tolist()
+    assert 3 not in owners and owners.count(0) == 3 and owners.count(1) == 1
+    assert t[owners.index(1)].tolist() == [1, 2, 3]
+
+
+def test_triangles_cap_of_64_in_fan_order(orc):
+    """20 particles within reach: owner 0's 19 neighbours give C(19, 2) = 171 pairs, of
+    which the first 64 in (a, b) order are kept."""
+    rng = np.random.default_rng(2)
+    pos = (0.5 + 0.001 * rng.random((20, 3))).astype(np.float32)
+    t = orc.triangles(pos, np.zeros(20, np.uint32), 0.05, 32, mode=0)
+    own0 = t[t[:, 0] == 0]
+    pairs = [(a, b) for a in range(1, 20) for b in range(a + 1, 20)][:64]
+    assert [tuple(x) for x in own0[:, 1:].tolist()] == pairs
+    counts = np.bincount(t[:, 0], minlength=20)
+    assert counts.tolist() == [min(64, (19 - i) * (18 - i) // 2) for i in range(20)]
+
+
+@pytest.mark.parametrize("maxc", [1, 2, 5, 32, 40])
+def test_triangles_grid_matches_reference_scan(orc, maxc):
+    cfg = dict(configs.load("c1_plain_10k"), n=3000, num_types=2, seed=5,
+               init=dict(kind="blobs", count=5, sd=0.01))
+    st = configs.make_state(cfg)
+    for dist in (0.003, 0.01):
+        a = orc.triangles(st["pos"], st["type"], dist, maxc, mode=0)
+        b = orc.triangles(st["pos"], st["type"], dist, maxc, mode=1)
+        np.testing.assert_array_equal(a, b)
+        if maxc >= 2:
+            assert len(a) > 0
+        # i < a < b, same type, both within reach, owners ascending, at most 64 each
+        assert np.all(a[:, 0] < a[:, 1]) and np.all(a[:, 1] < a[:, 2])
+        assert np.all(np.diff(a[:, 0]) >= 0)
+        ty = st["type"]
+        assert np.all(ty[a[:, 0]] == ty[a[:, 1]]) and np.all(ty[a[:, 0]] == ty[a[:, 2]])
+        p = st["pos"].astype(np.float64)
+        for k in (1, 2):
+            d = np.sqrt(((p[a[:, k]] - p[a[:, 0]]) ** 2).sum(1))
+            assert np.all(d < 1.2 * dist * (1 + 1e-5))
+        if len(a):
+            assert np.bincount(a[:, 0]).max() <= 64
+
+
+def test_triangle_vertices_normal_and_colour(orc):
+    pos = np.array([[0, 0, 0], [0.3, 0, 0], [0, 0.2, 0], [1, 1, 1], [2, 2, 2], [3, 3, 3]],
+                   np.float32)
+    ty = np.array([4, 0, 0, 1, 1, 1], np.uint32)
+    col = np.arange(18, dtype=np.float32).reshape(6, 3) / 10
+    v = orc.triangle_vertices(np.array([[0, 1, 2], [3, 4, 5]], np.int32), pos, ty, 3, col)
+    assert v.shape == (6, 9)
+    np.testing.assert_array_equal(v[:3, :3], pos[:3])
+    np.testing.assert_array_equal(v[:3, 3:6], [[0, 0, 1]] * 3)   # (x edge) x (y edge) = +z
+    np.testing.assert_array_equal(v[:3, 6:], [col[4 % 3]] * 3)  # the owner's type mod T
+    np.testing.assert_array_equal(v[3:, 3:6], 0)                 # collinear: left unnormalised
+
+
 # ---- binning ---------------------------------------------------------------------------
 
 
