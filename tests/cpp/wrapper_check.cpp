@@ -58,6 +58,10 @@ int main(int argc, char** argv) {
       sim.simulate(p);
       sim.generateProximityGraph(verts, vc, 0.02f, 16, colors);
     }
+    // generateTriangleMesh (.cu:689-754) of the last state; the graph stays readable
+    std::vector<float> tverts;
+    int tvc = 0;
+    sim.generateTriangleMesh(tverts, tvc, 0.05f, 8, colors);
     std::vector<cellflow::Particle> fin;
     sim.getParticleData(fin);
     const std::vector<int32_t> counts = sim.getNeighborCounts();
@@ -71,7 +75,9 @@ int main(int argc, char** argv) {
     dump(dir + "/counts.bin", counts.data(), counts.size());
     dump(dir + "/verts.bin", verts.data(), verts.size());
     dump(dir + "/edges.bin", edges.data(), edges.size());
-    std::printf("ok n=%d frames=%d vertices=%d edges=%zu\n", n, frames, vc, edges.size() / 2);
+    dump(dir + "/tverts.bin", tverts.data(), tverts.size());
+    std::printf("ok n=%d frames=%d vertices=%d edges=%zu triangle vertices=%d\n", n, frames,
+                vc, edges.size() / 2, tvc);
 
     // CellFlowWidget::loadPreset / savePreset: load a preset into a fresh simulation, run a
     // few frames, save it back with the original as the template.

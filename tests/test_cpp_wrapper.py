@@ -71,6 +71,12 @@ def test_wrapper_matches_python_binding(tmp_path):
     assert np.array_equal(sim.getNeighborCounts(), load("counts.bin", np.int32))
     assert np.array_equal(sim.getEdges().ravel(), load("edges.bin", np.int32))
     assert np.array_equal(sim.getLineVertices(colors).ravel(), load("verts.bin", np.float32))
+    # the wrapper built its triangle mesh after the last graph (and read the graph after it)
+    sim.generateTriangleMesh(0.05, 8)
+    tverts = load("tverts.bin", np.float32)
+    assert len(tverts) > 0
+    assert np.array_equal(sim.getTriangleVertices(colors).ravel(), tverts)
+    assert np.array_equal(sim.getEdges().ravel(), load("edges.bin", np.int32))
     # the preset loaded, stepped and saved by the C++ wrapper: same file, same table
     assert (tmp_path / "saved.json").read_bytes() == open(PRESET, "rb").read()
     pr = read_preset(PRESET)
