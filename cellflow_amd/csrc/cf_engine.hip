@@ -1014,6 +1014,10 @@ struct Engine {
     EdgeReq none;
     CFR(prepare(none));
     const SimDesc& d = hdesc[0];
+    // the counts are scanned in 32 bits: at most 64 triangles per owner must fit
+    if ((long long)d.n * 64 > 0xFFFFFFFFll)
+      return fail(CF_ERR_ARG, "%d particles: a triangle mesh holds at most %lld owners", d.n,
+                  0xFFFFFFFFll / 64);
     const float hm = 1.2f;                             // :722, a 20% larger radius
     const float d2 = distance * distance * (hm * hm);  // :723, FP32 left to right
     TriArgs ta{};

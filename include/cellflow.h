@@ -189,7 +189,9 @@ int cf_get_line_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float
  * 1.2 * distance (non-periodic), and the fan triangles (i, a, b) over the pairs a < b of
  * that list with i < a, at most 64 per owner.  Triangles are kept in canonical order
  * (owner ascending, then fan order) instead of the reference's atomicAdd order (:353),
- * and read with cf_get_triangles / cf_get_triangle_vertices (the GL VBO is replaced). */
+ * and read with cf_get_triangles / cf_get_triangle_vertices (the GL VBO is replaced).
+ * CF_ERR_ARG for a negative or non-finite distance, or above 67,108,863 particles (the
+ * triangle offsets are 32-bit); CF_ERR_STATE before the first cf_simulate. */
 int cf_build_triangles(cf_sim* sim, float distance, int max_connections);
 /* The last built mesh: `tris` holds capacity triangles of three int32 indices.  Returns
  * CF_ERR_CAPACITY with *count set to the needed number when capacity is too small. */
