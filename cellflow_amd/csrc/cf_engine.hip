@@ -171,6 +171,10 @@ struct Engine {
   int32_t* tris = nullptr;     // triangle mesh: 3 local indices per triangle
   long long tris_cap = 0;      // triangles `tris` holds
   long long tri_count = -1;    // -1: no mesh built
+  float* vtx = nullptr;        // vertex expansion output (line or triangle layout)
+  size_t vtx_cap = 0;          // floats `vtx` holds
+  float* vcol = nullptr;       // the caller's colours on the device
+  size_t vcol_cap = 0;         // floats `vcol` holds
   int edges_maxc = 0;
   uint32_t* slot_of = nullptr;
   // tile sort of the step: per-parity counters (the step with state buffer A counts into
@@ -222,9 +226,10 @@ struct Engine {
     dfree(b.hist); dfree(b.tmp_id); dfree(b.acc); dfree(b.pad);
     dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.eoff);
     dfree(b.edges); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
-    dfree(slot_of); dfree(tickets); dfree(tris);
+    dfree(slot_of); dfree(tickets); dfree(tris); dfree(vtx); dfree(vcol);
     tris_cap = 0;
     tri_count = -1;
+    vtx_cap = vcol_cap = 0;
     for (int i = 0; i < 2; ++i) {
       dfree(tcnt[i]);
       dfree(etcnt[i]);
@@ -557,7 +562,8 @@ struct Engine {
     CFR(build_descs(none, &changed));
     CFR(sort_into(b.tpos, b.tvel, b.tid, cur));
     load_pending = false;
-    edges_count = -1;
+    edges_count = -1;  // a graph or mesh belongs to the state it was built on
+    tri_count = -1;
     return CF_OK;
   }
 
@@ -989,6 +995,7 @@ struct Engine {
   }
 
   int get_edges(int32_t* pairs, long long capacity, long long* count) {
+    CFR(flush());
     if (edges_count < 0) return fail(CF_ERR_STATE, "no proximity graph has been built");
     *count = edges_count;
     if (capacity < edges_count)
@@ -1063,6 +1070,7 @@ struct Engine {
   }
 
   int get_triangles(int32_t* out, long long capacity, long long* count) {
+    CFR(flush());
     if (tri_count < 0) return fail(CF_ERR_STATE, "no triangle mesh has been built");
     *count = tri_count;
     if (capacity < tri_count)
@@ -1074,75 +1082,79 @@ struct Engine {
     return CF_OK;
   }
 
+  // Device buffers of the vertex expansions, kept for the next call (a render loop calls
+  // once per frame): `need` output floats, and the caller's colours uploaded.
+  int vertex_bufs(size_t need, const cf_color* colors, int ncolors) {
+    const size_t ncf = 3 * (size_t)std::max(1, ncolors);
+    if (need > vtx_cap) {
+      HIPR(hipStreamSynchronize(st));
+      dfree(vtx);
+      vtx_cap = 0;
+      CFR(dalloc(&vtx, need));
+      vtx_cap = need;
+    }
+    if (ncf > vcol_cap) {
+      HIPR(hipStreamSynchronize(st));
+      dfree(vcol);
+      vcol_cap = 0;
+      CFR(dalloc(&vcol, ncf));
+      vcol_cap = ncf;
+    }
+    if (ncolors > 0)
+      HIPR(hipMemcpyAsync(vcol, colors, 3 * (size_t)ncolors * sizeof(float),
+                          hipMemcpyHostToDevice, st));
+    HIPR(launch_slot_of(b.id[cur], slot_of, b.sims, 0, sims[0].n, st));
+    return CF_OK;
+  }
+
   int triangle_vertices(const cf_color* colors, int ncolors, float* out, long long cap_floats,
                         long long* vcount) {
+    CFR(flush());
     if (tri_count < 0) return fail(CF_ERR_STATE, "no triangle mesh has been built");
     *vcount = 3 * tri_count;
     const long long need = 27 * tri_count;
     if (cap_floats < need)
       return fail(CF_ERR_CAPACITY, "vertex buffer holds %lld floats, %lld needed", cap_floats, need);
     if (tri_count == 0) return CF_OK;
-    float* dcol = nullptr;
-    float* dout = nullptr;
-    int rc = dalloc(&dcol, 3 * (size_t)std::max(1, ncolors));
-    if (rc == CF_OK) rc = dalloc(&dout, (size_t)need);
-    if (rc == CF_OK && ncolors > 0 &&
-        hipMemcpyAsync(dcol, colors, 3 * (size_t)ncolors * sizeof(float), hipMemcpyHostToDevice,
-                       st) != hipSuccess)
-      rc = fail(CF_ERR_DEVICE, "colour upload failed");
-    if (rc == CF_OK && launch_slot_of(b.id[cur], slot_of, b.sims, 0, sims[0].n, st) != hipSuccess)
-      rc = fail(CF_ERR_DEVICE, "index map failed");
-    if (rc == CF_OK) {
-      TriVertArgs va{};
-      va.tris = tris;
-      va.pos = b.pos[cur];
-      va.slot_of = slot_of;
-      va.colors = dcol;
-      va.out = dout;
-      va.ntris = tri_count;
-      va.ncolors = ncolors;
-      va.T = sims[0].T;
-      if (launch_tri_vertices(va, st) != hipSuccess) rc = fail(CF_ERR_DEVICE, "vertex kernel failed");
-    }
-    if (rc == CF_OK && hipMemcpyAsync(out, dout, (size_t)need * sizeof(float),
-                                      hipMemcpyDeviceToHost, st) != hipSuccess)
-      rc = fail(CF_ERR_DEVICE, "vertex download failed");
-    (void)hipStreamSynchronize(st);
-    dfree(dcol);
-    dfree(dout);
-    return rc;
+    CFR(vertex_bufs((size_t)need, colors, ncolors));
+    TriVertArgs va{};
+    va.tris = tris;
+    va.pos = b.pos[cur];
+    va.slot_of = slot_of;
+    va.colors = vcol;
+    va.out = vtx;
+    va.ntris = tri_count;
+    va.ncolors = ncolors;
+    va.T = sims[0].T;
+    HIPR(launch_tri_vertices(va, st));
+    HIPR(hipMemcpyAsync(out, vtx, (size_t)need * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    return CF_OK;
   }
 
   int line_vertices(const cf_color* colors, int ncolors, float* out, long long cap_floats,
                     long long* vcount) {
+    CFR(flush());
     if (edges_count < 0) return fail(CF_ERR_STATE, "no proximity graph has been built");
     *vcount = 2 * edges_count;
     const long long need = 12 * edges_count;
     if (cap_floats < need)
       return fail(CF_ERR_CAPACITY, "vertex buffer holds %lld floats, %lld needed", cap_floats, need);
     if (edges_count == 0) return CF_OK;
-    float* dcol = nullptr;
-    float* dout = nullptr;
-    CFR(dalloc(&dcol, 3 * (size_t)std::max(1, ncolors)));
-    CFR(dalloc(&dout, (size_t)need));
-    if (ncolors > 0)
-      HIPR(hipMemcpyAsync(dcol, colors, 3 * (size_t)ncolors * sizeof(float), hipMemcpyHostToDevice, st));
-    HIPR(launch_slot_of(b.id[cur], slot_of, b.sims, 0, sims[0].n, st));
+    CFR(vertex_bufs((size_t)need, colors, ncolors));
     LineArgs la{};
     la.edges = b.edges;
     la.pos = b.pos[cur];
     la.slot_of = slot_of;
-    la.colors = dcol;
-    la.out = dout;
+    la.colors = vcol;
+    la.out = vtx;
     la.nedges = edges_count;
     la.ncolors = ncolors;
     la.T = sims[0].T;
     la.p0 = sims[0].p0;
     HIPR(launch_line_vertices(la, st));
-    HIPR(hipMemcpyAsync(out, dout, (size_t)need * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPR(hipMemcpyAsync(out, vtx, (size_t)need * sizeof(float), hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
-    dfree(dcol);
-    dfree(dout);
     return CF_OK;
   }
 

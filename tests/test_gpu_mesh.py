@@ -107,3 +107,29 @@ def test_mesh_errors():
         sim.generateTriangleMesh(float("nan"), 4)
     with pytest.raises(CellFlowError):
         sim.generateTriangleMesh(-1.0, 4)
+
+
+def test_vertex_buffers_reused_and_mesh_dropped_with_a_new_state(orc):
+    """The vertex expansions share device buffers across calls (they grow on demand); a
+    state loaded after a build drops the graph and the mesh, which belong to the old one."""
+    st, phys, params = state_of(small_cfg(n=3000, side=0.5))
+    sim = make_sim(st, phys)
+    sim.simulate(params, 1, edges=(0.03, 4))
+    got = sim.getParticleData()
+    small = sim.generateTriangleMesh(0.02, 4)
+    check_vertices(orc, sim, small, 6)
+    big = sim.generateTriangleMesh(0.05, 12)
+    assert len(big) > 4 * len(small) > 0
+    check_vertices(orc, sim, big, 6)
+    colors = np.eye(3, dtype=np.float32)[np.arange(6) % 3]
+    edges = sim.getEdges()
+    lv = sim.getLineVertices(colors)
+    assert np.array_equal(lv[0::2, :3], got["pos"][edges[:, 0]])
+    assert np.array_equal(lv[1::2, :3], got["pos"][edges[:, 1]])
+    assert np.array_equal(lv[0::2, 3:], colors[got["ptype"][edges[:, 0]] % 6])
+    check_vertices(orc, sim, big, 6)
+    sim.setParticles(got)
+    for read in (sim.getTriangles, sim.getEdges, lambda: sim.getTriangleVertices(colors),
+                 lambda: sim.getLineVertices(colors)):
+        with pytest.raises(CellFlowError):
+            read()
