@@ -175,6 +175,8 @@ struct Engine {
   size_t vtx_cap = 0;          // floats `vtx` holds
   float* vcol = nullptr;       // the caller's colours on the device
   size_t vcol_cap = 0;         // floats `vcol` holds
+  unsigned char* iobuf = nullptr;  // staging of the host get/set calls, kept between calls
+  size_t iobuf_cap = 0;            // bytes `iobuf` holds
   int edges_maxc = 0;
   uint32_t* slot_of = nullptr;
   // tile sort of the step: per-parity counters (the step with state buffer A counts into
@@ -226,10 +228,10 @@ struct Engine {
     dfree(b.hist); dfree(b.tmp_id); dfree(b.acc); dfree(b.pad);
     dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.eoff);
     dfree(b.edges); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
-    dfree(slot_of); dfree(tickets); dfree(tris); dfree(vtx); dfree(vcol);
+    dfree(slot_of); dfree(tickets); dfree(tris); dfree(vtx); dfree(vcol); dfree(iobuf);
     tris_cap = 0;
     tri_count = -1;
-    vtx_cap = vcol_cap = 0;
+    vtx_cap = vcol_cap = iobuf_cap = 0;
     for (int i = 0; i < 2; ++i) {
       dfree(tcnt[i]);
       dfree(etcnt[i]);
@@ -880,6 +882,22 @@ struct Engine {
     return a;
   }
 
+  // Device staging for `count` elements of T, shared by the get/set calls (each ends with
+  // a stream synchronisation, so no two uses overlap); grows on demand.
+  template <class T>
+  int io_stage(T** p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (bytes > iobuf_cap) {
+      HIPR(hipStreamSynchronize(st));
+      dfree(iobuf);
+      iobuf_cap = 0;
+      CFR(dalloc(&iobuf, bytes));
+      iobuf_cap = bytes;
+    }
+    *p = reinterpret_cast<T*>(iobuf);
+    return CF_OK;
+  }
+
   int load_aos(int s, const cf_particle* parts) {
     const SimHost& h = sims[s];
     for (int i = 0; i < h.n; ++i)
@@ -887,13 +905,12 @@ struct Engine {
         return fail(CF_ERR_ARG, "particle %d has type %u >= num_types %d", i, parts[i].ptype, h.T);
     CFR(begin_load());
     AosParticle* d = nullptr;
-    CFR(dalloc(&d, std::max(1, h.n)));
+    CFR(io_stage(&d, (size_t)std::max(1, h.n)));
     HIPR(hipMemcpyAsync(d, parts, (size_t)h.n * sizeof(AosParticle), hipMemcpyHostToDevice, st));
     IoArgs a = io_args(s);
     a.aos = d;
     HIPR(launch_io(0, a, h.n, st));
     HIPR(hipStreamSynchronize(st));
-    dfree(d);
     return CF_OK;
   }
 
@@ -901,13 +918,12 @@ struct Engine {
     CFR(flush());
     const SimHost& h = sims[s];
     AosParticle* d = nullptr;
-    CFR(dalloc(&d, std::max(1, h.n)));
+    CFR(io_stage(&d, (size_t)std::max(1, h.n)));
     IoArgs a = io_args(s);
     a.aos = d;
     HIPR(launch_io(1, a, h.n, st));
     HIPR(hipMemcpyAsync(parts, d, (size_t)h.n * sizeof(AosParticle), hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
-    dfree(d);
     return CF_OK;
   }
 
@@ -915,13 +931,12 @@ struct Engine {
     CFR(flush());
     const SimHost& h = sims[s];
     int32_t* d = nullptr;
-    CFR(dalloc(&d, std::max(1, h.n)));
+    CFR(io_stage(&d, (size_t)std::max(1, h.n)));
     IoArgs a = io_args(s);
     a.counts_out = d;
     HIPR(launch_io(2, a, h.n, st));
     HIPR(hipMemcpyAsync(out, d, (size_t)h.n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
-    dfree(d);
     return CF_OK;
   }
 
@@ -929,14 +944,13 @@ struct Engine {
     CFR(flush());
     const SimHost& h = sims[s];
     int32_t* d = nullptr;
-    CFR(dalloc(&d, std::max(1, h.n)));
+    CFR(io_stage(&d, (size_t)std::max(1, h.n)));
     HIPR(hipMemcpyAsync(d, in, (size_t)h.n * sizeof(int32_t), hipMemcpyHostToDevice, st));
     IoArgs a = io_args(s);
     a.counts_in = d;
     a.tvel = b.vel[cur];
     HIPR(launch_io(3, a, h.n, st));
     HIPR(hipStreamSynchronize(st));
-    dfree(d);
     return CF_OK;
   }
 
@@ -944,7 +958,7 @@ struct Engine {
     CFR(flush());
     const SimHost& h = sims[s];
     uint32_t* d = nullptr;
-    CFR(dalloc(&d, 2 * (size_t)std::max(1, h.n)));
+    CFR(io_stage(&d, 2 * (size_t)std::max(1, h.n)));
     IoArgs a = io_args(s);
     a.cell_out = d;
     a.index_out = d + h.n;
@@ -952,7 +966,6 @@ struct Engine {
     HIPR(hipMemcpyAsync(cells, d, (size_t)h.n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPR(hipMemcpyAsync(index, d + h.n, (size_t)h.n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
-    dfree(d);
     return CF_OK;
   }
 
