@@ -166,6 +166,31 @@ def clustered_leg():
     return out
 
 
+def frame_leg(sim, params, edges, T, frames=200, warm=5):
+    """CellFlowWidget's frame (CellFlowWidget.cpp:410-427, :551-575): one step with the
+    proximity graph, then the particle records and the graph's line vertices copied to host
+    memory.  The host-inclusive rate of the drop-in as the widget drives it; the headline
+    keeps the state resident."""
+    import numpy as np
+    colors = np.random.default_rng(0).random((T, 3)).astype(np.float32)
+    for _ in range(warm):
+        sim.simulate(params, 1, edges=edges)
+        sim.getParticleData()
+        sim.getLineVertices(colors)
+    t0 = time.perf_counter()
+    nbytes = 0
+    for _ in range(frames):
+        sim.simulate(params, 1, edges=edges)
+        parts = sim.getParticleData()
+        verts = sim.getLineVertices(colors)
+        nbytes += parts.nbytes + verts.nbytes
+    dt = time.perf_counter() - t0
+    n = sim.getParticleCount()
+    return {"frames": frames, "ms_per_frame": dt / frames * 1e3, "frames_per_s": frames / dt,
+            "particle_steps_per_s": n * frames / dt, "host_bytes_per_frame": nbytes // frames,
+            "note": "step + graph, records and line vertices to host each frame (PCIe-inclusive)"}
+
+
 def adjoint_leg():
     """Differentiable rollout (20k particles, 200 steps, checkpoints every 20): time of the
     gradient rollout against the forward rollout; central differences over the 37
@@ -223,6 +248,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--no-clustered", action="store_true", help="skip the 1M clustered leg")
+    ap.add_argument("--no-frame", action="store_true", help="skip the host-inclusive frame leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -300,6 +326,9 @@ def main():
                         "trials": [{k: t[k] for k in ("cell_divisor", "force_variant",
                                                       "lanes_per_particle", "block_size",
                                                       "us_per_step")} for t in tuned["trials"]]}}
+    if not args.no_frame:
+        log("[bench] frame loop (host-inclusive)")
+        out["frame_loop"] = frame_leg(sim, params, edges, cfg["num_types"])
     sim.close()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_steps)
