@@ -173,16 +173,19 @@ def frame_leg(sim, params, edges, T, frames=200, warm=5):
     keeps the state resident."""
     import numpy as np
     colors = np.random.default_rng(0).random((T, 3)).astype(np.float32)
+    # host buffers reused from frame to frame, as the widget reuses its vectors
+    pbuf = sim.getParticleData()
+    vbuf = np.zeros(2 * sim.getLineVertices(colors).size + 1024, np.float32)
     for _ in range(warm):
         sim.simulate(params, 1, edges=edges)
-        sim.getParticleData()
-        sim.getLineVertices(colors)
+        sim.getParticleData(out=pbuf)
+        sim.getLineVertices(colors, out=vbuf)
     t0 = time.perf_counter()
     nbytes = 0
     for _ in range(frames):
         sim.simulate(params, 1, edges=edges)
-        parts = sim.getParticleData()
-        verts = sim.getLineVertices(colors)
+        parts = sim.getParticleData(out=pbuf)
+        verts = sim.getLineVertices(colors, out=vbuf)
         nbytes += parts.nbytes + verts.nbytes
     dt = time.perf_counter() - t0
     n = sim.getParticleCount()

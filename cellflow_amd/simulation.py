@@ -76,6 +76,15 @@ def _ip(a, ct=C.c_int32):
     return a.ctypes.data_as(C.POINTER(ct))
 
 
+def _reuse(out, count, dtype):
+    """`out[:count]` when `out` is a contiguous array of `dtype` holding at least `count`
+    elements, else a new zeroed array."""
+    if (out is not None and isinstance(out, np.ndarray) and out.dtype == dtype and out.ndim == 1
+            and out.flags.c_contiguous and out.flags.writeable and len(out) >= count):
+        return out[:count]
+    return np.zeros(count, dtype)
+
+
 class ParticleSimulation:
     """Owner of one simulation's device state (ParticleSimulation.cuh:10-103)."""
 
@@ -123,10 +132,13 @@ class ParticleSimulation:
         check(lib().cf_set_particles(self._h, a.ctypes.data_as(C.POINTER(_abi.cf_particle)),
                                      len(a)))
 
-    def getParticleData(self) -> np.ndarray:
-        """getParticleData (ParticleSimulation.cu:558-562): records in index order."""
+    def getParticleData(self, out: np.ndarray = None) -> np.ndarray:
+        """getParticleData (ParticleSimulation.cu:558-562): records in index order.  `out`, a
+        contiguous array of PARTICLE_DTYPE with at least the particle count of records, is
+        filled and returned (sliced to the count) instead of a new array, as the reference
+        reuses the caller's vector."""
         n = self.getParticleCount()
-        out = np.zeros(n, PARTICLE_DTYPE)
+        out = _reuse(out, n, PARTICLE_DTYPE)
         check(lib().cf_get_particles(self._h, out.ctypes.data_as(C.POINTER(_abi.cf_particle)), n))
         return out
 
@@ -339,15 +351,16 @@ class ParticleSimulation:
                                              len(col), _fp(out), 9 * vc.value, C.byref(vc)))
         return out[: 9 * vc.value].reshape(-1, 9)
 
-    def getLineVertices(self, colors) -> np.ndarray:
-        """The reference's GL line-vertex layout (ParticleSimulation.cu:253-275)."""
+    def getLineVertices(self, colors, out: np.ndarray = None) -> np.ndarray:
+        """The reference's GL line-vertex layout (ParticleSimulation.cu:253-275).  `out`, a
+        contiguous float32 array, is reused when it holds enough floats."""
         col = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
         vc = C.c_longlong()
         rc = lib().cf_get_line_vertices(self._h, col.ctypes.data_as(C.POINTER(_abi.cf_color)),
                                         len(col), None, 0, C.byref(vc))
         if rc not in (_abi.CF_OK, _abi.CF_ERR_CAPACITY):
             check(rc)
-        out = np.zeros(max(6 * vc.value, 1), np.float32)
+        out = _reuse(None if out is None else out.reshape(-1), max(6 * vc.value, 1), np.float32)
         check(lib().cf_get_line_vertices(self._h, col.ctypes.data_as(C.POINTER(_abi.cf_color)),
                                          len(col), _fp(out), 6 * vc.value, C.byref(vc)))
         return out[: 6 * vc.value].reshape(-1, 6)

@@ -397,3 +397,23 @@ def test_profile_steps_times_each_kernel_and_steps_the_state():
     t0 = c.profileSteps(params, 3)
     assert list(t0) == ["force", "scan", "fill", "rank_scatter"]
     assert all(v > 0.0 for v in t0.values())
+
+
+def test_gets_into_caller_buffers():
+    """getParticleData / getLineVertices fill a caller's array when it is large enough (as
+    the reference fills the caller's vector) and give the same bytes as a fresh array."""
+    cfg = small_cfg(n=3000, side=0.4, T=3)
+    st, phys, params = state_of(cfg)
+    sim = make_sim(st, phys)
+    sim.simulate(params, 1, edges=(0.03, 4))
+    colors = np.eye(3, dtype=np.float32)
+    parts, lines = sim.getParticleData(), sim.getLineVertices(colors)
+    pbuf = np.zeros(len(parts) + 7, parts.dtype)
+    vbuf = np.zeros(lines.size + 5, np.float32)
+    p2, l2 = sim.getParticleData(out=pbuf), sim.getLineVertices(colors, out=vbuf)
+    assert np.shares_memory(p2, pbuf) and np.shares_memory(l2, vbuf)
+    assert p2.tobytes() == parts.tobytes() and np.array_equal(l2, lines)
+    small = np.zeros(10, parts.dtype)  # too small: a new array, the buffer untouched
+    p3 = sim.getParticleData(out=small)
+    assert not np.shares_memory(p3, small) and p3.tobytes() == parts.tobytes()
+    assert not small.tobytes().strip(b"\0")
