@@ -197,6 +197,8 @@ struct Engine {
   std::vector<uint32_t> htile_sim;
   cf_tuning tun{1, 0, 16, 256, 32, 1};
   uint32_t* hsmall = nullptr;  // pinned readback words
+  unsigned char* hpin = nullptr;  // pinned staging of the large readbacks, kept between calls
+  size_t hpin_cap = 0;            // bytes `hpin` holds
   // Captured step graphs.  A graph bakes kernel arguments (buffer pointers, sizes, tuning),
   // so the cache is keyed on them and dropped whenever a buffer is reallocated.
   struct GraphEntry {
@@ -241,6 +243,9 @@ struct Engine {
     for (auto& f : flags) dfree(f);
     if (hsmall) (void)hipHostFree(hsmall);
     hsmall = nullptr;
+    if (hpin) (void)hipHostFree(hpin);
+    hpin = nullptr;
+    hpin_cap = 0;
     if (st) (void)hipStreamDestroy(st);
     st = nullptr;
   }
@@ -898,6 +903,25 @@ struct Engine {
     return CF_OK;
   }
 
+  // Device-to-host copy of `bytes` into the caller's memory through the pinned staging
+  // buffer (a DMA into pinned memory, then one host copy); synchronous.  Splitting it into
+  // chunks whose host copies overlap the next DMA measured no faster.
+  int to_host(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return CF_OK;
+    if (bytes > hpin_cap) {
+      HIPR(hipStreamSynchronize(st));
+      if (hpin) (void)hipHostFree(hpin);
+      hpin = nullptr;
+      hpin_cap = 0;
+      HIPR(hipHostMalloc((void**)&hpin, bytes, hipHostMallocDefault));
+      hpin_cap = bytes;
+    }
+    HIPR(hipMemcpyAsync(hpin, src, bytes, hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    std::memcpy(dst, hpin, bytes);
+    return CF_OK;
+  }
+
   int load_aos(int s, const cf_particle* parts) {
     const SimHost& h = sims[s];
     for (int i = 0; i < h.n; ++i)
@@ -922,9 +946,7 @@ struct Engine {
     IoArgs a = io_args(s);
     a.aos = d;
     HIPR(launch_io(1, a, h.n, st));
-    HIPR(hipMemcpyAsync(parts, d, (size_t)h.n * sizeof(AosParticle), hipMemcpyDeviceToHost, st));
-    HIPR(hipStreamSynchronize(st));
-    return CF_OK;
+    return to_host(parts, d, (size_t)h.n * sizeof(AosParticle));
   }
 
   int get_counts(int s, int32_t* out) {
@@ -1140,9 +1162,7 @@ struct Engine {
     va.ncolors = ncolors;
     va.T = sims[0].T;
     HIPR(launch_tri_vertices(va, st));
-    HIPR(hipMemcpyAsync(out, vtx, (size_t)need * sizeof(float), hipMemcpyDeviceToHost, st));
-    HIPR(hipStreamSynchronize(st));
-    return CF_OK;
+    return to_host(out, vtx, (size_t)need * sizeof(float));
   }
 
   int line_vertices(const cf_color* colors, int ncolors, float* out, long long cap_floats,
@@ -1166,9 +1186,7 @@ struct Engine {
     la.T = sims[0].T;
     la.p0 = sims[0].p0;
     HIPR(launch_line_vertices(la, st));
-    HIPR(hipMemcpyAsync(out, vtx, (size_t)need * sizeof(float), hipMemcpyDeviceToHost, st));
-    HIPR(hipStreamSynchronize(st));
-    return CF_OK;
+    return to_host(out, vtx, (size_t)need * sizeof(float));
   }
 
   int profile(int nsteps, const EdgeReq& er, double* ms_out, int max_k, int* nk, char* names,
