@@ -194,6 +194,39 @@ def frame_leg(sim, params, edges, T, frames=200, warm=5):
             "note": "step + graph, records and line vertices to host each frame (PCIe-inclusive)"}
 
 
+def mesh_init_leg(sim, T, reps=20):
+    """Per-call times of the triangle mesh (generateTriangleMesh, ParticleSimulation.cu:
+    689-754) on the current 100k state and of its vertex expansion, and of the seeded
+    initialisation of 100k particles (the replacement for ParticleSimulation.cu:21-66)."""
+    import numpy as np
+    from cellflow_amd import ParticleSimulation
+    colors = np.random.default_rng(0).random((T, 3)).astype(np.float32)
+    tris = sim.generateTriangleMesh(0.02, 16)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.generateTriangleMesh(0.02, 16)
+    mesh_ms = (time.perf_counter() - t0) / reps * 1e3
+    sim.getTriangleVertices(colors)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        v = sim.getTriangleVertices(colors)
+    vert_ms = (time.perf_counter() - t0) / reps * 1e3
+    n = sim.getParticleCount()
+    fresh = ParticleSimulation(n, T)
+    fresh.initializeParticles(1, (2.0, 2.0, 2.0), 2.0)
+    t0 = time.perf_counter()
+    for k in range(reps):
+        fresh.initializeParticles(k, (2.0, 2.0, 2.0), 2.0)
+    init_ms = (time.perf_counter() - t0) / reps * 1e3
+    fresh.close()
+    return {"triangle_mesh": {"distance": 0.02, "max_connections": 16, "triangles": len(tris),
+                              "ms_per_build": mesh_ms, "vertices_ms": vert_ms,
+                              "vertex_bytes": int(v.nbytes)},
+            "seeded_init": {"particles": n, "ms_per_call": init_ms},
+            "note": "per call, host-synchronous: triangles stay on the device until read; "
+                    "the vertex expansion includes the copy to host"}
+
+
 def adjoint_leg():
     """Differentiable rollout (20k particles, 200 steps, checkpoints every 20): time of the
     gradient rollout against the forward rollout; central differences over the 37
@@ -252,6 +285,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--no-clustered", action="store_true", help="skip the 1M clustered leg")
     ap.add_argument("--no-frame", action="store_true", help="skip the host-inclusive frame leg")
+    ap.add_argument("--no-mesh", action="store_true", help="skip the triangle-mesh/init leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -332,6 +366,9 @@ def main():
     if not args.no_frame:
         log("[bench] frame loop (host-inclusive)")
         out["frame_loop"] = frame_leg(sim, params, edges, cfg["num_types"])
+    if not args.no_mesh:
+        log("[bench] triangle mesh and seeded init")
+        out["mesh_and_init"] = mesh_init_leg(sim, cfg["num_types"])
     sim.close()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_steps)
