@@ -113,6 +113,8 @@ SIGNATURES = {
     "cf_build_triangles": (_i, [_vp, _f, _i]),
     "cf_get_triangles": (_i, [_vp, P(C.c_int32), _ll, P(_ll)]),
     "cf_get_triangle_vertices": (_i, [_vp, P(cf_color), _i, P(_f), _ll, P(_ll)]),
+    "cf_write_line_vertices_device": (_i, [_vp, P(cf_color), _i, _vp, _ll, P(_ll)]),
+    "cf_write_triangle_vertices_device": (_i, [_vp, P(cf_color), _i, _vp, _ll, P(_ll)]),
     "cf_get_grid": (_i, [_vp, P(_i), P(_i)]),
     "cf_get_sort_order": (_i, [_vp, P(C.c_uint32), P(C.c_uint32), _i]),
     "cf_profile_steps": (_i, [_vp, P(cf_params), _i, P(cf_edge_opts), P(C.c_double), _i, P(_i),

@@ -854,6 +854,7 @@ struct Engine {
     const bool fuse = fusable(er);
     const int cap = 2 * er.maxc;
     edges_count = -1;
+    if (nsteps > 0) tri_count = -1;  // a mesh belongs to the state it was built on
     if (er.on && !fuse) {
       for (int k = 0; k < nsteps; ++k) {
         CFR(step(false, cap));
@@ -1118,7 +1119,8 @@ struct Engine {
   }
 
   // Device buffers of the vertex expansions, kept for the next call (a render loop calls
-  // once per frame): `need` output floats, and the caller's colours uploaded.
+  // once per frame): `need` output floats of staging (0: the caller's device buffer takes
+  // the output), and the caller's colours uploaded.
   int vertex_bufs(size_t need, const cf_color* colors, int ncolors) {
     const size_t ncf = 3 * (size_t)std::max(1, ncolors);
     if (need > vtx_cap) {
@@ -1142,8 +1144,48 @@ struct Engine {
     return CF_OK;
   }
 
-  int triangle_vertices(const cf_color* colors, int ncolors, float* out, long long cap_floats,
-                        long long* vcount) {
+  // `out` must be memory this engine's device can write: device or managed memory of that
+  // device, or pinned host memory (hipHostMalloc).  Anything else (pageable host memory, a
+  // pointer HIP does not know, another device's memory) is CF_ERR_ARG, checked on the
+  // host before any kernel runs.
+  int check_device_out(const void* out) {
+    if (!out) return fail(CF_ERR_ARG, "null output pointer");
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, out) != hipSuccess) {
+      (void)hipGetLastError();  // the query's error is not the engine's
+      return fail(CF_ERR_ARG, "the output is not memory the HIP runtime knows");
+    }
+    const bool dev = at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+    const bool pinned = at.type == hipMemoryTypeHost && at.devicePointer != nullptr;
+    if (!dev && !pinned)
+      return fail(CF_ERR_ARG, "the output is not device, managed or pinned host memory");
+    if (dev && at.device != device)
+      return fail(CF_ERR_ARG, "the output lives on device %d, the engine on device %d",
+                  at.device, device);
+    return CF_OK;
+  }
+
+  // A device output must hold `bytes` from `out` to the end of its allocation, where the
+  // runtime can tell (it cannot for every kind of memory; then the caller's capacity is
+  // taken as given).
+  int check_device_room(const void* out, size_t bytes) {
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)out) != hipSuccess) {
+      (void)hipGetLastError();
+      return CF_OK;
+    }
+    const size_t room = (size_t)((const char*)base + size - (const char*)out);
+    if (room < bytes)
+      return fail(CF_ERR_CAPACITY, "the output allocation holds %zu bytes from the pointer, "
+                  "%zu needed", room, bytes);
+    return CF_OK;
+  }
+
+  // The triangle-vertex expansion into `dev_out` (device-writable, see check_device_out)
+  // when non-null, else into `out` (host memory) through the engine's staging buffer.
+  int triangle_vertices(const cf_color* colors, int ncolors, float* out, float* dev_out,
+                        long long cap_floats, long long* vcount) {
     CFR(flush());
     if (tri_count < 0) return fail(CF_ERR_STATE, "no triangle mesh has been built");
     *vcount = 3 * tri_count;
@@ -1151,22 +1193,31 @@ struct Engine {
     if (cap_floats < need)
       return fail(CF_ERR_CAPACITY, "vertex buffer holds %lld floats, %lld needed", cap_floats, need);
     if (tri_count == 0) return CF_OK;
-    CFR(vertex_bufs((size_t)need, colors, ncolors));
+    if (dev_out) {
+      CFR(check_device_out(dev_out));
+      CFR(check_device_room(dev_out, (size_t)need * sizeof(float)));
+    }
+    CFR(vertex_bufs(dev_out ? 0 : (size_t)need, colors, ncolors));
     TriVertArgs va{};
     va.tris = tris;
     va.pos = b.pos[cur];
     va.slot_of = slot_of;
     va.colors = vcol;
-    va.out = vtx;
+    va.out = dev_out ? dev_out : vtx;
     va.ntris = tri_count;
     va.ncolors = ncolors;
     va.T = sims[0].T;
     HIPR(launch_tri_vertices(va, st));
+    if (dev_out) {
+      HIPR(hipStreamSynchronize(st));
+      return CF_OK;
+    }
     return to_host(out, vtx, (size_t)need * sizeof(float));
   }
 
-  int line_vertices(const cf_color* colors, int ncolors, float* out, long long cap_floats,
-                    long long* vcount) {
+  // The line-vertex expansion, into `dev_out` or `out` as triangle_vertices.
+  int line_vertices(const cf_color* colors, int ncolors, float* out, float* dev_out,
+                    long long cap_floats, long long* vcount) {
     CFR(flush());
     if (edges_count < 0) return fail(CF_ERR_STATE, "no proximity graph has been built");
     *vcount = 2 * edges_count;
@@ -1174,18 +1225,26 @@ struct Engine {
     if (cap_floats < need)
       return fail(CF_ERR_CAPACITY, "vertex buffer holds %lld floats, %lld needed", cap_floats, need);
     if (edges_count == 0) return CF_OK;
-    CFR(vertex_bufs((size_t)need, colors, ncolors));
+    if (dev_out) {
+      CFR(check_device_out(dev_out));
+      CFR(check_device_room(dev_out, (size_t)need * sizeof(float)));
+    }
+    CFR(vertex_bufs(dev_out ? 0 : (size_t)need, colors, ncolors));
     LineArgs la{};
     la.edges = b.edges;
     la.pos = b.pos[cur];
     la.slot_of = slot_of;
     la.colors = vcol;
-    la.out = vtx;
+    la.out = dev_out ? dev_out : vtx;
     la.nedges = edges_count;
     la.ncolors = ncolors;
     la.T = sims[0].T;
     la.p0 = sims[0].p0;
     HIPR(launch_line_vertices(la, st));
+    if (dev_out) {
+      HIPR(hipStreamSynchronize(st));
+      return CF_OK;
+    }
     return to_host(out, vtx, (size_t)need * sizeof(float));
   }
 
@@ -1197,6 +1256,10 @@ struct Engine {
     CFR(prepare(er));
     const bool fuse = fusable(er);
     const int cap = 2 * er.maxc;
+    if (nsteps > 0) {  // the profiled steps advance the state
+      edges_count = -1;
+      tri_count = -1;
+    }
     // the step's kernels (ns of them), then the standalone edge pass's three
     const bool tiles = tun.sort_variant == 1;
     const int ns = tiles ? 2 : 4;
@@ -1439,6 +1502,9 @@ struct Engine {
     if (!h.have_prm) return fail(CF_ERR_STATE, "parameters not set");
     EdgeReq none;
     CFR(prepare(none));
+    // the rollout advances the state: a graph or mesh of the old state no longer applies
+    edges_count = -1;
+    tri_count = -1;
     const int n = h.n, T = h.T;
     const long stride = ((long)n + 127) / 128 * 128;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
@@ -2234,7 +2300,17 @@ int cf_get_triangle_vertices(cf_sim* sim, const cf_color* colors, int ncolors, f
   CFR(check(sim));
   if (!vertex_count || (!out && capacity_floats > 0) || (!colors && ncolors > 0))
     return cf::fail(CF_ERR_ARG, "null argument");
-  return sim->e->triangle_vertices(colors, ncolors, out, capacity_floats, vertex_count);
+  return sim->e->triangle_vertices(colors, ncolors, out, nullptr, capacity_floats, vertex_count);
+}
+
+int cf_write_triangle_vertices_device(cf_sim* sim, const cf_color* colors, int ncolors,
+                                      float* device_out, long long capacity_floats,
+                                      long long* vertex_count) {
+  CFR(check(sim));
+  if (!vertex_count || (!device_out && capacity_floats > 0) || (!colors && ncolors > 0))
+    return cf::fail(CF_ERR_ARG, "null argument");
+  return sim->e->triangle_vertices(colors, ncolors, nullptr, device_out, capacity_floats,
+                                   vertex_count);
 }
 
 int cf_get_line_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float* out,
@@ -2242,7 +2318,17 @@ int cf_get_line_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float
   CFR(check(sim));
   if (!vertex_count || (!out && capacity_floats > 0) || (!colors && ncolors > 0))
     return cf::fail(CF_ERR_ARG, "null argument");
-  return sim->e->line_vertices(colors, ncolors, out, capacity_floats, vertex_count);
+  return sim->e->line_vertices(colors, ncolors, out, nullptr, capacity_floats, vertex_count);
+}
+
+int cf_write_line_vertices_device(cf_sim* sim, const cf_color* colors, int ncolors,
+                                  float* device_out, long long capacity_floats,
+                                  long long* vertex_count) {
+  CFR(check(sim));
+  if (!vertex_count || (!device_out && capacity_floats > 0) || (!colors && ncolors > 0))
+    return cf::fail(CF_ERR_ARG, "null argument");
+  return sim->e->line_vertices(colors, ncolors, nullptr, device_out, capacity_floats,
+                               vertex_count);
 }
 
 int cf_get_grid(cf_sim* sim, int dims[3], int* reach) {

@@ -6,9 +6,14 @@
 // integral numbers below 2^53 without a fraction, other numbers in the shortest form that
 // reads back to the same double (Python repr's layout, which is also what the reference's
 // files hold), a newline at the end.
+//
+// Numbers are converted with std::from_chars / std::to_chars, which do not depend on the
+// C locale: a host application that sets LC_NUMERIC to a comma-decimal locale (a Qt
+// application does, from the environment) still reads "0.25" as 0.25 and writes '.'.
 #pragma once
 
 #include <algorithm>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +24,22 @@
 
 namespace cf {
 namespace json {
+
+// A JSON number's text [b, e) as a double, locale-free.  Magnitudes beyond the double
+// range are rejected; ones below it read as a zero of the text's sign.
+static inline bool parse_double(const char* b, const char* e, double& v) {
+  const std::from_chars_result r = std::from_chars(b, e, v);
+  if (r.ptr != e) return false;
+  if (r.ec == std::errc()) return true;
+  if (r.ec != std::errc::result_out_of_range) return false;
+  const char* x = b;
+  while (x < e && *x != 'e' && *x != 'E') ++x;
+  if (x + 1 < e && x[1] == '-') {  // underflow
+    v = (*b == '-') ? -0.0 : 0.0;
+    return true;
+  }
+  return false;
+}
 
 struct JVal {
   enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
@@ -158,8 +179,7 @@ class Reader {
       if (q >= end_ || *q < '0' || *q > '9') return false;
       while (q < end_ && *q >= '0' && *q <= '9') ++q;
     }
-    const std::string text(p_, q);
-    v = std::strtod(text.c_str(), nullptr);
+    if (!parse_double(p_, q, v)) return false;
     p_ = q;
     return true;
   }
@@ -270,45 +290,23 @@ static inline std::string escape(const std::string& s) {
 }
 
 // Shortest decimal significand `digits` (no trailing zeros) and exponent `x` (value =
-// d.ddd... * 10^x) that read back to |v|.  At each length the correctly rounded candidate
-// is tried first; when it misses, its neighbour on the other side of v is the only other
-// candidate of that length that can hit, and it is the nearest one that does.
+// d.ddd... * 10^x) that read back to |v|: std::to_chars without a precision gives exactly
+// that (the shortest round-trip form, correctly rounded), in scientific layout.
 static inline void shortest(double v, std::string& digits, int& x) {
-  const double a = std::fabs(v);
   char b[64];
-  for (int p = 1; p <= 17; ++p) {
-    std::snprintf(b, sizeof b, "%.*e", p - 1, a);
-    const char* e = std::strchr(b, 'e');
-    std::string d;
-    for (const char* c = b; c < e; ++c)
-      if (*c != '.') d += *c;
-    int xe = std::atoi(e + 1);
-    double back = std::strtod(b, nullptr);
-    if (back != a && p < 17) {
-      unsigned long long m = std::strtoull(d.c_str(), nullptr, 10);
-      m = (back < a) ? m + 1 : m - 1;
-      std::string nd = std::to_string(m);
-      int nx = xe;
-      if ((int)nd.size() > p) {  // 99..9 + 1: one more digit, the exponent grows
-        nd.resize(p);
-        nx = xe + 1;
-      } else if ((int)nd.size() < p) {  // 10..0 - 1: 9..9 one digit shorter
-        nx = xe - 1;
-      }
-      std::string cand = nd.substr(0, 1) + "." + nd.substr(1) + "e" + std::to_string(nx);
-      if (std::strtod(cand.c_str(), nullptr) == a) {
-        d = nd;
-        xe = nx;
-        back = a;
-      }
-    }
-    if (back == a || p == 17) {
-      while (d.size() > 1 && d.back() == '0') d.pop_back();
-      digits = d;
-      x = xe;
-      return;
-    }
-  }
+  const std::to_chars_result r =
+      std::to_chars(b, b + sizeof b, std::fabs(v), std::chars_format::scientific);
+  const char* e = std::find(b, r.ptr, 'e');
+  std::string d;
+  for (const char* c = b; c < e; ++c)
+    if (*c != '.') d += *c;
+  int xe = 0;
+  const char* xs = e + 1;
+  if (xs < r.ptr && *xs == '+') ++xs;  // from_chars takes no leading '+'
+  std::from_chars(xs, r.ptr, xe);
+  while (d.size() > 1 && d.back() == '0') d.pop_back();
+  digits = d;
+  x = xe;
 }
 
 // Number text: integral values below 2^53 as integers, negative zero as -0.0 (it keeps its
@@ -318,8 +316,8 @@ static inline std::string number(double v) {
   char b[40];
   if (v == 0.0 && std::signbit(v)) return "-0.0";
   if (std::isfinite(v) && v == std::floor(v) && std::fabs(v) < 9007199254740992.0) {
-    std::snprintf(b, sizeof b, "%.0f", v);
-    return b;
+    const std::to_chars_result r = std::to_chars(b, b + sizeof b, v, std::chars_format::fixed, 0);
+    return std::string(b, r.ptr);
   }
   if (!std::isfinite(v)) return "null";  // not representable in JSON
   std::string d;

@@ -18,6 +18,7 @@
 
 #include <cerrno>
 #include <cmath>
+#include <charconv>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -138,13 +139,19 @@ static inline bool write_file(const char* path, const std::map<std::string, Entr
   bool first = true;
   for (const auto& kv : ents) {
     const cf_tuning& t = kv.second.tun;
+    // %d does not depend on the locale; the fraction is written with to_chars, which does
+    // not either (a comma-decimal LC_NUMERIC would make "%.4f" invalid JSON)
+    char us[64];
+    const double usv = std::isfinite(kv.second.us) ? kv.second.us : 0.0;  // JSON has no NaN
+    const std::to_chars_result r =
+        std::to_chars(us, us + sizeof us - 1, usv, std::chars_format::fixed, 4);
+    *r.ptr = '\0';
     std::fprintf(f,
                  "%s\n    \"%s\": {\"cell_divisor\": %d, \"force_variant\": %d, "
                  "\"lanes_per_particle\": %d, \"block_size\": %d, \"steps_per_graph\": %d, "
-                 "\"sort_variant\": %d, \"us_per_step\": %.4f}",
+                 "\"sort_variant\": %d, \"us_per_step\": %s}",
                  first ? "" : ",", escape(kv.first).c_str(), t.cell_divisor, t.force_variant,
-                 t.lanes_per_particle, t.block_size, t.steps_per_graph, t.sort_variant,
-                 kv.second.us);
+                 t.lanes_per_particle, t.block_size, t.steps_per_graph, t.sort_variant, us);
     first = false;
   }
   std::fprintf(f, "%s}\n}\n", first ? "" : "\n  ");

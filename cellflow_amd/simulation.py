@@ -77,12 +77,19 @@ def _ip(a, ct=C.c_int32):
 
 
 def _reuse(out, count, dtype):
-    """`out[:count]` when `out` is a contiguous array of `dtype` holding at least `count`
-    elements, else a new zeroed array."""
-    if (out is not None and isinstance(out, np.ndarray) and out.dtype == dtype and out.ndim == 1
-            and out.flags.c_contiguous and out.flags.writeable and len(out) >= count):
-        return out[:count]
-    return np.zeros(count, dtype)
+    """A flat view of `out`'s first `count` elements, to be filled in place, or a new zeroed
+    array when `out` is None.  `out` must be a writeable C-contiguous array of `dtype` (any
+    shape) holding at least `count` elements: anything else raises ValueError instead of
+    being silently replaced by a copy the caller never sees."""
+    if out is None:
+        return np.zeros(count, dtype)
+    if not isinstance(out, np.ndarray) or out.dtype != dtype:
+        raise ValueError(f"out must be a numpy array of dtype {dtype}")
+    if not (out.flags.c_contiguous and out.flags.writeable):
+        raise ValueError("out must be a writeable C-contiguous array")
+    if out.size < count:
+        raise ValueError(f"out holds {out.size} elements, {count} needed")
+    return out.reshape(-1)[:count]
 
 
 class ParticleSimulation:
@@ -336,16 +343,17 @@ class ParticleSimulation:
         check(lib().cf_get_triangles(self._h, _ip(out), cnt.value, C.byref(cnt)))
         return out[: cnt.value]
 
-    def getTriangleVertices(self, colors) -> np.ndarray:
+    def getTriangleVertices(self, colors, out: np.ndarray = None) -> np.ndarray:
         """The reference's triangle-vertex layout (ParticleSimulation.cu:362-418): (3T, 9)
-        rows of position, flat normal and colour."""
+        rows of position, flat normal and colour.  `out`, a writeable C-contiguous float32
+        array, is filled in place when given (ValueError if it is too small)."""
         col = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
         vc = C.c_longlong()
         rc = lib().cf_get_triangle_vertices(self._h, col.ctypes.data_as(C.POINTER(_abi.cf_color)),
                                             len(col), None, 0, C.byref(vc))
         if rc not in (_abi.CF_OK, _abi.CF_ERR_CAPACITY):
             check(rc)
-        out = np.zeros(max(9 * vc.value, 1), np.float32)
+        out = _reuse(out, max(9 * vc.value, 1), np.float32)
         check(lib().cf_get_triangle_vertices(self._h,
                                              col.ctypes.data_as(C.POINTER(_abi.cf_color)),
                                              len(col), _fp(out), 9 * vc.value, C.byref(vc)))
@@ -353,17 +361,40 @@ class ParticleSimulation:
 
     def getLineVertices(self, colors, out: np.ndarray = None) -> np.ndarray:
         """The reference's GL line-vertex layout (ParticleSimulation.cu:253-275).  `out`, a
-        contiguous float32 array, is reused when it holds enough floats."""
+        writeable C-contiguous float32 array, is filled in place when given (ValueError if
+        it is too small)."""
         col = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
         vc = C.c_longlong()
         rc = lib().cf_get_line_vertices(self._h, col.ctypes.data_as(C.POINTER(_abi.cf_color)),
                                         len(col), None, 0, C.byref(vc))
         if rc not in (_abi.CF_OK, _abi.CF_ERR_CAPACITY):
             check(rc)
-        out = _reuse(None if out is None else out.reshape(-1), max(6 * vc.value, 1), np.float32)
+        out = _reuse(out, max(6 * vc.value, 1), np.float32)
         check(lib().cf_get_line_vertices(self._h, col.ctypes.data_as(C.POINTER(_abi.cf_color)),
                                          len(col), _fp(out), 6 * vc.value, C.byref(vc)))
         return out[: 6 * vc.value].reshape(-1, 6)
+
+    def _write_device(self, fn, colors, device_ptr: int, capacity_floats: int) -> int:
+        col = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
+        vc = C.c_longlong()
+        check(fn(self._h, col.ctypes.data_as(C.POINTER(_abi.cf_color)), len(col),
+                 C.c_void_p(int(device_ptr)), int(capacity_floats), C.byref(vc)))
+        return vc.value
+
+    def writeLineVerticesDevice(self, colors, device_ptr: int, capacity_floats: int) -> int:
+        """generateProximityGraph's VBO write (ParticleSimulation.cu:253-275, :633-643): the
+        line vertices of the last graph written by the GPU into caller-owned device memory
+        at `device_ptr` (e.g. a mapped GL buffer, or a torch tensor's data_ptr()), holding
+        `capacity_floats` floats.  Returns the vertex count (2 per edge)."""
+        return self._write_device(lib().cf_write_line_vertices_device, colors, device_ptr,
+                                  capacity_floats)
+
+    def writeTriangleVerticesDevice(self, colors, device_ptr: int, capacity_floats: int) -> int:
+        """generateTriangleMesh's VBO write (ParticleSimulation.cu:362-418): the triangle
+        vertices of the last mesh into caller-owned device memory.  Returns the vertex
+        count (3 per triangle)."""
+        return self._write_device(lib().cf_write_triangle_vertices_device, colors, device_ptr,
+                                  capacity_floats)
 
     # -- differentiable rollout ------------------------------------------------------------
     def pairHistogram(self, bins: int, r_max: float) -> np.ndarray:

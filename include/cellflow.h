@@ -3,7 +3,8 @@
  * This is the drop-in boundary for the reference's simulation-step path, the C++ class
  * `ParticleSimulation` (cuda-native/include/ParticleSimulation.cuh:10-103, implemented in
  * cuda-native/src/ParticleSimulation.cu:426-754).  Each entry point below names the
- * reference member it replaces.  Plain C types and host pointers only; every call returns
+ * reference member it replaces.  Plain C types and pointers to host memory (the
+ * *_device entry points alone take caller-owned device memory); every call returns
  * an int status (CF_OK on success) and never exits the process (the reference's
  * CUDA_CHECK calls exit(1), ParticleSimulation.cu:10-18).  A handle is not thread-safe;
  * separate handles are independent.
@@ -202,6 +203,23 @@ int cf_get_triangles(cf_sim* sim, int32_t* tris, long long capacity, long long* 
  * generateTriangleMesh returns through outVertexCount). */
 int cf_get_triangle_vertices(cf_sim* sim, const cf_color* colors, int ncolors, float* out,
                              long long capacity_floats, long long* vertex_count);
+
+/* The GL buffer writes of generateProximityGraph and generateTriangleMesh (.cu:633-643 map
+ * the VBO, the kernels write vertices into it at :253-275 and :362-418, :681-686 unmap):
+ * the same bytes as cf_get_line_vertices / cf_get_triangle_vertices, written by the GPU
+ * straight into caller-owned device memory, for instance the pointer
+ * hipGraphicsResourceGetMappedPointer returns for a GL buffer the caller registered and
+ * mapped.  No host copy is made; the call returns once the writes are done.  `device_out`
+ * must be device or managed memory of the engine's device, or pinned host memory
+ * (CF_ERR_ARG otherwise, checked before any kernel runs); CF_ERR_CAPACITY when
+ * capacity_floats, or the allocation behind the pointer where the runtime can tell, is too
+ * small (*vertex_count is still set).  CF_ERR_STATE when no graph / mesh has been built. */
+int cf_write_line_vertices_device(cf_sim* sim, const cf_color* colors, int ncolors,
+                                  float* device_out, long long capacity_floats,
+                                  long long* vertex_count);
+int cf_write_triangle_vertices_device(cf_sim* sim, const cf_color* colors, int ncolors,
+                                      float* device_out, long long capacity_floats,
+                                      long long* vertex_count);
 
 /* ---- introspection (parity checks) -------------------------------------------------- */
 

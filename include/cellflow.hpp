@@ -8,7 +8,10 @@
 //   * errors throw cellflow::Error (status code + message) instead of exit(1)
 //     (ParticleSimulation.cu:10-18);
 //   * generateProximityGraph writes the reference's line vertices (2 per edge, each
-//     x y z r g b) into a host vector instead of mapping a GL VBO; the caller uploads them;
+//     x y z r g b) into a host vector, or (the overload taking a float*) straight into
+//     device memory the caller owns, such as the pointer of a GL VBO the caller registered
+//     and mapped with hipGraphicsGLRegisterBuffer / hipGraphicsMapResources; the reference
+//     maps the VBO inside the call (.cu:633-643);
 //   * random initialisation is seeded: particles by cf_init_particles with a per-object
 //     counter instead of curand seeded from time(nullptr) (.cu:480); the raw force table and
 //     the radius modifiers use std::rand() exactly as the reference does (.cu:516, :535).
@@ -248,6 +251,30 @@ class ParticleSimulation {
     vertices.resize((size_t)vc * 9);
     check(cf_get_triangle_vertices(h_, particleColors.data(), (int)particleColors.size(),
                                    vertices.data(), (long long)vertices.size(), &vc));
+    outVertexCount = (int)vc;
+  }
+  // generateProximityGraph writing the line vertices into caller-owned device memory
+  // (`deviceVertices`, capacityFloats floats), e.g. a mapped GL VBO: no host round trip.
+  void generateProximityGraph(float* deviceVertices, long long capacityFloats,
+                              int& outVertexCount, float proximityDistance,
+                              int maxConnectionsPerParticle,
+                              const std::vector<ParticleColor>& particleColors) {
+    check(cf_build_edges(h_, proximityDistance, maxConnectionsPerParticle));
+    long long vc = 0;
+    check(cf_write_line_vertices_device(h_, particleColors.data(), (int)particleColors.size(),
+                                        deviceVertices, capacityFloats, &vc));
+    outVertexCount = (int)vc;
+  }
+  // generateTriangleMesh into caller-owned device memory, as above.
+  void generateTriangleMesh(float* deviceVertices, long long capacityFloats,
+                            int& outVertexCount, float proximityDistance,
+                            int maxConnectionsPerParticle,
+                            const std::vector<ParticleColor>& particleColors) {
+    check(cf_build_triangles(h_, proximityDistance, maxConnectionsPerParticle));
+    long long vc = 0;
+    check(cf_write_triangle_vertices_device(h_, particleColors.data(),
+                                            (int)particleColors.size(), deviceVertices,
+                                            capacityFloats, &vc));
     outVertexCount = (int)vc;
   }
   // The last proximity graph as canonical (i, j) pairs.

@@ -9,7 +9,7 @@ flat normals are the same FP32 expression in both and are held to 1e-6 absolute.
 import numpy as np
 import pytest
 
-from cellflow_amd import CellFlowError, ParticleSimulation
+from cellflow_amd import CellFlowError, ParticleSimulation, _abi
 from helpers import make_sim, small_cfg, state_of
 
 pytestmark = pytest.mark.gpu
@@ -133,3 +133,117 @@ def test_vertex_buffers_reused_and_mesh_dropped_with_a_new_state(orc):
                  lambda: sim.getLineVertices(colors)):
         with pytest.raises(CellFlowError):
             read()
+
+
+def test_mesh_dropped_when_the_state_advances():
+    """A mesh belongs to the state it was built on: a step, a profiled step or a gradient
+    rollout advances the state, after which reading the mesh is CF_ERR_STATE.  A call that
+    does not step (simulate with 0 steps) keeps it."""
+    st, phys, params = state_of(small_cfg(n=2000, side=0.4))
+    sim = make_sim(st, phys)
+    sim.simulate(params, 1)
+    colors = np.ones((6, 3), np.float32)
+    advance = [lambda: sim.simulate(params, 1),
+               lambda: sim.profileSteps(params, 2),
+               lambda: sim.rolloutGrad(params, 2, 1, 8, 0.1, np.zeros(8))]
+    for step in advance:
+        tris = sim.generateTriangleMesh(0.03, 8)
+        assert len(tris) > 0
+        sim.simulate(params, 0)
+        assert np.array_equal(sim.getTriangles(), tris)
+        step()
+        for read in (sim.getTriangles, lambda: sim.getTriangleVertices(colors)):
+            with pytest.raises(CellFlowError) as e:
+                read()
+            assert e.value.code == _abi.CF_ERR_STATE
+    sim.close()
+
+
+def _hip():
+    """The HIP runtime through ctypes, for raw allocations and pointer queries in tests."""
+    import ctypes as C
+    h = C.CDLL("libamdhip64.so")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return h
+
+
+class _PtrAttr(__import__("ctypes").Structure):
+    import ctypes as C
+    _fields_ = [("type", C.c_int), ("device", C.c_int), ("devicePointer", C.c_void_p),
+                ("hostPointer", C.c_void_p), ("isManaged", C.c_int),
+                ("allocationFlags", C.c_uint)]
+
+
+def test_vertices_written_into_device_memory(orc):
+    """The device-side vertex writes (the reference maps its GL buffer and the kernels write
+    into it) give the same bytes as the host expansions; too small an output is
+    CF_ERR_CAPACITY, both by the stated capacity and by the allocation behind the pointer,
+    with nothing launched."""
+    import ctypes as C
+
+    import torch
+    st, phys, params = state_of(small_cfg(n=3000, side=0.5))
+    sim = make_sim(st, phys)
+    sim.simulate(params, 1, edges=(0.03, 6))
+    colors = np.random.default_rng(0).random((6, 3)).astype(np.float32)
+    lines = sim.getLineVertices(colors)
+    sim.generateTriangleMesh(0.03, 8)
+    tverts = sim.getTriangleVertices(colors)
+    assert len(lines) > 100 and len(tverts) > 100
+    for ref, write, width in ((lines, sim.writeLineVerticesDevice, 6),
+                              (tverts, sim.writeTriangleVerticesDevice, 9)):
+        need = ref.size
+        buf = torch.full((need + 64,), -7.0, dtype=torch.float32, device="cuda")
+        vc = write(colors, buf.data_ptr(), need + 64)
+        assert vc == len(ref)
+        got = buf.cpu().numpy()
+        assert got[:need].tobytes() == ref.tobytes()
+        assert np.all(got[need:] == -7.0)  # nothing past the vertices
+        pinned = torch.zeros(need, dtype=torch.float32, pin_memory=True)  # hipHostMalloc
+        assert write(colors, pinned.data_ptr(), need) == len(ref)
+        assert pinned.numpy().tobytes() == ref.tobytes()
+        with pytest.raises(CellFlowError) as e:
+            write(colors, buf.data_ptr(), need - 1)
+        assert e.value.code == _abi.CF_ERR_CAPACITY
+        # a capacity larger than the allocation behind the pointer is caught too
+        hip = _hip()
+        raw = C.c_void_p()
+        assert hip.hipMalloc(C.byref(raw), 4 * need) == 0
+        try:
+            with pytest.raises(CellFlowError) as e:
+                write(colors, raw.value + 4 * width, need)
+            assert e.value.code == _abi.CF_ERR_CAPACITY
+            assert write(colors, raw.value, need) == len(ref)
+            back = np.zeros(need, np.float32)
+            assert hip.hipMemcpy(back.ctypes.data, raw, 4 * need, 2) == 0  # device to host
+            assert back.tobytes() == ref.tobytes()
+        finally:
+            hip.hipFree(raw)
+    sim.close()
+
+
+def test_device_vertex_write_rejects_pageable_host_memory():
+    """Pageable host memory is not writable by the GPU here: the call must refuse it on the
+    host.  The case runs only when the runtime itself reports the buffer as unregistered or
+    unknown, which is what the engine's check tests before launching anything."""
+    import ctypes as C
+    st, phys, params = state_of(small_cfg(n=2000, side=0.4))
+    sim = make_sim(st, phys)
+    sim.simulate(params, 1, edges=(0.03, 6))
+    colors = np.ones((6, 3), np.float32)
+    need = sim.getLineVertices(colors).size
+    host = np.zeros(need, np.float32)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipPointerGetAttributes.argtypes = [C.POINTER(_PtrAttr), C.c_void_p]
+    at = _PtrAttr()
+    rc = hip.hipPointerGetAttributes(C.byref(at), host.ctypes.data)
+    hip.hipGetLastError()  # leave no error from the query for later calls to find
+    if rc == 0 and at.type != 0:
+        pytest.skip(f"the runtime reports pageable memory as type {at.type}")
+    with pytest.raises(CellFlowError) as e:
+        sim.writeLineVerticesDevice(colors, host.ctypes.data, need)
+    assert e.value.code == _abi.CF_ERR_ARG
+    assert not host.any()
+    sim.close()

@@ -400,20 +400,38 @@ def test_profile_steps_times_each_kernel_and_steps_the_state():
 
 
 def test_gets_into_caller_buffers():
-    """getParticleData / getLineVertices fill a caller's array when it is large enough (as
-    the reference fills the caller's vector) and give the same bytes as a fresh array."""
+    """getParticleData / getLineVertices / getTriangleVertices fill a caller's array in place
+    (as the reference fills the caller's vector) and give the same bytes as a fresh array; a
+    buffer that cannot be filled in place (too small, not contiguous, wrong dtype) raises
+    ValueError and is left untouched."""
     cfg = small_cfg(n=3000, side=0.4, T=3)
     st, phys, params = state_of(cfg)
     sim = make_sim(st, phys)
     sim.simulate(params, 1, edges=(0.03, 4))
     colors = np.eye(3, dtype=np.float32)
     parts, lines = sim.getParticleData(), sim.getLineVertices(colors)
+    sim.generateTriangleMesh(0.03, 6)
+    tverts = sim.getTriangleVertices(colors)
+    assert len(tverts) > 0
     pbuf = np.zeros(len(parts) + 7, parts.dtype)
-    vbuf = np.zeros(lines.size + 5, np.float32)
+    vbuf = np.zeros(lines.size + 6, np.float32).reshape(-1, 6)  # 2-D and contiguous: usable
+    tbuf = np.zeros(tverts.size + 9, np.float32)
     p2, l2 = sim.getParticleData(out=pbuf), sim.getLineVertices(colors, out=vbuf)
+    t2 = sim.getTriangleVertices(colors, out=tbuf)
     assert np.shares_memory(p2, pbuf) and np.shares_memory(l2, vbuf)
+    assert np.shares_memory(t2, tbuf)
     assert p2.tobytes() == parts.tobytes() and np.array_equal(l2, lines)
-    small = np.zeros(10, parts.dtype)  # too small: a new array, the buffer untouched
-    p3 = sim.getParticleData(out=small)
-    assert not np.shares_memory(p3, small) and p3.tobytes() == parts.tobytes()
-    assert not small.tobytes().strip(b"\0")
+    assert np.array_equal(t2, tverts)
+    bad = [np.zeros(10, parts.dtype),                         # too small
+           np.zeros(2 * (len(parts) + 1), parts.dtype)[::2],   # not contiguous
+           np.zeros(len(parts) + 1, np.float32)]               # wrong dtype
+    for buf in bad:
+        with pytest.raises(ValueError):
+            sim.getParticleData(out=buf)
+        assert not buf.tobytes().strip(b"\0")
+    strided = np.zeros((lines.size + 6) * 2, np.float32)[::2]
+    with pytest.raises(ValueError):
+        sim.getLineVertices(colors, out=strided)
+    assert not strided.any()
+    with pytest.raises(ValueError):
+        sim.getTriangleVertices(colors, out=np.zeros(tverts.size - 1, np.float32))

@@ -130,3 +130,64 @@ def test_property_number_text_is_python_repr(tmp_path_factory, vals):
     np.testing.assert_array_equal(read_preset(out)["rawForceTable"], raw)
     doc = json.loads(text)
     assert [float(x) for x in doc["rawForceTable"]] == [float(x) for x in raw]
+
+
+COMMA_LOCALES = ["de_DE.UTF-8", "de_DE.utf8", "de_DE", "fr_FR.UTF-8", "fr_FR.utf8", "fr_FR",
+                 "ru_RU.UTF-8", "nl_NL.UTF-8", "pt_BR.UTF-8", "it_IT.UTF-8"]
+
+
+@pytest.fixture
+def comma_locale():
+    """LC_NUMERIC switched to an installed comma-decimal locale (as a Qt application sets it
+    from the environment) for the duration of the test; skipped when none is installed."""
+    import locale
+    old = locale.setlocale(locale.LC_NUMERIC)
+    for name in COMMA_LOCALES:
+        try:
+            locale.setlocale(locale.LC_NUMERIC, name)
+        except locale.Error:
+            continue
+        if locale.localeconv()["decimal_point"] == ",":
+            break
+        locale.setlocale(locale.LC_NUMERIC, old)
+    else:
+        pytest.skip("no comma-decimal locale is installed")
+    yield
+    locale.setlocale(locale.LC_NUMERIC, old)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_roundtrip_under_a_comma_decimal_locale(tmp_path, comma_locale, name):
+    """The reader and writer convert numbers without the C locale: under a comma-decimal
+    LC_NUMERIC a preset still reads its fractions and writes back byte for byte."""
+    pr = read_preset(golden(name))
+    doc = json.load(open(golden(name)))
+    assert pr["radius"] == np.float32(doc["radius"])
+    out = tmp_path / "out.json"
+    write_preset(out, pr, template=golden(name))
+    assert out.read_bytes() == open(golden(name), "rb").read()
+
+
+def test_tuning_cache_under_a_comma_decimal_locale(tmp_path, comma_locale):
+    from cellflow_amd import tuning_cache_lookup, tuning_cache_store
+    p = tmp_path / "tune.json"
+    t = dict(cell_divisor=1, force_variant=3, lanes_per_particle=8, block_size=256,
+             steps_per_graph=32, sort_variant=1)
+    tuning_cache_store(p, "k", t, 46.25)
+    json.load(open(p))  # valid JSON: '.' as the decimal point
+    assert tuning_cache_lookup(p, "k") == (t, 46.25)
+
+
+@pytest.mark.parametrize("text,value", [("1e-400", 0.0), ("-1e-400", -0.0), ("0.25", 0.25),
+                                        ("-2.5E+3", -2500.0), ("1e999", None)])
+def test_number_range(tmp_path, text, value):
+    """Numbers below the double range read as a zero of their sign; numbers above it are
+    rejected rather than read as infinity."""
+    p = tmp_path / "p.json"
+    p.write_text('{"radius": %s}\n' % text)
+    if value is None:
+        with pytest.raises(CellFlowError):
+            read_preset(p)
+    else:
+        r = read_preset(p)["radius"]
+        assert r == value and np.signbit(r) == np.signbit(value)
