@@ -62,34 +62,116 @@ def read_traffic(path, kernel):
     return float(vals["traffic_bytes_per_launch"])
 
 
-def cpu_baseline(cfg, budget_s=10.0):
+def _oracle_rate(oracle, cfg, steps, mode, threads, edges=None, keep=0):
+    """Particle-steps/s of the oracle over `steps` steps of a workload from its seeded start
+    (FP64, mode 0 all-pairs as the reference enumerates, 1 grid), with the graph of every
+    post-step state when `edges` is given; also the first `keep` states."""
+    import numpy as np
+
+    from cellflow_amd import configs
+    st, phys = configs.make_state(cfg), configs.physics(cfg)
+    pos, vel, prev = st["pos"].astype(np.float64), st["vel"].astype(np.float64), st["prev"].copy()
+    kept = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        out = oracle.step(pos, vel, st["type"], prev, phys, st["table"], st["rbt"], mode=mode,
+                          nthreads=threads)
+        pos, vel, prev = out["pos"], out["vel"], out["count"]
+        if edges:
+            oracle.edges(pos.astype(np.float32), st["type"], edges[0], edges[1], mode=mode,
+                         nthreads=threads)
+        if k < keep:
+            kept.append((pos, vel, prev))
+    el = time.perf_counter() - t0
+    return cfg["n"] * steps / el, el, kept
+
+
+def cpu_baseline(cfg, params, budget_s=10.0):
+    """The CPU-baseline leg.  The oracle (FP64 scalar C++ restatement of the reference step)
+    on the host cores: the 100k workload with edges on all threads for about `budget_s`
+    seconds (the reported value), plus single-thread and 10k-workload samples in both
+    enumerations.  The oracle's first 10 states of the 100k sample then check a 10-step
+    engine rollout from the same start (parity_sample).  Returns (baseline, parity)."""
+    import oracle
+    from cellflow_amd import configs
+    e = cfg["edges"]
+    edges = (e["distance"], e["max_connections"])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # the main sample: as many steps as fit the budget (calibrated on two steps)
+    r2, el2, _ = _oracle_rate(oracle, cfg, 2, 1, threads, edges)
+    steps = max(10, min(5000, int(budget_s / (el2 / 2))))
+    rate, el, kept = _oracle_rate(oracle, cfg, steps, 1, threads, edges, keep=10)
+    one, el1, _ = _oracle_rate(oracle, cfg, 5, 1, 1, edges)
+    c1 = configs.load("c1_plain_10k")
+    samples = {
+        "c2_grid_all_threads": {"value": rate, "steps": steps, "s": el, "threads": threads},
+        "c2_grid_1_thread": {"value": one, "steps": 5, "s": el1, "threads": 1},
+    }
+    for mode, tag, n_all, n_one in ((1, "grid", c1["steps"], c1["steps"]), (0, "all_pairs", 10, 2)):
+        ra, ea, _ = _oracle_rate(oracle, c1, n_all, mode, threads)
+        r1, e1, _ = _oracle_rate(oracle, c1, n_one, mode, 1)
+        samples[f"c1_{tag}_all_threads"] = {"value": ra, "steps": n_all, "s": ea, "threads": threads}
+        samples[f"c1_{tag}_1_thread"] = {"value": r1, "steps": n_one, "s": e1, "threads": 1}
+    base = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the 100k workload (grid enumeration, FP64, edges on), "
+                      f"{el:.1f} s on {threads} threads",
+            "single_thread": one, "samples": samples,
+            "note": "10k samples: the plain 10k workload (no edges); all_pairs enumerates every "
+                    "pair as the reference kernel does (ParticleSimulation.cu:86-89), grid "
+                    "uses the cell stencil (same neighbour set)"}
+    return base, parity_sample(cfg, params, kept)
+
+
+def parity_sample(cfg, params, kept):
+    """Part of the CPU-baseline leg (the oracle is the checker): the engine's first 10
+    steps of the 100k workload (edges on) against the oracle's FP64 states from the CPU
+    baseline's sample, from the same FP32 start.  Errors are over
+    the particles outside the light cone of every decision flip (count mismatch): a flip
+    marks both particles and each step marks every particle in range of a marked one.
+    pos: max |x - x64| / L (periodic); vel: max |v - v64| / max |v64|."""
     import numpy as np
 
     import oracle
-    from cellflow_amd import configs
-    st = configs.make_state(cfg)
-    phys = configs.physics(cfg)
+    from cellflow_amd import ParticleSimulation, configs, particles_from_state
+    st, phys = configs.make_state(cfg), configs.physics(cfg)
     e = cfg["edges"]
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    pos = st["pos"].astype(np.float64)
-    vel = st["vel"].astype(np.float64)
-    prev = st["prev"].copy()
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        out = oracle.step(pos, vel, st["type"], prev, phys, st["table"], st["rbt"], mode=1,
-                          nthreads=threads)
-        pos, vel, prev = out["pos"], out["vel"], out["count"]
-        oracle.edges(pos.astype(np.float32), st["type"], e["distance"], e["max_connections"],
-                     mode=1, nthreads=threads)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 5000:
-            break
-    return {"value": cfg["n"] * steps / el, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{steps} steps of the 100k workload (grid enumeration, FP64, edges on), "
-                      f"{el:.1f} s"}
+    L = np.asarray(phys["canvas"], np.float64)
+    sim = ParticleSimulation(cfg["n"], cfg["num_types"])
+    sim.setForceTable(st["table"])
+    sim.setRadioByType(st["rbt"])
+    sim.setParticles(particles_from_state(st["pos"], st["vel"], st["type"]))
+    marked = np.zeros(cfg["n"], bool)
+    prev_ref = st["pos"]
+    rows = []
+    for k, (pos64, vel64, cnt64) in enumerate(kept):
+        cur = sim.getParticleData()
+        for p in (prev_ref, cur["pos"]):
+            offs, nbr = oracle.pairs(np.asarray(p, np.float32), st["type"], phys, st["rbt"])
+            owner = np.repeat(np.arange(cfg["n"]), np.diff(offs))
+            hit = np.zeros(cfg["n"], bool)
+            np.logical_or.at(hit, owner, marked[nbr])
+            marked |= hit
+        sim.simulate(params, 1, edges=(e["distance"], e["max_connections"]))
+        got = sim.getParticleData()
+        flips = sim.getNeighborCounts() != cnt64
+        marked |= flips
+        clean = ~marked
+        d = got["pos"].astype(np.float64) - pos64
+        d -= L * np.round(d / L)
+        ex = float(np.abs(d[clean]).max() / L.max())
+        ev = float(np.abs(got["vel"].astype(np.float64) - vel64)[clean].max() / np.abs(vel64).max())
+        rows.append({"step": k + 1, "flips": int(flips.sum()),
+                     "flip_free_fraction": float(clean.mean()), "pos_rel": ex, "vel_rel": ev})
+        prev_ref = pos64.astype(np.float32)
+    sim.close()
+    last = rows[-1]
+    return {"workload": cfg["name"], "steps": len(rows), "reference": "oracle FP64, same FP32 start",
+            "max_pos_rel": max(r["pos_rel"] for r in rows),
+            "max_vel_rel": max(r["vel_rel"] for r in rows),
+            "first_step": rows[0], "last_step": last, "per_step": rows,
+            "note": "free-running; errors over the particles outside the light cone of "
+                    "every decision flip (the dynamics are chaotic: an FP64 run started one "
+                    "FP32 rounding away diverges by a similar amount)"}
 
 
 def ensemble_leg(steps, warm=3):
@@ -164,6 +246,51 @@ def clustered_leg():
            "mean_neighbours_final": float(counts.mean()), "edges_final": len(sim.getEdges())}
     sim.close()
     return out
+
+
+def frame_gpu_leg(sim, params, edges, frames=50):
+    """GPU time of the widget's per-frame work (CellFlowWidget.cpp:424, :569): one step,
+    then the proximity graph of the post-step state by the standalone pass, per kernel
+    (HIP events on the engine's stream), averaged over `frames` frames."""
+    acc = {}
+    for _ in range(frames):
+        for k, v in sim.profileSteps(params, 1, edges=edges).items():
+            acc[k] = acc.get(k, 0.0) + v
+    per = {k: v / frames for k, v in acc.items()}
+    return {"frames": frames, "kernel_ms": {k: round(v, 5) for k, v in per.items()},
+            "gpu_ms_per_frame": sum(per.values()),
+            "graph_ms": sum(v for k, v in per.items() if k.startswith("edge")),
+            "note": "simulate(1 step, edges): the step's kernels, then the standalone graph "
+                    "pass of the post-step state, as the widget drives it once per frame"}
+
+
+def read_counters(path, kernel):
+    """Per-launch counter values of `kernel` from a committed rocprofv3 PMC summary
+    ('counter: value' lines after a 'kernel: <name>' line), if the summary is of that
+    kernel."""
+    vals, cur = {}, None
+    try:
+        with open(path) as f:
+            for line in f:
+                if ":" not in line or line.startswith((" ", "|", "#")):
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k == "kernel":
+                    cur = v
+                elif cur == kernel:
+                    try:
+                        vals[k] = float(v)
+                    except ValueError:
+                        pass
+    except OSError:
+        return None
+    return vals or None
+
+
+# gfx950: 256 CUs; a CU issues one wave64 VALU instruction per cycle (4 SIMD16s, 4 cycles
+# each); 2.4 GHz peak engine clock; FP32 vector peak 157.3 TFLOP/s
+VALU_ISSUE_PEAK = 256 * 2.4e9
+FP32_PEAK = 157.3e12
 
 
 def frame_leg(sim, params, edges, T, frames=200, warm=5):
@@ -349,6 +476,30 @@ def main():
             "kernel_ms_per_launch": {k: round(v, 5) for k, v in kt.items()},
             "note": "the force kernel runs every step; the standalone edge pass (edges, "
                     "edge_scan, edge_write) runs once per call, after the last step"}
+    ctr = read_counters(os.path.join(ROOT, "profiles", "k_force_c2_valu.txt"), kname)
+    if ctr and "SQ_INSTS_VALU" in ctr:
+        import numpy as np
+        # useful FP32 work: 12 FLOP per stencil candidate (difference, distance, cutoff)
+        # plus 25 per in-range pair (the force term), on the state after the timed run
+        dims, _ = sim.getGrid()
+        cells, _ = sim.getSortOrder()
+        occ = np.bincount(cells, minlength=int(np.prod(dims))).reshape(dims[::-1])
+        stencil = sum(np.roll(occ, (dz, dy, dx), (0, 1, 2))
+                      for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1))
+        cand = float((occ * stencil).sum())
+        inrange = float(sim.getNeighborCounts().sum())
+        flops = 12.0 * cand + 25.0 * inrange
+        vi = ctr["SQ_INSTS_VALU"]
+        roof["valu"] = {"instr_per_launch": vi, "salu_instr_per_launch": ctr.get("SQ_INSTS_SALU"),
+                        "issue_us": vi / VALU_ISSUE_PEAK * 1e6,
+                        "issue_frac": vi / VALU_ISSUE_PEAK / (force_ms * 1e-3),
+                        "useful_flop_per_launch": flops,
+                        "flop_frac": flops / (force_ms * 1e-3) / FP32_PEAK,
+                        "source": "profiles/k_force_c2_valu.txt (rocprofv3 --pmc)",
+                        "note": "issue_frac: VALU wave instructions at one per CU per cycle "
+                                "(2.4 GHz) over the launch time; flop_frac: 12 FLOP per "
+                                "27-cell stencil candidate + 25 per in-range pair over FP32 "
+                                "peak"}
 
     out = {"metric": METRIC, "value": rate, "unit": "particle-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
@@ -364,6 +515,8 @@ def main():
                                                       "lanes_per_particle", "block_size",
                                                       "us_per_step")} for t in tuned["trials"]]}}
     if not args.no_frame:
+        log("[bench] widget frame: GPU time per kernel")
+        out["frame_gpu"] = frame_gpu_leg(sim, params, edges)
         log("[bench] frame loop (host-inclusive)")
         out["frame_loop"] = frame_leg(sim, params, edges, cfg["num_types"])
     if not args.no_mesh:
@@ -378,8 +531,8 @@ def main():
     if not args.no_clustered:
         out["clustered"] = clustered_leg()
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle)")
-        out["cpu_baseline"] = cpu_baseline(cfg)
+        log("[bench] CPU baseline (oracle) and a 10-step parity sample against it")
+        out["cpu_baseline"], out["parity"] = cpu_baseline(cfg, params)
     print(json.dumps(out), flush=True)
 
 

@@ -109,6 +109,7 @@ SIGNATURES = {
     "cf_simulate": (_i, [_vp, P(cf_params), _i, P(cf_edge_opts)]),
     "cf_build_edges": (_i, [_vp, _f, _i]),
     "cf_get_edges": (_i, [_vp, P(C.c_int32), _ll, P(_ll)]),
+    "cf_get_step_edges": (_i, [_vp, P(C.c_int32), _ll, P(_ll)]),
     "cf_get_line_vertices": (_i, [_vp, P(cf_color), _i, P(_f), _ll, P(_ll)]),
     "cf_build_triangles": (_i, [_vp, _f, _i]),
     "cf_get_triangles": (_i, [_vp, P(C.c_int32), _ll, P(_ll)]),

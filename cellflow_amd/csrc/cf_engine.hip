@@ -168,6 +168,9 @@ struct Engine {
   int edge_cap_alloc = 0;   // candidate rows allocated (2*maxc)
   int edge_maxc_alloc = 0;
   long long edges_count = -1;  // -1: no edge list built
+  // the graph the last call's final step built inside its force pass and sort (that of
+  // the state before the step); -1: none (fewer than two steps, or no fused edges)
+  long long step_edges_count = -1;
   int32_t* tris = nullptr;     // triangle mesh: 3 local indices per triangle
   long long tris_cap = 0;      // triangles `tris` holds
   long long tri_count = -1;    // -1: no mesh built
@@ -229,7 +232,7 @@ struct Engine {
     dfree(b.tpos); dfree(b.tvel); dfree(b.tid); dfree(b.key); dfree(b.arrival);
     dfree(b.hist); dfree(b.tmp_id); dfree(b.acc); dfree(b.pad);
     dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.eoff);
-    dfree(b.edges); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
+    dfree(b.edges); dfree(b.sedges); dfree(b.stotal); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
     dfree(slot_of); dfree(tickets); dfree(tris); dfree(vtx); dfree(vcol); dfree(iobuf);
     tris_cap = 0;
     tri_count = -1;
@@ -283,6 +286,7 @@ struct Engine {
     CFR(dalloc(&b.sims, nsims)); CFR(dalloc(&b.wave_sim, nwaves));
     CFR(dalloc(&b.pairs, (size_t)nsims * kMaxTypes * kMaxTypes));
     CFR(dalloc(&b.flag, 4)); CFR(dalloc(&tickets, 3)); CFR(dalloc(&slot_of, P));
+    CFR(dalloc(&b.stotal, 1));
     netiles = (int)((P + kEdgeTile - 1) / kEdgeTile);
     for (int i = 0; i < 2; ++i) CFR(dalloc(&etcnt[i], (size_t)netiles * kEdgeSub * kCtrStride));
     CFR(dalloc(&farn, 2)); CFR(dalloc(&far_slot, P));
@@ -570,6 +574,7 @@ struct Engine {
     CFR(sort_into(b.tpos, b.tvel, b.tid, cur));
     load_pending = false;
     edges_count = -1;  // a graph or mesh belongs to the state it was built on
+    step_edges_count = -1;
     tri_count = -1;
     return CF_OK;
   }
@@ -614,6 +619,9 @@ struct Engine {
       HIPR(hipStreamSynchronize(st));
       dfree(b.edges);
       b.edges = grown;
+      dfree(b.sedges);
+      step_edges_count = -1;
+      CFR(dalloc(&b.sedges, (size_t)maxc * P));
       edge_maxc_alloc = maxc;
     }
     return CF_OK;
@@ -677,7 +685,8 @@ struct Engine {
     if (edges) {
       sa.estage = b.estage;
       sa.eoff = b.eoff;
-      sa.edges = b.edges;
+      sa.edges = b.sedges;
+      sa.etotal = b.stotal;
     }
     sa.P = P;
     sa.nwaves = nwaves;
@@ -716,7 +725,8 @@ struct Engine {
       ta.etcount = etcnt[A];
       ta.etcount_next = etcnt[B];
       ta.estage = b.estage;
-      ta.edges = b.edges;
+      ta.edges = b.sedges;
+      ta.etotal = b.stotal;
       ta.edge_room = (long)edge_maxc_alloc * P;
       ta.netiles = netiles;
     }
@@ -834,12 +844,15 @@ struct Engine {
     return true;
   }
 
-  int finish_call(bool edges_built) {
+  int finish_call(bool edges_built, bool step_edges_built = false) {
     HIPR(hipMemcpyAsync(hsmall, b.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (edges_built)
       HIPR(hipMemcpyAsync(hsmall + 1, b.eoff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (step_edges_built)
+      HIPR(hipMemcpyAsync(hsmall + 3, b.stotal, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
     if (edges_built) edges_count = (long long)hsmall[1];
+    if (step_edges_built) step_edges_count = (long long)hsmall[3];
     if (hsmall[0] & 1u) return fail(CF_ERR_NONFINITE, "a position or velocity became non-finite");
     if (hsmall[0] & 4u) return fail(CF_ERR_DEVICE, "internal: the step sort lost track of a particle");
     return CF_OK;
@@ -854,6 +867,7 @@ struct Engine {
     const bool fuse = fusable(er);
     const int cap = 2 * er.maxc;
     edges_count = -1;
+    step_edges_count = -1;
     if (nsteps > 0) tri_count = -1;  // a mesh belongs to the state it was built on
     if (er.on && !fuse) {
       for (int k = 0; k < nsteps; ++k) {
@@ -868,7 +882,7 @@ struct Engine {
     }
     if (er.on && (fuse || nsteps == 0)) CFR(edge_pass(cap));
     if (er.on) edges_maxc = er.maxc;
-    return finish_call(er.on);
+    return finish_call(er.on, er.on && fuse && nsteps >= 2);
   }
 
   // ---- state I/O ---------------------------------------------------------------------
@@ -1039,6 +1053,21 @@ struct Engine {
                   edges_count);
     if (edges_count > 0)
       HIPR(hipMemcpyAsync(pairs, b.edges, (size_t)edges_count * sizeof(int2),
+                          hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    return CF_OK;
+  }
+
+  int get_step_edges(int32_t* pairs, long long capacity, long long* count) {
+    CFR(flush());
+    if (step_edges_count < 0)
+      return fail(CF_ERR_STATE, "the last call built no graph inside its steps");
+    *count = step_edges_count;
+    if (capacity < step_edges_count)
+      return fail(CF_ERR_CAPACITY, "edge buffer holds %lld pairs, %lld needed", capacity,
+                  step_edges_count);
+    if (step_edges_count > 0)
+      HIPR(hipMemcpyAsync(pairs, b.sedges, (size_t)step_edges_count * sizeof(int2),
                           hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
     return CF_OK;
@@ -1260,6 +1289,7 @@ struct Engine {
       edges_count = -1;
       tri_count = -1;
     }
+    step_edges_count = -1;
     // the step's kernels (ns of them), then the standalone edge pass's three
     const bool tiles = tun.sort_variant == 1;
     const int ns = tiles ? 2 : 4;
@@ -1328,7 +1358,7 @@ struct Engine {
       std::snprintf(names, names_cap, "%s", nm.c_str());
     }
     if (er.on) edges_maxc = er.maxc;
-    return finish_call(er.on);
+    return finish_call(er.on, er.on && fuse && nsteps >= 2);
   }
 
   // ---- differentiable rollout -----------------------------------------------------------
@@ -1504,6 +1534,7 @@ struct Engine {
     CFR(prepare(none));
     // the rollout advances the state: a graph or mesh of the old state no longer applies
     edges_count = -1;
+    step_edges_count = -1;
     tri_count = -1;
     const int n = h.n, T = h.T;
     const long stride = ((long)n + 127) / 128 * 128;
@@ -1835,6 +1866,7 @@ struct Engine {
     dfree(acc_snap);
     drop_graphs();
     edges_count = -1;
+    step_edges_count = -1;
     return rc;
   }
 };
@@ -2277,6 +2309,12 @@ int cf_get_edges(cf_sim* sim, int32_t* pairs, long long capacity, long long* cou
   CFR(check(sim));
   if (!count || (!pairs && capacity > 0)) return cf::fail(CF_ERR_ARG, "null argument");
   return sim->e->get_edges(pairs, capacity, count);
+}
+
+int cf_get_step_edges(cf_sim* sim, int32_t* pairs, long long capacity, long long* count) {
+  CFR(check(sim));
+  if (!count || (!pairs && capacity > 0)) return cf::fail(CF_ERR_ARG, "null argument");
+  return sim->e->get_step_edges(pairs, capacity, count);
 }
 
 int cf_build_triangles(cf_sim* sim, float distance, int max_connections) {

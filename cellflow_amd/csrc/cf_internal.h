@@ -113,7 +113,9 @@ struct DevBufs {
   uint32_t* estage;     // [maxc][P] kept neighbour index by rank
   uint32_t* ecount;     // [P + 1] kept edges per index
   uint32_t* eoff;       // [P + 1] exclusive prefix of ecount
-  int2* edges;          // canonical output pairs
+  int2* edges;          // canonical pairs of the standalone edge pass (the current state)
+  int2* sedges;         // canonical pairs the step's sort writes (the pre-step state)
+  uint32_t* stotal;     // their number
   // descriptors
   SimDesc* sims;
   uint32_t* wave_sim;   // simulation index of every 64-slot wave
@@ -203,6 +205,7 @@ struct SortArgs {
   const uint32_t* __restrict__ estage;
   const uint32_t* __restrict__ eoff;
   int2* __restrict__ edges;
+  uint32_t* etotal;     // k_fill: receives the number of edges written (eoff[P])
   long P;
   int nwaves;
 };
@@ -241,6 +244,7 @@ struct TileSortArgs {
   uint32_t* etcount_next;
   const uint32_t* estage;               // [rank][owner index]
   int2* edges;
+  uint32_t* etotal;                     // the last edge tile writes the edge count here
   long edge_room;                       // entries `edges` holds
   const uint32_t* __restrict__ wave_sim;
   int netiles;

@@ -1570,6 +1570,7 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
     if (tid == 0) atomicOr(a.flag, 4u);
     return;
   }
+  if (tid == 0 && u == a.netiles - 1) *a.etotal = base + total;  // the whole list's size
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     if (ec[i]) {
@@ -1950,6 +1951,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(ScanPlan plan) {
 
 __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
   const int gw = xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (a.edges && gw == 0 && threadIdx.x % kWave == 0) *a.etotal = a.eoff[a.P];
   if (gw >= a.nwaves) return;
   const SimDesc sd = a.sims[a.wave_sim[gw]];
   const int p = gw * kWave + (threadIdx.x & (kWave - 1));

@@ -327,6 +327,18 @@ class ParticleSimulation:
         check(lib().cf_get_edges(self._h, _ip(out), cnt.value, C.byref(cnt)))
         return out[: cnt.value]
 
+    def getStepEdges(self) -> np.ndarray:
+        """The graph the last simulate call built inside its final step (that of the state
+        before the final step), in the layout of :meth:`getEdges`; raises CF_ERR_STATE when
+        that call built none (fewer than two steps, or no edges)."""
+        cnt = C.c_longlong()
+        rc = lib().cf_get_step_edges(self._h, None, 0, C.byref(cnt))
+        if rc not in (_abi.CF_OK, _abi.CF_ERR_CAPACITY):
+            check(rc)
+        out = np.zeros((max(cnt.value, 1), 2), np.int32)
+        check(lib().cf_get_step_edges(self._h, _ip(out), cnt.value, C.byref(cnt)))
+        return out[: cnt.value]
+
     def generateTriangleMesh(self, proximityDistance: float, maxConnectionsPerParticle: int):
         """generateTriangleMesh (ParticleSimulation.cu:689-754) on the current state; returns
         the (T, 3) int32 triangles in canonical (owner, fan) order."""

@@ -228,6 +228,14 @@ int cf_get_grid(cf_sim* sim, int dims[3], int* reach);
 /* The canonical (cell, index) order of the current binning: for slot k, the particle
  * index and its linear cell id (x fastest). */
 int cf_get_sort_order(cf_sim* sim, uint32_t* cell_of_slot, uint32_t* index_of_slot, int count);
+/* The proximity graph the last cf_simulate / cf_profile_steps call built inside its final
+ * step's force pass and sort: that of the state before the final step (the graph of every
+ * intermediate post-step state is built this way and replaced by the next one; the state
+ * after the final step gets its graph from the standalone pass that cf_get_edges reads).
+ * Same layout and order as cf_get_edges.  CF_ERR_STATE when the last call built none:
+ * fewer than two steps, no edges, or an edge distance beyond the force stencil (then
+ * every step uses the standalone pass). */
+int cf_get_step_edges(cf_sim* sim, int32_t* pairs, long long capacity, long long* count);
 /* Per-kernel average durations (ms) over `nsteps` steps timed with HIP events on the
  * engine's stream: names is a comma-separated list written into names_buf. */
 int cf_profile_steps(cf_sim* sim, const cf_params* params, int nsteps, const cf_edge_opts* edges,

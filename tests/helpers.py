@@ -60,3 +60,21 @@ def small_cfg(n=3000, side=0.5, T=6, radius=0.05, ratio=0.3, seed=0, **over):
 def state_of(cfg, seed=None):
     return configs.make_state(cfg, seed), configs.physics(cfg), \
         SimulationParams.from_physics(configs.physics(cfg))
+
+
+def check_step_edges(orc, sim, twin, params, edges, mode=0, steps=2):
+    """The graph a step builds inside its force pass (the fused path, of the state before
+    the step) against the oracle.  `sim` and `twin` hold the same state and tuning: `sim`
+    runs `steps` steps with edges, so its last step builds the graph of the state after
+    steps - 1; `twin` runs steps - 1 steps to that state (the engine is deterministic).
+    Returns the number of edges."""
+    sim.simulate(params, steps, edges=edges)
+    twin.simulate(params, steps - 1, edges=edges)
+    mid = twin.getParticleData()
+    ref = orc.edges(mid["pos"], mid["ptype"], edges[0], edges[1], mode=mode)
+    assert np.array_equal(twin.getEdges(), ref), "standalone graph"
+    assert np.array_equal(sim.getStepEdges(), ref), "graph built inside the step"
+    end = sim.getParticleData()
+    assert np.array_equal(sim.getEdges(),
+                          orc.edges(end["pos"], end["ptype"], edges[0], edges[1], mode=mode))
+    return len(ref)

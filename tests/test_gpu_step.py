@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from cellflow_amd import CellFlowError, ParticleSimulation, configs, particles_from_state
-from helpers import check_step, make_sim, small_cfg, state_of
+from helpers import check_step, check_step_edges, make_sim, small_cfg, state_of
 
 pytestmark = pytest.mark.gpu
 
@@ -193,24 +193,41 @@ def test_fused_edges_dense_blobs(orc, variant, lanes):
     list and take the streaming fallback (and, in variant 3, hold several candidate-list
     chunks); the graph must still equal the oracle's."""
     st, phys, params = clustered_state(n=5000, blobs=4, sd=0.006)
-    sim = make_sim(st, phys)
-    sim.setTuning(force_variant=variant, lanes_per_particle=lanes)
-    for k in range(2):
-        sim.simulate(params, 1, edges=(0.01, 16))
-        got = sim.getParticleData()
-        assert np.array_equal(sim.getEdges(), orc.edges(got["pos"], got["ptype"], 0.01, 16, mode=0))
+    sim, twin = make_sim(st, phys), make_sim(st, phys)
+    for h in (sim, twin):
+        h.setTuning(force_variant=variant, lanes_per_particle=lanes)
+    for sort in (1, 0):
+        for h in (sim, twin):
+            h.setTuning(force_variant=variant, lanes_per_particle=lanes, sort_variant=sort)
+        check_step_edges(orc, sim, twin, params, (0.01, 16), steps=2)
+        twin.simulate(params, 1)  # back in step with sim
 
 
 def test_fused_edges_every_step(orc):
-    """With edges on, each step's graph is of the post-step state (the last one is kept)."""
+    """With edges on, each step's graph is of the post-step state (the last one is kept by
+    the standalone pass; the one before it was built inside the last step)."""
     cfg = small_cfg(n=4000, side=0.5)
     st, phys, params = state_of(cfg)
-    sim = make_sim(st, phys)
+    sim, twin = make_sim(st, phys), make_sim(st, phys)
     for k in range(3):
-        sim.simulate(params, 2, edges=(0.03, 5))
-        got = sim.getParticleData()
-        ref = orc.edges(got["pos"], got["ptype"], 0.03, 5, mode=0)
-        assert np.array_equal(sim.getEdges(), ref)
+        check_step_edges(orc, sim, twin, params, (0.03, 5), steps=2 + k)
+        twin.simulate(params, 1)  # back in step with sim
+
+
+def test_step_edges_state_errors():
+    """The in-step graph exists only after a call of two or more steps with edges, and a
+    newly loaded state drops it."""
+    st, phys, params = state_of(small_cfg(n=2000, side=0.4))
+    sim = make_sim(st, phys)
+    for nsteps, edges in ((1, (0.03, 5)), (3, None)):
+        sim.simulate(params, nsteps, edges=edges)
+        with pytest.raises(CellFlowError):
+            sim.getStepEdges()
+    sim.simulate(params, 2, edges=(0.03, 5))
+    assert len(sim.getStepEdges()) > 0
+    sim.setParticles(sim.getParticleData())
+    with pytest.raises(CellFlowError):
+        sim.getStepEdges()
 
 
 def test_run_twice_bit_identical():
@@ -358,10 +375,13 @@ def test_variants_match_oracle(orc, variant, lanes, divisor, block, name, sort):
     got = sim.getParticleData()
     check_step(got, ref, phys, st["vel"])
     assert np.array_equal(sim.getEdges(), orc.edges(got["pos"], got["ptype"], 0.02, 6, mode=0))
-    # a second step exercises the fused edge candidates of the pre-step state
-    sim.simulate(params, 1, edges=(0.02, 6))
-    got2 = sim.getParticleData()
-    assert np.array_equal(sim.getEdges(), orc.edges(got2["pos"], got2["ptype"], 0.02, 6, mode=0))
+    # two more steps: the second builds the graph of the state between them inside its
+    # force pass and sort (the fused path); a twin handle at the same state supplies it
+    twin = make_sim(st, phys)
+    twin.setTuning(cell_divisor=divisor, force_variant=variant, lanes_per_particle=lanes,
+                   block_size=block, sort_variant=sort)
+    twin.simulate(params, 1, edges=(0.02, 6))
+    check_step_edges(orc, sim, twin, params, (0.02, 6))
 
 
 @pytest.mark.parametrize("variant,lanes", [(0, 16), (3, 8)])
