@@ -7,6 +7,7 @@
 // entry point at the bottom of this file names the member it replaces.
 
 #include "../../include/cellflow.h"
+#include "cf_host.h"
 #include "cf_internal.h"
 #include "cf_tune_cache.h"
 
@@ -27,19 +28,8 @@ namespace cf {
 // ------------------------------------------------------------------------------------
 // errors
 
-static thread_local std::string g_err;
-
-// Records the message of a failing call (read back by cf_last_error_string) and returns
-// `code`.  Also used by the host-only sources (cf_preset.cpp).
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
+// fail() and the thread's error text live in cf_host.cpp (host-only, shared with the
+// preset and tuning-cache code).
 
 #define HIPR(call)                                                                         \
   do {                                                                                     \
@@ -1855,11 +1845,11 @@ struct Engine {
     (void)hipStreamSynchronize(st);
     if (rc != CF_OK && snap.pos) {
       // leave the call's state in place even when a trial failed
-      const std::string msg = g_err;
+      const std::string msg = error_text();
       if (restore(snap, orig, er) == CF_OK && acc_snap)
         (void)hipMemcpyAsync(b.acc, acc_snap, P * sizeof(float4), hipMemcpyDeviceToDevice, st);
       (void)hipStreamSynchronize(st);
-      g_err = msg;
+      error_text() = msg;
     }
     if (rc != CF_OK) tun = orig;
     free_state(snap);
@@ -2015,7 +2005,6 @@ int make_engine(std::unique_ptr<cf::Engine>& out, int kind, int nsims, const int
 
 extern "C" {
 
-const char* cf_last_error_string(void) { return cf::g_err.c_str(); }
 int cf_version(void) { return 100; }
 
 int cf_device_count(int* count) {
@@ -2398,25 +2387,7 @@ int cf_profile_steps(cf_sim* sim, const cf_params* params, int nsteps, const cf_
   return sim->e->profile(nsteps, er, ms_out, max_kernels, nkernels, names_buf, names_cap);
 }
 
-static int check_tuning(const cf_tuning* t) {
-  if (!t) return cf::fail(CF_ERR_ARG, "tuning is null");
-  if (t->cell_divisor != 1 && t->cell_divisor != 2)
-    return cf::fail(CF_ERR_ARG, "cell_divisor must be 1 or 2");
-  if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
-    return cf::fail(CF_ERR_ARG, "block_size must be 64, 128 or 256");
-  if (t->force_variant < 0 || t->force_variant > 3)
-    return cf::fail(CF_ERR_ARG, "force_variant must be 0, 1, 2 or 3");
-  if (t->lanes_per_particle != 4 && t->lanes_per_particle != 8 && t->lanes_per_particle != 16 &&
-      t->lanes_per_particle != 32)
-    return cf::fail(CF_ERR_ARG, "lanes_per_particle must be 4, 8, 16 or 32");
-  if (t->force_variant == 3 && t->lanes_per_particle < 4 * t->cell_divisor)
-    return cf::fail(CF_ERR_ARG, "force_variant 3 needs lanes_per_particle >= 8 with "
-                                "cell_divisor 2");
-  if (t->steps_per_graph < 1) return cf::fail(CF_ERR_ARG, "steps_per_graph must be >= 1");
-  if (t->sort_variant != 0 && t->sort_variant != 1)
-    return cf::fail(CF_ERR_ARG, "sort_variant must be 0 or 1");
-  return CF_OK;
-}
+using cf::check_tuning;
 
 int cf_set_tuning(cf_sim* sim, const cf_tuning* tuning) {
   CFR(check(sim));
@@ -2461,39 +2432,6 @@ int cf_tuning_key(cf_sim* sim, const cf_params* params, const cf_edge_opts* edge
   if ((int)k.size() >= key_cap)
     return cf::fail(CF_ERR_CAPACITY, "key needs %d bytes", (int)k.size() + 1);
   std::snprintf(key, key_cap, "%s", k.c_str());
-  return CF_OK;
-}
-
-int cf_tuning_cache_lookup(const char* cache_path, const char* key, cf_tuning* tuning,
-                           double* us_per_step, int* found) {
-  if (!cache_path || !key || !tuning || !found) return cf::fail(CF_ERR_ARG, "null argument");
-  *found = 0;
-  std::map<std::string, cf::tunecache::Entry> ents;
-  std::string why;
-  const int st = cf::tunecache::read_file(cache_path, ents, why);
-  if (st == cf::tunecache::kBad)
-    return cf::fail(CF_ERR_ARG, "tuning cache %s: %s", cache_path, why.c_str());
-  auto it = ents.find(key);
-  if (it == ents.end() || !cf::check_tuning_values(it->second.tun)) return CF_OK;
-  *tuning = it->second.tun;
-  if (us_per_step) *us_per_step = it->second.us;
-  *found = 1;
-  return CF_OK;
-}
-
-int cf_tuning_cache_store(const char* cache_path, const char* key, const cf_tuning* tuning,
-                          double us_per_step) {
-  if (!cache_path || !key) return cf::fail(CF_ERR_ARG, "null argument");
-  CFR(check_tuning(tuning));
-  if (!std::isfinite(us_per_step)) return cf::fail(CF_ERR_ARG, "us_per_step must be finite");
-  std::map<std::string, cf::tunecache::Entry> ents;
-  std::string why;
-  const int st = cf::tunecache::read_file(cache_path, ents, why);
-  if (st == cf::tunecache::kBad)
-    return cf::fail(CF_ERR_ARG, "tuning cache %s: %s", cache_path, why.c_str());
-  ents[key] = cf::tunecache::Entry{*tuning, us_per_step};
-  if (!cf::tunecache::write_file(cache_path, ents, why))
-    return cf::fail(CF_ERR_ARG, "cannot write tuning cache %s: %s", cache_path, why.c_str());
   return CF_OK;
 }
 

@@ -12,11 +12,8 @@
 #include <unistd.h>
 
 #include "../../include/cellflow.h"
+#include "cf_host.h"
 #include "cf_json.h"
-
-namespace cf {
-int fail(int code, const char* fmt, ...);  // cf_engine.hip: sets the thread's error string
-}
 
 namespace {
 

@@ -579,3 +579,14 @@ def test_move_dyadic_cases_and_inverse(orc):
     # a zero shift of a position already in [0, L) gives fmodf(x + L, L): x up to rounding
     z = orc.move(x, 0.0, 0.0, 0.0, L)
     assert np.all(np.abs(z - x) <= np.spacing(np.float32(2 * 4.0)))
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (700, 2), (3000, 3)])
+def test_oracle_under_sanitizers(n, seed):
+    """The oracle's step (both enumerations), edges, triangles, pair lists (including a list
+    too small for its content) and histogram under AddressSanitizer and
+    UndefinedBehaviorSanitizer (a g++ build of oracle/oracle.cpp)."""
+    import host_sanitizer
+    out = host_sanitizer.run("oracle", n, seed)
+    assert out.count("rc=0") >= 8
+    assert "pairs (capacity 4) rc=3" in out or n < 100

@@ -114,3 +114,27 @@ def test_property_python_json_roundtrip(tmp_path_factory, ents, indent, ascii_):
     for k, (t, u) in ents.items():
         if k != "\u0001ctl":
             assert back[k] == dict(t, us_per_step=u)
+
+
+@pytest.mark.parametrize("text", [
+    None, "", "[]", "{", '{"format": "cellflow-tuning",}', '{"format": "other", "entries": {}}',
+    '{"format": "cellflow-tuning", "entries": {"a": {"cell_divisor": 01}}}',
+    '{"format": "cellflow-tuning", "x": "\\ud800"}',
+    '{"format": "cellflow-tuning", "x": ' + "[" * 40 + "]" * 40 + "}",
+    json.dumps({"format": "cellflow-tuning", "version": 1,
+                "entries": {"k": dict(T0, us_per_step=1.5), "bad": dict(T0, block_size=96),
+                            "é\U0001f600": 3}}),
+])
+def test_sanitized_cache_reader_and_writer(tmp_path, text):
+    """The cache reader and writer under AddressSanitizer and UndefinedBehaviorSanitizer (a
+    g++ build of the host-only sources), on missing, malformed and valid files."""
+    import host_sanitizer
+    p = tmp_path / "tune.json"
+    if text is not None:
+        p.write_text(text, encoding="utf-8")
+    look = host_sanitizer.run("cache-lookup", p, "k")
+    store = host_sanitizer.run("cache-store", p, "k")
+    assert "rc=" in look and "rc=" in store
+    if text is None or (text.startswith("{\"format\": \"cellflow-tuning\", \"version\"")):
+        assert "rc=0" in store
+        assert "rc=0 found=1" in host_sanitizer.run("cache-lookup", p, "k")
