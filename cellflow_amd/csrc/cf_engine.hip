@@ -221,7 +221,7 @@ struct Engine {
     }
     dfree(b.tpos); dfree(b.tvel); dfree(b.tid); dfree(b.key); dfree(b.arrival);
     dfree(b.hist); dfree(b.tmp_id); dfree(b.acc); dfree(b.pad);
-    dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.eoff);
+    dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.fcount); dfree(b.eoff);
     dfree(b.edges); dfree(b.sedges); dfree(b.stotal); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
     dfree(slot_of); dfree(tickets); dfree(tris); dfree(vtx); dfree(vcol); dfree(iobuf);
     tris_cap = 0;
@@ -272,7 +272,7 @@ struct Engine {
     CFR(dalloc(&b.tpos, P)); CFR(dalloc(&b.tvel, P)); CFR(dalloc(&b.tid, P));
     CFR(dalloc(&b.key, P)); CFR(dalloc(&b.arrival, P)); CFR(dalloc(&b.tmp_id, P));
     CFR(dalloc(&b.acc, P)); CFR(dalloc(&b.pad, P));
-    CFR(dalloc(&b.ecount, P + 1)); CFR(dalloc(&b.eoff, P + 1));
+    CFR(dalloc(&b.ecount, P + 1)); CFR(dalloc(&b.eoff, P + 1)); CFR(dalloc(&b.fcount, P + 1));
     CFR(dalloc(&b.sims, nsims)); CFR(dalloc(&b.wave_sim, nwaves));
     CFR(dalloc(&b.pairs, (size_t)nsims * kMaxTypes * kMaxTypes));
     CFR(dalloc(&b.flag, 4)); CFR(dalloc(&tickets, 3)); CFR(dalloc(&slot_of, P));
@@ -282,6 +282,7 @@ struct Engine {
     CFR(dalloc(&farn, 2)); CFR(dalloc(&far_slot, P));
     HIPR(hipMemsetAsync(far_slot, 0, P * sizeof(uint32_t), st));
     HIPR(hipMemsetAsync(b.ecount, 0, (P + 1) * sizeof(uint32_t), st));
+    HIPR(hipMemsetAsync(b.fcount, 0, (P + 1) * sizeof(uint32_t), st));
     HIPR(hipMemsetAsync(b.tpos, 0, P * sizeof(float4), st));
     HIPR(hipMemsetAsync(b.tvel, 0, P * sizeof(float4), st));
     HIPR(hipMemsetAsync(b.acc, 0, P * sizeof(float4), st));
@@ -496,7 +497,9 @@ struct Engine {
     const int tc = (int)((C + kScanTile - 1) / kScanTile);
     const int te = (int)((P + kScanTile - 1) / kScanTile);
     ScanSeg cells{b.hist, cell_out, (int)C, tc};
-    ScanSeg edges{b.ecount, b.eoff, (int)P, te};
+    // the step's fused graph (plan 1) counts into fcount; the standalone pass and the mesh
+    // (plan 2) into ecount
+    ScanSeg edges{which == 1 ? b.fcount : b.ecount, b.eoff, (int)P, te};
     if (which == 0) {
       sp.seg[0] = cells;
       sp.nseg = 1;
@@ -619,7 +622,7 @@ struct Engine {
 
   // ---- the step ----------------------------------------------------------------------
 
-  ForceArgs force_args(bool edges, int cap) {
+  ForceArgs force_args(bool edges, int cap, bool write_acc = true) {
     const int A = cur;
     ForceArgs fa{};
     fa.pos = b.pos[A];
@@ -631,7 +634,7 @@ struct Engine {
     fa.key = b.key;
     fa.arrival = b.arrival;
     fa.hist = b.hist;
-    fa.acc = b.acc;
+    fa.acc = write_acc ? b.acc : nullptr;
     fa.sims = b.sims;
     fa.wave_sim = b.wave_sim;
     fa.pairs = b.pairs;
@@ -639,7 +642,7 @@ struct Engine {
     fa.ecand_d2 = b.ecand_d2;
     fa.ecand_id = b.ecand_id;
     fa.estage = b.estage;
-    fa.ecount = b.ecount;
+    fa.ecount = b.fcount;
     fa.P = P;
     fa.nwaves = nwaves;
     fa.edges = edges ? 1 : 0;
@@ -677,6 +680,7 @@ struct Engine {
       sa.eoff = b.eoff;
       sa.edges = b.sedges;
       sa.etotal = b.stotal;
+      sa.fcount = b.fcount;
     }
     sa.P = P;
     sa.nwaves = nwaves;
@@ -711,7 +715,7 @@ struct Engine {
     ta.flag = b.flag;
     ta.ntiles = ntiles;
     if (edges) {
-      ta.ecount = b.ecount;
+      ta.ecount = b.fcount;
       ta.etcount = etcnt[A];
       ta.etcount_next = etcnt[B];
       ta.estage = b.estage;
@@ -728,10 +732,12 @@ struct Engine {
 
   // Kernel sequence of one step: the force pass, then either the tile sort (which also
   // writes the step's canonical edge list) or the three-pass counting sort; `ev`
-  // (optional) receives an event after each kernel.
-  int step(bool fused_edges, int cap, hipEvent_t* ev = nullptr) {
+  // (optional) receives an event after each kernel.  `write_acc` false leaves the
+  // accelerations unstored: they are output-only (ParticleSimulation.cu:146), so only the
+  // last step of a call stores them.
+  int step(bool fused_edges, int cap, hipEvent_t* ev = nullptr, bool write_acc = true) {
     const int B = 1 - cur;
-    ForceArgs fa = force_args(fused_edges, cap);
+    ForceArgs fa = force_args(fused_edges, cap, write_acc);
     HIPR(launch_force(fa, fused_edges, tun.block_size, tun.force_variant,
                       std::max(1, std::min(2, tun.cell_divisor)), tun.lanes_per_particle, st));
     if (ev) HIPR(hipEventRecord(ev[0], st));
@@ -786,10 +792,11 @@ struct Engine {
 
   // `count` steps (fused edge candidates when `fused`), replaying a captured graph of
   // 2*m steps where possible.  Even graph lengths return the state to the same buffer, so
-  // one graph serves every replay.
+  // one graph serves every replay.  The graph's steps store no accelerations; the last
+  // step is always launched directly and stores them.
   int run_steps(int count, bool fused, int cap) {
     const int span = std::max(2, (tun.steps_per_graph + 1) / 2 * 2);
-    if (tun.steps_per_graph >= 2 && count >= span) {
+    if (tun.steps_per_graph >= 2 && count > span) {
       std::vector<long long> key = {buf_gen, cur, fused ? 1 : 0, cap, span, tun.cell_divisor,
                                     tun.force_variant, tun.lanes_per_particle, tun.block_size,
                                     tun.sort_variant, C, P};
@@ -801,7 +808,7 @@ struct Engine {
         HIPR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const int cur0 = cur;
         int rc = CF_OK;
-        for (int k = 0; k < span && rc == CF_OK; ++k) rc = step(fused, cap);
+        for (int k = 0; k < span && rc == CF_OK; ++k) rc = step(fused, cap, nullptr, false);
         hipError_t ce = hipStreamEndCapture(st, &graph);
         cur = cur0;
         if (rc != CF_OK) {
@@ -814,12 +821,12 @@ struct Engine {
         HIPR(ie);
         graphs.push_back(GraphEntry{key, exec});
       }
-      while (count >= span) {
+      while (count > span) {
         HIPR(hipGraphLaunch(exec, st));
         count -= span;
       }
     }
-    for (int k = 0; k < count; ++k) CFR(step(fused, cap));
+    for (int k = 0; k < count; ++k) CFR(step(fused, cap, nullptr, k == count - 1));
     return CF_OK;
   }
 
@@ -1306,7 +1313,7 @@ struct Engine {
         rc = fail(CF_ERR_DEVICE, "hipEventRecord failed");
         break;
       }
-      rc = step(er.on && fuse && k > 0, cap, e + 1);
+      rc = step(er.on && fuse && k > 0, cap, e + 1, k == nsteps - 1);
       nev[k] = ns;
       if (rc == CF_OK && er.on && !fuse) {
         rc = edge_pass(cap, e + ns + 1);

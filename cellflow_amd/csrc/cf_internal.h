@@ -111,8 +111,11 @@ struct DevBufs {
   float* ecand_d2;      // [cap][P]
   uint32_t* ecand_id;   // [cap][P]
   uint32_t* estage;     // [maxc][P] kept neighbour index by rank
-  uint32_t* ecount;     // [P + 1] kept edges per index
-  uint32_t* eoff;       // [P + 1] exclusive prefix of ecount
+  uint32_t* ecount;     // [P + 1] kept edges per index (standalone pass, mesh)
+  uint32_t* fcount;     // [P + 1] kept edges per index of the graph a step builds inside
+                        // its force pass: zero between steps (the force pass writes the
+                        // non-zero counts, the sort that reads them clears them)
+  uint32_t* eoff;       // [P + 1] exclusive prefix of ecount (or of fcount)
   int2* edges;          // canonical pairs of the standalone edge pass (the current state)
   int2* sedges;         // canonical pairs the step's sort writes (the pre-step state)
   uint32_t* stotal;     // their number
@@ -141,7 +144,7 @@ struct ForceArgs {
   uint32_t* __restrict__ key;
   uint32_t* __restrict__ arrival;
   uint32_t* hist;
-  float4* __restrict__ acc;
+  float4* __restrict__ acc;  // null: not stored (a step whose acceleration nobody reads)
   const SimDesc* __restrict__ sims;
   const uint32_t* __restrict__ wave_sim;
   const PairConst* __restrict__ pairs;
@@ -206,6 +209,7 @@ struct SortArgs {
   const uint32_t* __restrict__ eoff;
   int2* __restrict__ edges;
   uint32_t* etotal;     // k_fill: receives the number of edges written (eoff[P])
+  uint32_t* fcount;     // k_fill: the step's per-owner edge counts, cleared after use
   long P;
   int nwaves;
 };
@@ -239,7 +243,7 @@ struct TileSortArgs {
   uint32_t* flag;                       // bit 1: a cell outgrew the LDS; bit 2: lost particle
   int ntiles;
   // edges (edge tiles; null edges: no edge tiles)
-  const uint32_t* ecount;
+  uint32_t* ecount;                     // per-owner counts: read, then cleared
   const uint32_t* etcount;
   uint32_t* etcount_next;
   const uint32_t* estage;               // [rank][owner index]

@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs a) {
 
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
-  a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  if (a.acc) a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
   if (a.tile_sort) commit_cell<true>(a, sd, p, cy, cz, x, y, z);
   else commit_cell<false>(a, sd, p, cy, cz, x, y, z);
 
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs a) {
     if (do_edges) {
       const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage,
                                 a.P, p, a.tile_sort ? idi : (uint32_t)p);
-      a.ecount[idi] = (uint32_t)e;
+      if (e != 0) a.ecount[idi] = (uint32_t)e;  // zero already (the sorts clear it)
       if (a.tile_sort) count_edges<true>(a, idi, (uint32_t)e);
     }
   }
@@ -663,14 +663,14 @@ __global__ void __launch_bounds__(64) k_force_lds(ForceArgs a) {
   if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(acc.cnt));
-  a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  if (a.acc) a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
   if (a.tile_sort) commit_cell<true>(a, sd, p, cy, cz, x, y, z);
   else commit_cell<false>(a, sd, p, cy, cz, x, y, z);
   if (EDGES) {
     if (do_edges) {
       const int e = edge_finish(acc.ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id,
                                 a.estage, a.P, p, a.tile_sort ? idi : (uint32_t)p);
-      a.ecount[idi] = (uint32_t)e;
+      if (e != 0) a.ecount[idi] = (uint32_t)e;  // zero already (the sorts clear it)
       if (a.tile_sort) count_edges<true>(a, idi, (uint32_t)e);
     }
   }
@@ -1039,7 +1039,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
         e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
                                   TILE ? idi : (uint32_t)p);
       }
-      a.ecount[idi] = e;
+      if (e != 0u) a.ecount[idi] = e;  // zero already (the sorts clear it)
       count_edges<TILE>(a, idi, e);
     }
   }
@@ -1064,7 +1064,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
-  a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  if (a.acc) a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
   commit_cell<TILE>(a, sd, p, cy, cz, x, y, z);
 }
 
@@ -1416,7 +1416,7 @@ k_force_list(ForceArgs a) {
         e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
                                   TILE ? idi : (uint32_t)p);
       }
-      a.ecount[idi] = e;
+      if (e != 0u) a.ecount[idi] = e;  // zero already (the sorts clear it)
       count_edges<TILE>(a, idi, e);
     }
   }
@@ -1441,7 +1441,7 @@ k_force_list(ForceArgs a) {
   if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
   a.tpos[p] = make_float4(x, y, z, pi.w);
   a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
-  a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  if (a.acc) a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
   commit_cell<TILE>(a, sd, p, c.cy, c.cz, x, y, z);
 }
 
@@ -1553,6 +1553,9 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
   uint32_t ec[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) ec[i] = (id0 + i < a.P) ? a.ecount[id0 + i] : 0u;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i)  // the next step's force pass writes only non-zero counts
+    if (ec[i]) a.ecount[id0 + i] = 0u;
   const int p0 = (id0 < a.P) ? a.sims[a.wave_sim[id0 / kWave]].p0 : 0;
   uint32_t part = 0, mine = 0;
   for (int v = tid; v < (u + 1) * kEdgeSub; v += kSortThreads) {
@@ -1965,6 +1968,7 @@ __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
   if (a.edges) {
     const uint32_t off = a.eoff[me];
     const int e = (int)(a.eoff[me + 1] - off);
+    if (e) a.fcount[me] = 0u;  // the next step's force pass writes only non-zero counts
     for (int k = 0; k < e; ++k)
       a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
   }

@@ -160,12 +160,20 @@ def test_mesh_dropped_when_the_state_advances():
 
 
 def _hip():
-    """The HIP runtime through ctypes, for raw allocations and pointer queries in tests."""
+    """The HIP runtime the engine library runs on, through ctypes (dlopen by its soname
+    returns the copy already loaded), for raw allocations in tests.  Torch's own device
+    memory is not used here: torch may bring a second copy of the runtime into the process."""
     import ctypes as C
-    h = C.CDLL("libamdhip64.so")
+    from cellflow_amd import _abi
+    _abi.lib()  # the engine and its runtime are loaded first
+    h = C.CDLL("libamdhip64.so.7")
     h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
     h.hipFree.argtypes = [C.c_void_p]
+    h.hipHostFree.argtypes = [C.c_void_p]
+    h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
     h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipPointerGetAttributes.argtypes = [C.POINTER(_PtrAttr), C.c_void_p]
     return h
 
 
@@ -178,12 +186,10 @@ class _PtrAttr(__import__("ctypes").Structure):
 
 def test_vertices_written_into_device_memory(orc):
     """The device-side vertex writes (the reference maps its GL buffer and the kernels write
-    into it) give the same bytes as the host expansions; too small an output is
-    CF_ERR_CAPACITY, both by the stated capacity and by the allocation behind the pointer,
-    with nothing launched."""
+    into it) give the same bytes as the host expansions, into device memory and into pinned
+    host memory; too small an output is CF_ERR_CAPACITY, both by the stated capacity and by
+    the allocation behind the pointer, with nothing launched."""
     import ctypes as C
-
-    import torch
     st, phys, params = state_of(small_cfg(n=3000, side=0.5))
     sim = make_sim(st, phys)
     sim.simulate(params, 1, edges=(0.03, 6))
@@ -192,35 +198,33 @@ def test_vertices_written_into_device_memory(orc):
     sim.generateTriangleMesh(0.03, 8)
     tverts = sim.getTriangleVertices(colors)
     assert len(lines) > 100 and len(tverts) > 100
+    hip = _hip()
     for ref, write, width in ((lines, sim.writeLineVerticesDevice, 6),
                               (tverts, sim.writeTriangleVerticesDevice, 9)):
         need = ref.size
-        buf = torch.full((need + 64,), -7.0, dtype=torch.float32, device="cuda")
-        vc = write(colors, buf.data_ptr(), need + 64)
-        assert vc == len(ref)
-        got = buf.cpu().numpy()
-        assert got[:need].tobytes() == ref.tobytes()
-        assert np.all(got[need:] == -7.0)  # nothing past the vertices
-        pinned = torch.zeros(need, dtype=torch.float32, pin_memory=True)  # hipHostMalloc
-        assert write(colors, pinned.data_ptr(), need) == len(ref)
-        assert pinned.numpy().tobytes() == ref.tobytes()
-        with pytest.raises(CellFlowError) as e:
-            write(colors, buf.data_ptr(), need - 1)
-        assert e.value.code == _abi.CF_ERR_CAPACITY
-        # a capacity larger than the allocation behind the pointer is caught too
-        hip = _hip()
-        raw = C.c_void_p()
-        assert hip.hipMalloc(C.byref(raw), 4 * need) == 0
+        dev, pin = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(dev), 4 * (need + 64)) == 0
+        assert hip.hipHostMalloc(C.byref(pin), 4 * need, 0) == 0
         try:
+            assert hip.hipMemset(dev, 0x7F, 4 * (need + 64)) == 0
+            assert write(colors, dev.value, need + 64) == len(ref)
+            back = np.zeros(need + 64, np.float32)
+            assert hip.hipMemcpy(back.ctypes.data, dev, 4 * (need + 64), 2) == 0  # to host
+            assert back[:need].tobytes() == ref.tobytes()
+            assert back[need:].view(np.uint32).tolist() == [0x7F7F7F7F] * 64  # nothing past
+            assert write(colors, pin.value, need) == len(ref)
+            pinned = np.ctypeslib.as_array(C.cast(pin, C.POINTER(C.c_float)), (need,))
+            assert pinned.tobytes() == ref.tobytes()
             with pytest.raises(CellFlowError) as e:
-                write(colors, raw.value + 4 * width, need)
+                write(colors, dev.value, need - 1)
             assert e.value.code == _abi.CF_ERR_CAPACITY
-            assert write(colors, raw.value, need) == len(ref)
-            back = np.zeros(need, np.float32)
-            assert hip.hipMemcpy(back.ctypes.data, raw, 4 * need, 2) == 0  # device to host
-            assert back.tobytes() == ref.tobytes()
+            # a capacity beyond the allocation behind the pointer is caught too
+            with pytest.raises(CellFlowError) as e:
+                write(colors, dev.value + 4 * (64 + width), need)
+            assert e.value.code == _abi.CF_ERR_CAPACITY
         finally:
-            hip.hipFree(raw)
+            hip.hipFree(dev)
+            hip.hipHostFree(pin)
     sim.close()
 
 
@@ -235,8 +239,7 @@ def test_device_vertex_write_rejects_pageable_host_memory():
     colors = np.ones((6, 3), np.float32)
     need = sim.getLineVertices(colors).size
     host = np.zeros(need, np.float32)
-    hip = C.CDLL("libamdhip64.so")
-    hip.hipPointerGetAttributes.argtypes = [C.POINTER(_PtrAttr), C.c_void_p]
+    hip = _hip()
     at = _PtrAttr()
     rc = hip.hipPointerGetAttributes(C.byref(at), host.ctypes.data)
     hip.hipGetLastError()  # leave no error from the query for later calls to find
