@@ -475,7 +475,7 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": force_ms,
             "kernel_ms_per_launch": {k: round(v, 5) for k, v in kt.items()},
             "note": "the force kernel runs every step; the standalone edge pass (edges, "
-                    "edge_scan, edge_write) runs once per call, after the last step"}
+                    "edge_emit) runs once per call, after the last step"}
     ctr = read_counters(os.path.join(ROOT, "profiles", "k_force_c2_valu.txt"), kname)
     if ctr and "SQ_INSTS_VALU" in ctr:
         import numpy as np

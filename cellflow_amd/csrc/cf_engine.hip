@@ -176,6 +176,10 @@ struct Engine {
   // parity A and clears parity B for the next step), far list, simulation of each tile
   uint32_t* tcnt[2] = {nullptr, nullptr};
   uint32_t* etcnt[2] = {nullptr, nullptr};
+  // edge-tile counters of the standalone graph pass, in two parities (a pass counts into
+  // one and its edge tiles clear the other for the next pass)
+  uint32_t* etcs[2] = {nullptr, nullptr};
+  int epar = 0;
   uint32_t* farn = nullptr;  // two words
   uint32_t* far_slot = nullptr;
   uint32_t* ftn[2] = {nullptr, nullptr};  // far movers per tile
@@ -222,7 +226,7 @@ struct Engine {
     dfree(b.tpos); dfree(b.tvel); dfree(b.tid); dfree(b.key); dfree(b.arrival);
     dfree(b.hist); dfree(b.tmp_id); dfree(b.acc); dfree(b.pad);
     dfree(b.ecand_d2); dfree(b.ecand_id); dfree(b.estage); dfree(b.ecount); dfree(b.fcount); dfree(b.eoff);
-    dfree(b.edges); dfree(b.sedges); dfree(b.stotal); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
+    dfree(b.edges); dfree(b.sedges); dfree(b.stotal); dfree(b.gtotal); dfree(b.sims); dfree(b.wave_sim); dfree(b.pairs); dfree(b.flag);
     dfree(slot_of); dfree(tickets); dfree(tris); dfree(vtx); dfree(vcol); dfree(iobuf);
     tris_cap = 0;
     tri_count = -1;
@@ -230,6 +234,7 @@ struct Engine {
     for (int i = 0; i < 2; ++i) {
       dfree(tcnt[i]);
       dfree(etcnt[i]);
+      dfree(etcs[i]);
       dfree(ftn[i]);
     }
     dfree(farn); dfree(far_slot); dfree(tile_sim); dfree(ftl);
@@ -277,8 +282,14 @@ struct Engine {
     CFR(dalloc(&b.pairs, (size_t)nsims * kMaxTypes * kMaxTypes));
     CFR(dalloc(&b.flag, 4)); CFR(dalloc(&tickets, 3)); CFR(dalloc(&slot_of, P));
     CFR(dalloc(&b.stotal, 1));
+    CFR(dalloc(&b.gtotal, 1));
     netiles = (int)((P + kEdgeTile - 1) / kEdgeTile);
-    for (int i = 0; i < 2; ++i) CFR(dalloc(&etcnt[i], (size_t)netiles * kEdgeSub * kCtrStride));
+    for (int i = 0; i < 2; ++i) {
+      CFR(dalloc(&etcnt[i], (size_t)netiles * kEdgeSub * kCtrStride));
+      CFR(dalloc(&etcs[i], (size_t)netiles * kEdgeSub * kCtrStride));
+      HIPR(hipMemsetAsync(etcs[i], 0, (size_t)netiles * kEdgeSub * kCtrStride * sizeof(uint32_t),
+                          st));
+    }
     CFR(dalloc(&farn, 2)); CFR(dalloc(&far_slot, P));
     HIPR(hipMemsetAsync(far_slot, 0, P * sizeof(uint32_t), st));
     HIPR(hipMemsetAsync(b.ecount, 0, (P + 1) * sizeof(uint32_t), st));
@@ -757,7 +768,8 @@ struct Engine {
     return CF_OK;
   }
 
-  // Standalone proximity-graph pass on the current state.
+  // Standalone proximity-graph pass on the current state: the candidates (k_edges_g),
+  // then the canonical list by the edge tiles of k_tile_sort (launched without cell tiles).
   int edge_pass(int cap, hipEvent_t* ev = nullptr) {
     EdgeArgs ea{};
     ea.pos = b.pos[cur];
@@ -768,25 +780,31 @@ struct Engine {
     ea.ecand_d2 = b.ecand_d2;
     ea.ecand_id = b.ecand_id;
     ea.estage = b.estage;
-    ea.ecount = b.ecount;
+    ea.ecount = b.fcount;
+    ea.etcount = etcs[epar];
     ea.P = P;
     ea.nwaves = nwaves;
     ea.edge_cap = cap;
     HIPR(launch_edges(ea, st));
     if (ev) HIPR(hipEventRecord(ev[0], st));
-    HIPR(launch_scan(plan(2, nullptr), st));
+    TileSortArgs ta{};
+    ta.sims = b.sims;
+    ta.wave_sim = b.wave_sim;
+    ta.flag = b.flag;
+    ta.ntiles = 0;
+    ta.ecount = b.fcount;
+    ta.etcount = etcs[epar];
+    ta.etcount_next = etcs[1 - epar];
+    ta.estage = b.estage;
+    ta.edges = b.edges;
+    ta.etotal = b.gtotal;
+    ta.edge_room = (long)edge_maxc_alloc * P;
+    ta.netiles = netiles;
+    ta.P = P;
+    ta.one_sim = sims.size() == 1 ? 1 : 0;
+    HIPR(launch_tile_sort(ta, st));
     if (ev) HIPR(hipEventRecord(ev[1], st));
-    SortArgs sa{};
-    sa.src_id = b.id[cur];
-    sa.sims = b.sims;
-    sa.wave_sim = b.wave_sim;
-    sa.estage = b.estage;
-    sa.eoff = b.eoff;
-    sa.edges = b.edges;
-    sa.P = P;
-    sa.nwaves = nwaves;
-    HIPR(launch_edge_write(sa, st));
-    if (ev) HIPR(hipEventRecord(ev[2], st));
+    epar ^= 1;
     return CF_OK;
   }
 
@@ -844,7 +862,7 @@ struct Engine {
   int finish_call(bool edges_built, bool step_edges_built = false) {
     HIPR(hipMemcpyAsync(hsmall, b.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (edges_built)
-      HIPR(hipMemcpyAsync(hsmall + 1, b.eoff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      HIPR(hipMemcpyAsync(hsmall + 1, b.gtotal, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     if (step_edges_built)
       HIPR(hipMemcpyAsync(hsmall + 3, b.stotal, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
@@ -1287,15 +1305,15 @@ struct Engine {
       tri_count = -1;
     }
     step_edges_count = -1;
-    // the step's kernels (ns of them), then the standalone edge pass's three
+    // the step's kernels (ns of them), then the standalone edge pass's two
     const bool tiles = tun.sort_variant == 1;
     const int ns = tiles ? 2 : 4;
-    const char* kt[] = {"force", "sort", "edges", "edge_scan", "edge_write"};
-    const char* kc[] = {"force", "scan", "fill", "rank_scatter", "edges", "edge_scan", "edge_write"};
+    const char* kt[] = {"force", "sort", "edges", "edge_emit"};
+    const char* kc[] = {"force", "scan", "fill", "rank_scatter", "edges", "edge_emit"};
     const char* const* kn = tiles ? kt : kc;
-    const int K = ns + 3;
-    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+    const int K = ns + 2;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int cnt[6] = {0, 0, 0, 0, 0, 0};
     // Every step is queued before any is waited for, each with its own events, so the GPU
     // runs the kernels back to back: an interval holds one kernel and the event marker
     // after it, never a host launch.  The first step's intervals are left out when there
@@ -1317,7 +1335,7 @@ struct Engine {
       nev[k] = ns;
       if (rc == CF_OK && er.on && !fuse) {
         rc = edge_pass(cap, e + ns + 1);
-        nev[k] = ns + 3;
+        nev[k] = ns + 2;
       }
     }
     hipEvent_t* fe = &ev[(size_t)nsteps * per];  // the final standalone edge pass (fused case)
@@ -1338,7 +1356,7 @@ struct Engine {
     for (int k = (nsteps > 1 ? 1 : 0); k < nsteps && rc == CF_OK; ++k)
       for (int i = 0; i < nev[k] && rc == CF_OK; ++i) rc = add(&ev[(size_t)k * per], i);
     if (rc == CF_OK && er.on && fuse)
-      for (int i = ns; i < ns + 3 && rc == CF_OK; ++i) rc = add(fe, i);
+      for (int i = ns; i < ns + 2 && rc == CF_OK; ++i) rc = add(fe, i);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (rc != CF_OK) return rc;

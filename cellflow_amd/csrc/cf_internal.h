@@ -117,6 +117,7 @@ struct DevBufs {
                         // non-zero counts, the sort that reads them clears them)
   uint32_t* eoff;       // [P + 1] exclusive prefix of ecount (or of fcount)
   int2* edges;          // canonical pairs of the standalone edge pass (the current state)
+  uint32_t* gtotal;     // number of pairs in `edges`
   int2* sedges;         // canonical pairs the step's sort writes (the pre-step state)
   uint32_t* stotal;     // their number
   // descriptors
@@ -175,16 +176,20 @@ struct ForceArgs {
   uint32_t* nhits;
 };
 
+// The standalone proximity-graph pass (k_edges_g); k_tile_sort's edge tiles then write the
+// canonical list from estage, ecount and etcount.
 struct EdgeArgs {
   const float4* __restrict__ pos;
   const uint32_t* __restrict__ id;
   const uint32_t* __restrict__ start;
   const SimDesc* __restrict__ sims;
   const uint32_t* __restrict__ wave_sim;
-  float* ecand_d2;
+  float* ecand_d2;      // [cap][P] by slot: scratch of the streaming selection
   uint32_t* ecand_id;
-  uint32_t* estage;
-  uint32_t* ecount;
+  uint32_t* estage;     // [maxc][P] kept partners by owner index and rank
+  uint32_t* ecount;     // kept partners per owner index: non-zero counts only (zero between
+                        // passes: the edge tiles clear what they read)
+  uint32_t* etcount;    // per edge tile, in kEdgeSub parts (as ForceArgs::etcount)
   long P;
   int nwaves;
   int edge_cap;
@@ -407,7 +412,6 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
                         int lanes, hipStream_t st);
 hipError_t launch_edges(const EdgeArgs& a, hipStream_t st);
 hipError_t launch_tile_sort(const TileSortArgs& a, hipStream_t st);
-hipError_t launch_edge_write(const SortArgs& a, hipStream_t st);
 hipError_t launch_scan(const ScanPlan& p, hipStream_t st);
 hipError_t launch_fill(const SortArgs& a, hipStream_t st);
 hipError_t launch_rank_scatter(const SortArgs& a, hipStream_t st);

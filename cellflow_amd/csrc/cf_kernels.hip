@@ -1446,51 +1446,191 @@ k_force_list(ForceArgs a) {
 }
 
 // ------------------------------------------------------------------------------------
-// k_edges: standalone proximity-graph candidates on the current sorted state, with a
-// non-periodic stencil of per-axis reach (the edge distance may exceed the cell size).
+// k_edges_g: the standalone proximity-graph pass (the state after a call's last step, or
+// every step when the edge distance reaches beyond the force stencil).  G lanes per owner
+// take the rows of the owner's non-periodic stencil (ParticleSimulation.cu:222-227: no
+// minimum image) of per-axis reach edge_r*; rows and row ends farther from the owner than
+// the edge distance are dropped (the same gap test as the force stencil's pruning, with its
+// margins), which leaves about a fifth of the cells at the 100k workload's 0.02 over a
+// 0.052 cell.  Qualifying candidates (same type :219, larger index :215, d^2 < D^2 :227) go
+// to the owner's LDS list; the two-phase selection then keeps the 2*maxC smallest indices
+// and ranks them by (d^2, index) (:228-246).  An owner with more than kListEdges candidates
+// takes the exact streaming selection of one lane.  Kept partners are staged by owner index
+// (estage[rank][index]), the non-zero per-owner counts go to `ecount` and to the edge-tile
+// counters, and k_tile_sort's edge tiles write the canonical list.
 
+constexpr int kEdgeLanes = 4;
 
-__global__ void __launch_bounds__(256) k_edges(EdgeArgs a) {
+// Row r of an owner's non-periodic edge stencil: false when it lies outside the grid or
+// farther than the pruning radius; else its cell range [cell0, cell1] (one row, x-adjacent
+// cells, so one run of slots from start[cell0] to start[cell1 + 1]).
+__device__ __forceinline__ bool edge_row(const SimDesc& sd, int cx, int cy, int cz, float fxr,
+                                         float fyr, float fzr, float cut2, int r, int& cell0,
+                                         int& cell1) {
+  const int nyr = 2 * sd.edge_ry + 1;
+  const int dy = r % nyr - sd.edge_ry, dz = r / nyr - sd.edge_rz;
+  const int y = cy + dy, z = cz + dz;
+  if (y < 0 || y >= sd.ny || z < 0 || z >= sd.nz) return false;
+  const float gy = cell_gap(dy, fyr) * sd.hy, gz = cell_gap(dz, fzr) * sd.hz;
+  const float gyz = fmaf(gy, gy, gz * gz);
+  if (!(gyz < cut2)) return false;
+  int xlo = max(cx - sd.edge_rx, 0), xhi = min(cx + sd.edge_rx, sd.nx - 1);
+  while (xlo < cx) {
+    const float gx = cell_gap(xlo - cx, fxr) * sd.hx;
+    if (fmaf(gx, gx, gyz) < cut2) break;
+    ++xlo;
+  }
+  while (xhi > cx) {
+    const float gx = cell_gap(xhi - cx, fxr) * sd.hx;
+    if (fmaf(gx, gx, gyz) < cut2) break;
+    --xhi;
+  }
+  const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+  cell0 = rowc + xlo;
+  cell1 = rowc + xhi;
+  return true;
+}
+
+// Runs a lane's stencil rows r = r0, r0 + G, ... (kEdgeRowBatch of them at a time, their
+// cell-start loads issued together) and calls f(q, pj) for each slot q, positions loaded
+// kEdgeLoadBatch at a time.
+constexpr int kEdgeRowBatch = 3;
+constexpr int kEdgeLoadBatch = 4;
+
+template <class F>
+__device__ __forceinline__ void edge_walk(const EdgeArgs& a, const SimDesc& sd, int cx, int cy,
+                                          int cz, float fxr, float fyr, float fzr, float cut2,
+                                          int r0, int step, int nrows, F&& f) {
+  for (int rb = r0; rb < nrows; rb += kEdgeRowBatch * step) {
+    int c0[kEdgeRowBatch], c1[kEdgeRowBatch];
+    bool on[kEdgeRowBatch];
+#pragma unroll
+    for (int i = 0; i < kEdgeRowBatch; ++i) {
+      const int r = rb + i * step;
+      on[i] = r < nrows && edge_row(sd, cx, cy, cz, fxr, fyr, fzr, cut2, r, c0[i], c1[i]);
+    }
+    int qa[kEdgeRowBatch], qb[kEdgeRowBatch];
+#pragma unroll
+    for (int i = 0; i < kEdgeRowBatch; ++i) {
+      qa[i] = on[i] ? (int)a.start[c0[i]] : 0;
+      qb[i] = on[i] ? (int)a.start[c1[i] + 1] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kEdgeRowBatch; ++i)
+      for (int q = qa[i]; q < qb[i]; q += kEdgeLoadBatch) {
+        float4 pj[kEdgeLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kEdgeLoadBatch; ++u)
+          pj[u] = (q + u < qb[i]) ? a.pos[q + u] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+#pragma unroll
+        for (int u = 0; u < kEdgeLoadBatch; ++u)
+          if (q + u < qb[i]) f(q + u, pj[u]);
+      }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_edges_g(EdgeArgs a) {
+  constexpr int G = kEdgeLanes;
+  constexpr int kOwn = kWave / G;
+  __shared__ float2 el_d[4][kOwn][kListEdges];
+  __shared__ uint32_t el_i[4][kOwn][kListEdges];
+  __shared__ uint32_t el_n[4][kOwn];
   const int lane = threadIdx.x & (kWave - 1);
-  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (gw >= a.nwaves) return;
-  const SimDesc sd = a.sims[a.wave_sim[gw]];
-  const int p = gw * kWave + lane;
-  if (p - sd.p0 >= sd.n) return;
-  const float4 pi = a.pos[p];
-  const uint32_t idi = a.id[p];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gw = xcd_block() * (blockDim.x >> 6) + wv;
+  const long base = (long)gw * kOwn;
+  const bool wave_ok = base < (long)a.nwaves * kWave;
+  const int o = lane / G, g = lane % G;
+  const int p = (int)base + o;
+  const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
+  const SimDesc sd = a.sims[s];
+  const bool active = wave_ok && (p - sd.p0) < sd.n;
+  float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t idi = 0;
+  if (active) {
+    pi = a.pos[p];
+    idi = a.id[p];
+  }
   const uint32_t ti = __float_as_uint(pi.w);
   const int cx = bin_axis(pi.x, sd.ihx, sd.nx);
   const int cy = bin_axis(pi.y, sd.ihy, sd.ny);
   const int cz = bin_axis(pi.z, sd.ihz, sd.nz);
-  const int xlo = max(cx - sd.edge_rx, 0), xhi = min(cx + sd.edge_rx, sd.nx - 1);
-  const int ylo = max(cy - sd.edge_ry, 0), yhi = min(cy + sd.edge_ry, sd.ny - 1);
-  const int zlo = max(cz - sd.edge_rz, 0), zhi = min(cz + sd.edge_rz, sd.nz - 1);
-  EdgeAcc ea;
-  ea.q = 0;
-  ea.maxid = 0;
-  ea.maxk = 0;
-  for (int z = zlo; z <= zhi; ++z)
-    for (int y = ylo; y <= yhi; ++y) {
-      const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
-      const int q0 = (int)a.start[rowc + xlo];
-      const int q1 = (int)a.start[rowc + xhi + 1];
-      for (int q = q0; q < q1; ++q) {
-        const float4 pj = a.pos[q];
-        if (__float_as_uint(pj.w) != ti) continue;       // :219
-        const float dx = pj.x - pi.x;                     // :222-225, non-periodic
-        const float dy = pj.y - pi.y;
-        const float dz = pj.z - pi.z;
-        const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        if (!(s2 < sd.edge_d2)) continue;                 // :227
-        const uint32_t jid = a.id[q];
-        if (jid <= idi) continue;                         // j > i, :215
-        edge_push(ea, s2, jid, a.edge_cap, a.ecand_d2, a.ecand_id, a.P, p);
-      }
+  const float fxr = fminf(fmaxf(pi.x * sd.ihx - (float)cx, 0.0f), 1.0f);
+  const float fyr = fminf(fmaxf(pi.y * sd.ihy - (float)cy, 0.0f), 1.0f);
+  const float fzr = fminf(fmaxf(pi.z * sd.ihz - (float)cz, 0.0f), 1.0f);
+  // pruning radius: the edge distance with the force stencil's margins
+  const float cut2 = sd.edge_d2 * 1.001f + 1e-12f;
+  const int nrows = (2 * sd.edge_ry + 1) * (2 * sd.edge_rz + 1);
+  if (g == 0) el_n[wv][o] = 0u;
+  wave_lds_sync();
+  if (active)
+    edge_walk(a, sd, cx, cy, cz, fxr, fyr, fzr, cut2, g, G, nrows,
+              [&](int q, const float4& pj) {
+                if (__float_as_uint(pj.w) != ti) return;        // :219
+                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (!(s2 < sd.edge_d2)) return;                  // :227
+                const uint32_t jid = a.id[q];
+                if (jid <= idi) return;                          // j > i, :215
+                const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
+                if (kk < (uint32_t)kListEdges) {
+                  el_d[wv][o][kk] = make_float2(s2, 0.0f);
+                  el_i[wv][o][kk] = jid;
+                }
+              });
+  wave_lds_sync();
+  const uint32_t n = active ? el_n[wv][o] : 0u;
+  const bool fits = n <= (uint32_t)kListEdges;
+  const int cap = a.edge_cap, maxc = sd.edge_maxc;
+  // phase 1: keep the `cap` smallest indices (the reference's first 2*maxC by index, :215)
+  if (fits)
+    for (uint32_t k = g; k < n; k += G) {
+      const uint32_t ik = el_i[wv][o][k];
+      uint32_t below = 0;
+      for (uint32_t k2 = 0; k2 < n; ++k2) below += (el_i[wv][o][k2] < ik) ? 1u : 0u;
+      el_d[wv][o][k].y = (below < (uint32_t)cap) ? 1.0f : 0.0f;
     }
-  const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
-                            (uint32_t)p);
-  a.ecount[idi] = (uint32_t)e;
+  wave_lds_sync();
+  // phase 2: rank the kept ones by (d^2, index), keep the first maxC (:235-246)
+  if (fits)
+    for (uint32_t k = g; k < n; k += G) {
+      const float2 dk = el_d[wv][o][k];
+      if (dk.y == 0.0f) continue;
+      const uint32_t ik = el_i[wv][o][k];
+      uint32_t rank = 0;
+      for (uint32_t k2 = 0; k2 < n; ++k2) {
+        const float2 d2 = el_d[wv][o][k2];
+        const uint32_t i2 = el_i[wv][o][k2];
+        rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
+      }
+      if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + idi] = ik;
+    }
+  if (!active || g != 0) return;
+  uint32_t e;
+  if (fits) {
+    e = (uint32_t)min(min((int)n, cap), maxc);
+  } else {
+    // more candidates than the LDS list holds: exact streaming selection by this lane
+    EdgeAcc ea;
+    ea.q = 0;
+    ea.maxid = 0;
+    ea.maxk = 0;
+    edge_walk(a, sd, cx, cy, cz, fxr, fyr, fzr, cut2, 0, 1, nrows,
+              [&](int q, const float4& pj) {
+                if (__float_as_uint(pj.w) != ti) return;
+                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (!(s2 < sd.edge_d2)) return;
+                const uint32_t jid = a.id[q];
+                if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
+              });
+    e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
+  }
+  if (e) {  // zero already: the edge tiles that read the counts clear them
+    a.ecount[idi] = e;
+    atomicAdd(&a.etcount[((idi / kEdgeTile) * kEdgeSub + (blockIdx.x & (kEdgeSub - 1))) *
+                         kCtrStride], e);
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -2003,21 +2143,6 @@ __global__ void __launch_bounds__(256) k_rank_scatter(SortArgs a) {
   a.dst_id[dst] = me;
 }
 
-// Canonical edge list of the standalone edge pass: owner index ascending (offsets from
-// the scan of per-owner counts), then distance rank.
-__global__ void __launch_bounds__(256) k_edge_write(SortArgs a) {
-  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (gw >= a.nwaves) return;
-  const SimDesc sd = a.sims[a.wave_sim[gw]];
-  const int p = gw * kWave + (threadIdx.x & (kWave - 1));
-  if (p - sd.p0 >= sd.n) return;
-  const uint32_t me = a.src_id[p];
-  const uint32_t off = a.eoff[me];
-  const int e = (int)(a.eoff[me + 1] - off);
-  for (int k = 0; k < e; ++k)
-    a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
-}
-
 // ------------------------------------------------------------------------------------
 // binning of a given order (load, grid change)
 
@@ -2414,7 +2539,8 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
 }
 
 hipError_t launch_edges(const EdgeArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_edges, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  // kWave / kEdgeLanes owners per wave, four waves per block: a.nwaves blocks
+  hipLaunchKernelGGL(k_edges_g, dim3(a.nwaves), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
@@ -2436,11 +2562,6 @@ hipError_t launch_fill(const SortArgs& a, hipStream_t st) {
 
 hipError_t launch_rank_scatter(const SortArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_rank_scatter, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_edge_write(const SortArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_edge_write, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -407,7 +407,7 @@ def test_profile_steps_times_each_kernel_and_steps_the_state():
     st, phys, params = state_of(small_cfg(n=4000, side=0.5))
     a, b = make_sim(st, phys), make_sim(st, phys)
     t = a.profileSteps(params, 6, edges=(0.02, 8))
-    assert list(t) == ["force", "sort", "edges", "edge_scan", "edge_write"]
+    assert list(t) == ["force", "sort", "edges", "edge_emit"]
     assert all(v > 0.0 for v in t.values())
     b.simulate(params, 6, edges=(0.02, 8))
     assert a.getParticleData().tobytes() == b.getParticleData().tobytes()
