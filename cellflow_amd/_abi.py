@@ -121,6 +121,7 @@ SIGNATURES = {
     "cf_profile_steps": (_i, [_vp, P(cf_params), _i, P(cf_edge_opts), P(C.c_double), _i, P(_i),
                               C.c_char_p, _i]),
     "cf_pair_histogram": (_i, [_vp, _i, _f, P(C.c_double)]),
+    "cf_pair_histogram_noalloc": (_i, [_vp, _i, _f, P(C.c_double)]),
     "cf_rollout_grad": (_i, [_vp, P(cf_params), _i, _i, _i, _f, P(C.c_double), P(C.c_double),
                              P(C.c_double), P(C.c_double), P(C.c_double)]),
     "cf_set_tuning": (_i, [_vp, P(cf_tuning)]),

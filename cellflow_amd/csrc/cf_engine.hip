@@ -98,6 +98,49 @@ int grid_dims(const cf_params& p, const float* rbt, int T, int n, int divisor, i
 }
 
 // ------------------------------------------------------------------------------------
+// scratch of the allocation-free pair histogram (cf_pair_histogram_noalloc)
+//
+// A fixed device array instead of memory allocated per call: the per-particle partial
+// histograms (bins x stride doubles) and the reduced histogram (bins doubles).  Every GPU
+// the library runs on holds its own copy until the process ends.  In host code the name
+// `scratch` is not a device address: its address on each device comes from
+// hipGetSymbolAddress, taken once per device when the first engine is created there, and
+// only that address is passed to kernels.  Calls that overlap on one GPU share the array
+// and race; nothing in it must start at zero (the kernels overwrite what they read).
+constexpr size_t kScratchBytes = size_t(32) << 20;  // 32 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kScratchDevices] = {};
+
+// The address of `scratch` on `device`, taken with that device current; the caller's
+// current device is restored.
+static int scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices)
+    return fail(CF_ERR_DEVICE, "device %d: scratch is set up for devices 0..%d", device,
+                kScratchDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return CF_OK;
+  int prev = 0;
+  HIPR(hipGetDevice(&prev));
+  if (prev != device) HIPR(hipSetDevice(device));
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (prev != device) (void)hipSetDevice(prev);
+  if (e != hipSuccess || !p)
+    return fail(CF_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return CF_OK;
+}
+
+static void* scratch_ptr(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+// ------------------------------------------------------------------------------------
 // engine
 
 struct SimHost {
@@ -253,6 +296,7 @@ struct Engine {
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev <= 0) return fail(CF_ERR_DEVICE, "no HIP device available");
     HIPR(hipGetDevice(&device));
+    CFR(scratch_setup(device));
     HIPR(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPR(hipHostMalloc((void**)&hsmall, 64 * sizeof(uint32_t), hipHostMallocDefault));
     sims.assign(nsims, SimHost{});
@@ -1538,6 +1582,34 @@ struct Engine {
     return CF_OK;
   }
 
+  // histogram() with its partial sums and result in the device's `scratch` array: no
+  // allocation.  The state must already be binned for its parameters (a load since the last
+  // step would re-bin, which may allocate), and the partials must fit the array.
+  int histogram_noalloc(int bins, double r_max, double* hist_out) {
+    if (sims.size() != 1) return fail(CF_ERR_STATE, "histograms are defined for one simulation");
+    const int n = sims[0].n;
+    const long stride = ((long)n + 127) / 128 * 128;
+    const size_t need = ((size_t)bins * (size_t)stride + (size_t)bins) * sizeof(double);
+    if (need > kScratchBytes)
+      return fail(CF_ERR_ARG, "%d particles x %d bins need %zu bytes of scratch; the "
+                  "allocation-free histogram holds at most %zu", n, bins, need, kScratchBytes);
+    if (!sims[0].have_prm || load_pending)
+      return fail(CF_ERR_STATE, "the state is not binned: step it (cf_simulate, 0 steps "
+                                "bins it) after loading, before the allocation-free histogram");
+    double* partial = static_cast<double*>(scratch_ptr(device));
+    if (!partial) return fail(CF_ERR_DEVICE, "no scratch on device %d", device);
+    double* hd = partial + (size_t)bins * stride;
+    StateBuf s;
+    s.pos = b.pos[cur];
+    s.id = b.id[cur];
+    s.start = b.start[cur];
+    HIPR(launch_hist(hist_args(s, bins, r_max, partial, stride), n, st));
+    HIPR(launch_reduce_rows(partial, stride, n, bins, 1.0 / n, hd, 0, st));
+    HIPR(hipMemcpyAsync(hist_out, hd, bins * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    return CF_OK;
+  }
+
   // Gradient of the final-state histogram loss after `nsteps` steps with respect to the
   // force table and the ratio, by the adjoint method with checkpoints every `ckpt` steps.
   int rollout_grad(int nsteps, int ckpt, int bins, double r_max, const double* target,
@@ -2465,6 +2537,13 @@ int cf_pair_histogram(cf_sim* sim, int bins, float r_max, double* hist) {
   if (!hist || bins < 1 || bins > cf::kMaxBins || !(r_max > 0.0f))
     return cf::fail(CF_ERR_ARG, "bins must be 1..%d and r_max > 0", cf::kMaxBins);
   return sim->e->histogram(bins, r_max, hist);
+}
+
+int cf_pair_histogram_noalloc(cf_sim* sim, int bins, float r_max, double* hist) {
+  CFR(check(sim));
+  if (!hist || bins < 1 || bins > cf::kMaxBins || !(r_max > 0.0f))
+    return cf::fail(CF_ERR_ARG, "bins must be 1..%d and r_max > 0", cf::kMaxBins);
+  return sim->e->histogram_noalloc(bins, r_max, hist);
 }
 
 int cf_rollout_grad(cf_sim* sim, const cf_params* params, int nsteps, int checkpoint_every,

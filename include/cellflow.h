@@ -248,6 +248,15 @@ int cf_profile_steps(cf_sim* sim, const cf_params* params, int nsteps, const cf_
  * w = r_max / bins, c_b = (b + 1/2) w, B3 the cubic B-spline on [-2, 2], r_ij the minimum-
  * image distance.  bins <= 64.  FP64, deterministic. */
 int cf_pair_histogram(cf_sim* sim, int bins, float r_max, double* hist);
+/* cf_pair_histogram without allocating device memory: no counterpart in the reference.
+ * For callers that cannot allocate during their loop (serving loops, harnesses that forbid
+ * allocation).  The partial sums live in one 32 MiB device array per GPU, held until the
+ * process ends: bins * (particles rounded up to 128) + bins doubles must fit in it
+ * (CF_ERR_ARG otherwise, before anything runs; about 131,000 particles at 32 bins).  The
+ * state must be binned: after loading particles, step once (cf_simulate with 0 steps bins
+ * without stepping), else CF_ERR_STATE.  Same result as cf_pair_histogram, bit for bit.
+ * Calls that overlap on one GPU share the array and race. */
+int cf_pair_histogram_noalloc(cf_sim* sim, int bins, float r_max, double* hist);
 /* Runs `nsteps` steps from the current state (leaving the simulation at the final state)
  * and returns the loss L = sum_b (h_b - target_b)^2 of the final histogram and its gradient
  * with respect to the num_types^2 force-table entries (row-major) and ratio_with_lfo, by
