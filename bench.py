@@ -413,6 +413,10 @@ def main():
     ap.add_argument("--no-clustered", action="store_true", help="skip the 1M clustered leg")
     ap.add_argument("--no-frame", action="store_true", help="skip the host-inclusive frame leg")
     ap.add_argument("--no-mesh", action="store_true", help="skip the triangle-mesh/init leg")
+    ap.add_argument("--tuning", default="",
+                    help="force_variant,lanes,cell_divisor,sort_variant: use this tuning instead "
+                         "of autotuning (for profiling runs whose counters should hold the "
+                         "timed steps only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -449,7 +453,13 @@ def main():
     sim.setForceTable(st["table"])
     sim.setRadioByType(st["rbt"])
     sim.setParticles(particles_from_state(st["pos"], st["vel"], st["type"]))
-    tuned = sim.autotune(params, edges=edges, trial_steps=64)
+    if args.tuning:
+        v, g, d, so = (int(x) for x in args.tuning.split(","))
+        sim.setTuning(cell_divisor=d, force_variant=v, lanes_per_particle=g, sort_variant=so)
+        tuned = {"tuning": sim.getTuning(), "key": sim.tuningKey(params, edges),
+                 "us_per_step": float("nan"), "trials": []}
+    else:
+        tuned = sim.autotune(params, edges=edges, trial_steps=64)
     log(f"[bench] tuning {tuned['tuning']} ({tuned['us_per_step']:.1f} us/step in trials)")
     log(f"[bench] warmup {args.warmup} steps")
     sim.simulate(params, args.warmup, edges=edges)

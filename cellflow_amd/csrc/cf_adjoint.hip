@@ -4,8 +4,10 @@
 // within the cutoff, the neighbour counts and hence the density factor) are frozen: the
 // adjoint differentiates the smooth force and integration of ParticleSimulation.cu:116-161
 // with respect to the positions, velocities, the force table and the ratio.  Decisions
-// are recomputed bit for bit from the stored FP32 states with the same FP32 predicate; all
-// adjoint arithmetic is FP64.
+// are recomputed bit for bit from the stored FP32 states with the same FP32 predicate.  The
+// continuous state the loss and the adjoint are evaluated on is an FP64 shadow of the
+// rollout along those decisions (k_shadow_step), so the gradient is exactly that of one
+// FP64 function; all adjoint arithmetic is FP64.
 //
 // One forward step for particle i (pre-step state x, v; counts c, prev frozen):
 //   F_i = sum_j f_ij,  f_ij = fv_ij g(r_ij) d_ij / dist_ij,  d_ij = minimage(x_j - x_i),
@@ -84,6 +86,13 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ 
 
 // ---- one backward step -----------------------------------------------------------------
 
+// The FP64 density factor of a frozen count (the oracle's op order).
+__device__ __forceinline__ double shadow_fac(const AdjConst& c, int cnt, int prev) {
+  const double avg = (double)(cnt + prev) * 0.5;
+  const double dens = fmin(avg / c.max_exp, 1.0);
+  return 1.0 - (1.0 - c.balance) * dens;
+}
+
 // phi_i = dt m_i (lv'_i + dt lx'_i),  lv_i = fr (lv'_i + dt lx'_i) of particle k; phi is
 // stored by the pre-step slot, where the pair kernel finds both ends of a pair without an
 // index lookup.  The walk kernel computes it on the side (it needs nothing the walk makes).
@@ -92,13 +101,10 @@ __device__ __forceinline__ void adj_phi(const AdjArgs& a, const SimDesc& sd, int
   const uint32_t id = a.id[p];
   const int prev = __float_as_int(a.vel[p].w);
   const int cnt = __float_as_int(a.tvel[p].w);  // this step's count, by pre-step slot
-  // the forward's FP32 factor (ParticleSimulation.cu:136-143)
-  const float avg = (float)(cnt + prev) * 0.5f;
-  const float dens = fminf(avg / sd.max_exp, 1.0f);
-  const float fac = 1.0f - sd.one_minus_balance * dens;
-  const double m = (double)(sd.fm * fac);
+  // the shadow's FP64 factor of the frozen counts (ParticleSimulation.cu:136-143)
+  const double m = a.c.fm * shadow_fac(a.c, cnt, prev);
   const double4 lx = a.lam_x_next[id], lv = a.lam_v_next[id];
-  const double dt = (double)sd.dt, fr = (double)sd.friction;
+  const double dt = a.c.dt, fr = a.c.friction;
   const double mx = lv.x + dt * lx.x, my = lv.y + dt * lx.y, mz = lv.z + dt * lx.z;
   a.phi[p] = make_double4(dt * m * mx, dt * m * my, dt * m * mz, 0.0);  // by pre-step slot
   a.lam_v[id] = make_double4(fr * mx, fr * my, fr * mz, 0.0);
@@ -163,6 +169,13 @@ __device__ __forceinline__ void for_cells_split(const SimDesc& sd, const uint32_
 constexpr int kHistG = 32;
 constexpr int kHistBlock = 256;
 
+// The FP64 position of slot q for the histogram: the shadow's when given, else the FP32
+// state's.
+__device__ __forceinline__ double4 hist_pos(const HistArgs& a, int q, const float4& x) {
+  if (a.x64) return a.x64[a.id[q]];
+  return make_double4((double)x.x, (double)x.y, (double)x.z, 0.0);
+}
+
 __global__ void __launch_bounds__(kHistBlock) k_hist_g(HistArgs a) {
   constexpr int G = kHistG, kPer = kHistBlock / G;
   __shared__ unsigned long long hq[kPer * kMaxBins];
@@ -174,10 +187,11 @@ __global__ void __launch_bounds__(kHistBlock) k_hist_g(HistArgs a) {
   unsigned long long* h = &hq[o * kMaxBins];
   const double unit = ldexp(1.0, 62 - (32 - __clz(sd.n)));
   const double w = a.r_max / a.bins, reach = a.r_max + 2.0 * w;
-  const float reach2f = (float)(reach * reach * (1.0 + 1e-5));
+  const float reach2f = (float)((reach + a.slack) * (reach + a.slack) * (1.0 + 1e-5));
   if (k < sd.n) {
     const int p = sd.p0 + k;
     const float4 pi = a.pos[p];
+    const double4 xi = hist_pos(a, p, pi);
     const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
               cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
     for_cells_split(sd, a.start, cx, cy, cz, a.reach, g, G, [&](uint32_t q) {
@@ -186,9 +200,10 @@ __global__ void __launch_bounds__(kHistBlock) k_hist_g(HistArgs a) {
       const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
                   fz = min_image_f(pj.z - pi.z, sd.Lz);
       if (fx * fx + fy * fy + fz * fz > reach2f) return;
-      const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
-      const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
-      const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
+      const double4 xj = hist_pos(a, q, pj);
+      const double dx = min_image_d(xj.x - xi.x, sd.Lx);
+      const double dy = min_image_d(xj.y - xi.y, sd.Ly);
+      const double dz = min_image_d(xj.z - xi.z, sd.Lz);
       const double r = sqrt(dx * dx + dy * dy + dz * dz);
       if (r >= reach) return;
       const double u = r / w - 0.5;
@@ -210,12 +225,13 @@ __global__ void __launch_bounds__(128) k_hist_grad_g(HistArgs a) {
   const int k = blockIdx.x * (128 / G) + o;
   const bool active = k < sd.n;
   const double w = a.r_max / a.bins, reach = a.r_max + 2.0 * w;
-  const float reach2f = (float)(reach * reach * (1.0 + 1e-5));
+  const float reach2f = (float)((reach + a.slack) * (reach + a.slack) * (1.0 + 1e-5));
   double gx = 0.0, gy = 0.0, gz = 0.0;
   const int p = sd.p0 + k;
   float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (active) {
     pi = a.pos[p];
+    const double4 xi = hist_pos(a, p, pi);
     const int cx = bin_axis_a(pi.x, sd.ihx, sd.nx), cy = bin_axis_a(pi.y, sd.ihy, sd.ny),
               cz = bin_axis_a(pi.z, sd.ihz, sd.nz);
     for_cells_split(sd, a.start, cx, cy, cz, a.reach, g, G, [&](uint32_t q) {
@@ -224,9 +240,10 @@ __global__ void __launch_bounds__(128) k_hist_grad_g(HistArgs a) {
       const float fx = min_image_f(pj.x - pi.x, sd.Lx), fy = min_image_f(pj.y - pi.y, sd.Ly),
                   fz = min_image_f(pj.z - pi.z, sd.Lz);
       if (fx * fx + fy * fy + fz * fz > reach2f) return;
-      const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
-      const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
-      const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
+      const double4 xj = hist_pos(a, q, pj);
+      const double dx = min_image_d(xj.x - xi.x, sd.Lx);
+      const double dy = min_image_d(xj.y - xi.y, sd.Ly);
+      const double dz = min_image_d(xj.z - xi.z, sd.Lz);
       const double r = sqrt(dx * dx + dy * dy + dz * dz);
       if (r >= reach || r <= 0.0) return;
       const double u = r / w - 0.5;
@@ -303,6 +320,100 @@ __device__ __forceinline__ bool adj_in_range(const AdjArgs& a, const SimDesc& sd
   return s2f + 1e-4f < a.thr[c.ti * a.c.T + tj];
 }
 
+// ---- the FP64 shadow of the rollout ------------------------------------------------------
+//
+// The forward rollout is the FP32 step; its decisions (in-range pairs, counts) define the
+// function the gradient is of.  The shadow carries the same rollout in FP64 along those
+// decisions: x64' = wrap(x64 + dt v64'), v64' = fr v64 + dt fm fac(c + prev) F64, with F64
+// summed over the FP32 step's in-range pairs from FP64 positions (ParticleSimulation.cu:
+// 92-161 with the cutoff test of the FP32 state).  The loss and every adjoint term are
+// evaluated on the shadow, so the gradient is that of one smooth FP64 function of the table
+// and the ratio, the function central differences of the oracle's frozen-decision rollout
+// evaluate.  Shadow states are by particle index.
+
+__global__ void __launch_bounds__(256) k_shadow_init(AdjArgs a) {
+  const SimDesc sd = a.sims[0];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int p = sd.p0 + k;
+  const float4 x = a.pos[p], v = a.vel[p];
+  const uint32_t id = a.id[p];
+  a.x64_next[id] = make_double4((double)x.x, (double)x.y, (double)x.z, 0.0);
+  a.v64_next[id] = make_double4((double)v.x, (double)v.y, (double)v.z, 0.0);
+}
+
+// One FP64 shadow step, G lanes per particle over the FP32 state's stencil; lane sums are
+// reduced in a fixed order.
+__global__ void __launch_bounds__(kAdjBlock) k_shadow_step(AdjArgs a) {
+  constexpr int G = kAdjG;
+  const SimDesc sd = a.sims[0];
+  const int o = threadIdx.x / G, g = threadIdx.x % G;
+  const int k = blockIdx.x * kAdjPer + o;
+  const bool active = k < sd.n;
+  const AdjOwn c = adj_own(a, sd, k, active);
+  const int T = a.c.T;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  int cnt = 0;
+  uint32_t idi = 0;
+  double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (active) {
+    idi = a.id[c.p];
+    xi = a.x64[idi];
+    const double R = a.c.radius, rho = a.c.ratio;
+    const double effi = R * (1.0 + a.c.rbt[c.ti] * rho);
+    for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
+      if (!adj_in_range(a, sd, c, q)) return;
+      ++cnt;
+      const int tj = (int)__float_as_uint(a.pos[q].w);
+      const double4 xj = a.x64[a.id[q]];
+      const double dx = min_image_d(xj.x - xi.x, sd.Lx);
+      const double dy = min_image_d(xj.y - xi.y, sd.Ly);
+      const double dz = min_image_d(xj.z - xi.z, sd.Lz);
+      const double dist = sqrt(dx * dx + dy * dy + dz * dz + (double)1e-4f);
+      const double eff = (effi + R * (1.0 + a.c.rbt[tj] * rho)) * 0.5;
+      const double r = dist / eff;
+      const double net = (a.c.rep * exp(-a.c.k * r * r) - a.c.att * r) * a.c.table[c.ti * T + tj];
+      fx += dx / dist * net;
+      fy += dy / dist * net;
+      fz += dz / dist * net;
+    });
+  }
+#pragma unroll
+  for (int m = G / 2; m > 0; m >>= 1) {
+    fx += __shfl_xor(fx, m, 64);
+    fy += __shfl_xor(fy, m, 64);
+    fz += __shfl_xor(fz, m, 64);
+    cnt += __shfl_xor(cnt, m, 64);
+  }
+  if (!active || g != 0) return;
+  const int prev = __float_as_int(a.vel[c.p].w);
+  const double m = a.c.fm * shadow_fac(a.c, cnt, prev) * a.c.dt;
+  const double4 vi = a.v64[idi];
+  const double vx = vi.x * a.c.friction + fx * m;
+  const double vy = vi.y * a.c.friction + fy * m;
+  const double vz = vi.z * a.c.friction + fz * m;
+  a.v64_next[idi] = make_double4(vx, vy, vz, 0.0);
+  a.x64_next[idi] = make_double4(fmod(xi.x + vx * a.c.dt + (double)sd.Lx, (double)sd.Lx),
+                                 fmod(xi.y + vy * a.c.dt + (double)sd.Ly, (double)sd.Ly),
+                                 fmod(xi.z + vz * a.c.dt + (double)sd.Lz, (double)sd.Lz), 0.0);
+}
+
+// Sets flag bit 3 when a shadow position lies more than slack/2 (periodic, per axis) from
+// the FP32 position it shadows.
+__global__ void __launch_bounds__(256) k_shadow_drift(AdjArgs a) {
+  const SimDesc sd = a.sims[0];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int p = sd.p0 + k;
+  const float4 x = a.pos[p];
+  const double4 s = a.x64[a.id[p]];
+  const double h = 0.5 * a.slack;
+  const double dx = fabs(min_image_d(s.x - (double)x.x, sd.Lx));
+  const double dy = fabs(min_image_d(s.y - (double)x.y, sd.Ly));
+  const double dz = fabs(min_image_d(s.z - (double)x.z, sd.Lz));
+  if (!(dx <= h && dy <= h && dz <= h)) atomicOr(a.flag, 8u);
+}
+
 // The walk: G lanes per particle take the FP32 decision over the stencil and list the
 // in-range slots in LDS, which are then put in slot order (rank by counting, slots are
 // distinct) and written to hits[k][0..); nhits[k] is the particle's pair count, also when
@@ -363,23 +474,24 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
   for (int t = 0; t < kMaxTypes; ++t) gfv[t] = 0.0;
   const AdjOwn c = adj_own(a, sd, k, active);
   const int p = c.p, ti = c.ti;
-  const float4 pi = c.pi;
   uint32_t idi = 0, nh = 0;
   double4 phi_i = make_double4(0.0, 0.0, 0.0, 0.0);
+  double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
   if (active) {
     idi = a.id[p];
     phi_i = a.phi[p];
     nh = a.nhits[k];
+    xi = a.x64[idi];
   }
   // the FP64 terms of one in-range pair
   auto pair = [&](uint32_t q) {
     const double R = a.c.radius, rho = a.c.ratio, rep = a.c.rep, att = a.c.att, kk = a.c.k;
     const double eps = (double)1e-4f;
-    const float4 pj = a.pos[q];
-    const int tj = (int)__float_as_uint(pj.w);
-    const double dx = min_image_d((double)pj.x - (double)pi.x, sd.Lx);
-    const double dy = min_image_d((double)pj.y - (double)pi.y, sd.Ly);
-    const double dz = min_image_d((double)pj.z - (double)pi.z, sd.Lz);
+    const int tj = (int)__float_as_uint(a.pos[q].w);
+    const double4 xj = a.x64[a.id[q]];  // the shadow positions
+    const double dx = min_image_d(xj.x - xi.x, sd.Lx);
+    const double dy = min_image_d(xj.y - xi.y, sd.Ly);
+    const double dz = min_image_d(xj.z - xi.z, sd.Lz);
     const double dist = sqrt(dx * dx + dy * dy + dz * dz + eps);
     const double dR = 0.5 * R * (a.c.rbt[ti] + a.c.rbt[tj]);
     const double eff = R + dR * rho;
@@ -497,6 +609,21 @@ hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st, bool walk) {
   }
   hipLaunchKernelGGL(k_adj_walk, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
   hipLaunchKernelGGL(k_adj_pair_g, dim3(adj_pair_blocks(n)), dim3(kAdjBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_init(const AdjArgs& a, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_shadow_init, dim3((n + 255) / 256), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_step(const AdjArgs& a, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_shadow_step, dim3((n + kAdjPer - 1) / kAdjPer), dim3(kAdjBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_drift(const AdjArgs& a, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_shadow_drift, dim3((n + 255) / 256), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

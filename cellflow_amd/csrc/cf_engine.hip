@@ -1427,13 +1427,19 @@ struct Engine {
     float4* vel = nullptr;
     uint32_t* id = nullptr;
     uint32_t* start = nullptr;
+    double4* x64 = nullptr;  // the gradient rollout's FP64 shadow, by particle index
+    double4* v64 = nullptr;
   };
 
-  int alloc_state(StateBuf& s) {
+  int alloc_state(StateBuf& s, bool shadow = false) {
     CFR(dalloc(&s.pos, P));
     CFR(dalloc(&s.vel, P));
     CFR(dalloc(&s.id, P));
     CFR(dalloc(&s.start, C + 1));
+    if (shadow) {
+      CFR(dalloc(&s.x64, P));
+      CFR(dalloc(&s.v64, P));
+    }
     return CF_OK;
   }
 
@@ -1442,6 +1448,8 @@ struct Engine {
     dfree(s.vel);
     dfree(s.id);
     dfree(s.start);
+    dfree(s.x64);
+    dfree(s.v64);
   }
 
   // The gradient rollout's buffers, kept from one call to the next of the same shape
@@ -1452,6 +1460,8 @@ struct Engine {
     std::vector<StateBuf> ck, seg;
     std::vector<float4*> tv;
     StateBuf fin;
+    double4* fwx[2] = {nullptr, nullptr};  // the forward's shadow states that are not kept
+    double4* fwv[2] = {nullptr, nullptr};
     double4* lx[2] = {nullptr, nullptr};
     double4* lv[2] = {nullptr, nullptr};
     double4* phi = nullptr;
@@ -1471,6 +1481,8 @@ struct Engine {
     for (int i = 0; i < 2; ++i) {
       dfree(adj.lx[i]);
       dfree(adj.lv[i]);
+      dfree(adj.fwx[i]);
+      dfree(adj.fwv[i]);
     }
     dfree(adj.phi);
     for (auto& x : adj.hits) dfree(x);
@@ -1497,13 +1509,15 @@ struct Engine {
     adj.ck.resize((size_t)std::max(nseg, 1L));
     adj.seg.resize((size_t)ckpt + 1);
     adj.tv.assign((size_t)ckpt, nullptr);
-    for (auto& s : adj.ck) CFR(alloc_state(s));
-    for (size_t i = 1; i < adj.seg.size(); ++i) CFR(alloc_state(adj.seg[i]));
+    for (auto& s : adj.ck) CFR(alloc_state(s, true));
+    for (size_t i = 1; i < adj.seg.size(); ++i) CFR(alloc_state(adj.seg[i], true));
     for (auto& t : adj.tv) CFR(dalloc(&t, P));
     CFR(alloc_state(adj.fin));
     for (int i = 0; i < 2; ++i) {
       CFR(dalloc(&adj.lx[i], P));
       CFR(dalloc(&adj.lv[i], P));
+      CFR(dalloc(&adj.fwx[i], P));
+      CFR(dalloc(&adj.fwv[i], P));
     }
     CFR(dalloc(&adj.phi, P));
     adj.hits.assign((size_t)ckpt, nullptr);
@@ -1539,7 +1553,9 @@ struct Engine {
     return CF_OK;
   }
 
-  HistArgs hist_args(const StateBuf& s, int bins, double r_max, double* partial, long stride) {
+  // `x64` (optional): FP64 positions by index that lie within slack/2 of s.pos per axis.
+  HistArgs hist_args(const StateBuf& s, int bins, double r_max, double* partial, long stride,
+                     const double4* x64 = nullptr, double slack = 0.0) {
     const SimDesc& d = hdesc[0];
     HistArgs h{};
     h.pos = s.pos;
@@ -1548,7 +1564,9 @@ struct Engine {
     h.sims = b.sims;
     h.bins = bins;
     h.r_max = r_max;
-    const double reach = r_max * (1.0 + 2.0 / bins) * (1.0 + 1e-3);
+    h.x64 = x64;
+    h.slack = x64 ? slack : 0.0;
+    const double reach = (r_max * (1.0 + 2.0 / bins) + h.slack) * (1.0 + 1e-3);
     const double ih[3] = {d.ihx, d.ihy, d.ihz};
     int r = 1;
     for (int a = 0; a < 3; ++a) r = std::max(r, (int)std::ceil(reach * ih[a]));
@@ -1580,6 +1598,25 @@ struct Engine {
     dfree(partial);
     dfree(hd);
     return CF_OK;
+  }
+
+  // The rollout's FP64 constants (from the FP32 parameters).
+  AdjConst adj_const(const SimHost& h) const {
+    AdjConst c{};
+    c.T = h.T;
+    c.radius = h.prm.radius;
+    c.ratio = h.prm.ratio_with_lfo;
+    c.rep = h.prm.repulsion;
+    c.att = h.prm.attraction;
+    c.k = h.prm.k;
+    c.dt = h.prm.delta_t;
+    c.friction = h.prm.friction;
+    c.fm = h.prm.force_multiplier;
+    c.balance = h.prm.balance;
+    c.max_exp = (double)h.prm.max_expected_neighbors;
+    for (int t = 0; t < h.T; ++t) c.rbt[t] = h.rbt[t];
+    for (int t = 0; t < h.T * h.T; ++t) c.table[t] = h.table[t];
+    return c;
   }
 
   // histogram() with its partial sums and result in the device's `scratch` array: no
@@ -1662,10 +1699,45 @@ struct Engine {
         b.id[side] = s.id;
         b.start[side] = s.start;
       };
-      // forward with checkpoints; the last segment's states are kept as they are made
+      // the FP64 shadow of the rollout (decisions of the FP32 steps, state in FP64): the
+      // constants, and where the shadow state after s steps lives (checkpoints and the last
+      // segment's states are kept; the others alternate between two buffers)
+      AdjArgs sa{};
+      sa.sims = b.sims;
+      sa.thr = thr;
+      sa.c = adj_const(h);
+      sa.flag = b.flag;
+      sa.slack = 1e-3 * std::max({h.prm.canvas_width, h.prm.canvas_height, h.prm.canvas_depth});
       const int last0 = (nseg - 1) * ckpt;
+      auto shadow = [&](int s, bool vel) -> double4* {
+        if (s > last0) return vel ? seg[s - last0].v64 : seg[s - last0].x64;
+        if (s % ckpt == 0) return vel ? ck[s / ckpt].v64 : ck[s / ckpt].x64;
+        return vel ? adj.fwv[s & 1] : adj.fwx[s & 1];
+      };
+      // one shadow step from the engine's current (FP32, pre-step) state
+      auto shadow_step = [&](double4* x, double4* v, double4* xn, double4* vn) -> int {
+        sa.pos = b.pos[cur];
+        sa.vel = b.vel[cur];
+        sa.id = b.id[cur];
+        sa.start = b.start[cur];
+        sa.x64 = x;
+        sa.v64 = v;
+        sa.x64_next = xn;
+        sa.v64_next = vn;
+        HIPR(launch_shadow_step(sa, n, st));
+        return CF_OK;
+      };
+      sa.pos = b.pos[cur];
+      sa.vel = b.vel[cur];
+      sa.id = b.id[cur];
+      sa.x64_next = shadow(0, false);
+      sa.v64_next = shadow(0, true);
+      HIPR(launch_shadow_init(sa, n, st));
+      // forward with checkpoints; the last segment's states are kept as they are made
       for (int s = 0; s < nsteps; ++s) {
         if (s % ckpt == 0) CFR(save_state(ck[s / ckpt]));
+        CFR(shadow_step(shadow(s, false), shadow(s, true), shadow(s + 1, false),
+                        shadow(s + 1, true)));
         if (s >= last0) {
           point(1 - cur, seg[s - last0 + 1]);
           b.tvel = tv[s - last0];
@@ -1676,12 +1748,23 @@ struct Engine {
       fhits = fnhits = nullptr;
       CFR(save_state(fin));
       b = own;
-      // loss of the final state
-      HIPR(launch_hist(hist_args(fin, bins, r_max, partial, stride), n, st));
+      const double4* xfin = shadow(nsteps, false);
+      // the shadow stays close enough to the FP32 state for the histogram's cell reach
+      sa.pos = fin.pos;
+      sa.id = fin.id;
+      sa.x64 = xfin;
+      HIPR(launch_shadow_drift(sa, n, st));
+      // loss of the final (shadow) state
+      HIPR(launch_hist(hist_args(fin, bins, r_max, partial, stride, xfin, sa.slack), n, st));
       HIPR(launch_reduce_rows(partial, stride, n, bins, 1.0 / n, hd, 0, st));
       std::vector<double> hh(bins), g(bins);
       HIPR(hipMemcpyAsync(hh.data(), hd, bins * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPR(hipMemcpyAsync(hsmall, b.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
       HIPR(hipStreamSynchronize(st));
+      if (hsmall[0] & 1u) return fail(CF_ERR_NONFINITE, "a position or velocity became non-finite");
+      if (hsmall[0] & 8u)
+        return fail(CF_ERR_STATE, "the rollout's FP64 shadow strayed more than %g from the FP32 "
+                    "trajectory: too chaotic for a gradient with frozen decisions", 0.5 * sa.slack);
       double loss = 0.0;
       for (int bb = 0; bb < bins; ++bb) {
         loss += (hh[bb] - target[bb]) * (hh[bb] - target[bb]);
@@ -1690,7 +1773,7 @@ struct Engine {
       *loss_out = loss;
       if (hist_out) std::copy(hh.begin(), hh.end(), hist_out);
       HIPR(hipMemcpyAsync(gh, g.data(), bins * sizeof(double), hipMemcpyHostToDevice, st));
-      HistArgs ha = hist_args(fin, bins, r_max, partial, stride);
+      HistArgs ha = hist_args(fin, bins, r_max, partial, stride, xfin, sa.slack);
       ha.gh = gh;
       ha.lam_x = lx[0];
       ha.lam_v = lv[0];
@@ -1700,23 +1783,18 @@ struct Engine {
       AdjArgs aa{};
       aa.sims = b.sims;
       aa.thr = thr;
-      aa.c.T = T;
-      aa.c.radius = h.prm.radius;
-      aa.c.ratio = h.prm.ratio_with_lfo;
-      aa.c.rep = h.prm.repulsion;
-      aa.c.att = h.prm.attraction;
-      aa.c.k = h.prm.k;
-      for (int t = 0; t < T; ++t) aa.c.rbt[t] = h.rbt[t];
-      for (int t = 0; t < T * T; ++t) aa.c.table[t] = h.table[t];
+      aa.c = sa.c;
       aa.phi = phi;
       aa.partial = partial;
       aa.stride = nblk;
       int L = 0;  // lx[L] holds dL/dx of the state after the step being reversed
       for (int sg = nseg - 1; sg >= 0; --sg) {
         const int k0 = sg * ckpt, k1 = std::min(nsteps, k0 + ckpt);
-        if (sg != nseg - 1) {  // recompute the segment from its checkpoint
+        if (sg != nseg - 1) {  // recompute the segment (and its shadow) from its checkpoint
           point(cur, ck[sg]);
           for (int s = k0; s < k1; ++s) {
+            const StateBuf& sp = (s == k0) ? ck[sg] : seg[s - k0];
+            CFR(shadow_step(sp.x64, sp.v64, seg[s - k0 + 1].x64, seg[s - k0 + 1].v64));
             point(1 - cur, seg[s - k0 + 1]);
             b.tvel = tv[s - k0];
             listing(s - k0);
@@ -1731,6 +1809,7 @@ struct Engine {
           aa.vel = pre.vel;
           aa.id = pre.id;
           aa.start = pre.start;
+          aa.x64 = pre.x64;
           aa.tvel = tv[s - k0];
           aa.lam_x_next = lx[L];
           aa.lam_v_next = lv[L];

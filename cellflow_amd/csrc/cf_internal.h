@@ -357,6 +357,7 @@ constexpr int kMaxBins = 64;
 struct AdjConst {
   int T;
   double radius, ratio, rep, att, k;
+  double dt, friction, fm, balance, max_exp;  // the step's scalars as FP64 (from FP32)
   double rbt[kMaxTypes];
   double table[kMaxTypes * kMaxTypes];
 };
@@ -374,6 +375,11 @@ struct HistArgs {
   const double* gh;     // dL/dh_b
   double4* lam_x;
   double4* lam_v;
+  // optional FP64 positions by particle index (the rollout's FP64 shadow), used for the
+  // distances instead of `pos`, which still gives the cells; `slack` bounds how far they
+  // may lie from `pos` (it widens the cell reach and the FP32 pre-test)
+  const double4* x64;
+  double slack;
 };
 
 struct AdjArgs {
@@ -395,6 +401,14 @@ struct AdjArgs {
   double* partial;          // [T*T + 1][stride] parameter partials, one column per block,
                             // added to (the caller zeroes them once per rollout)
   long stride;              // >= adj_pair_blocks(n)
+  // the FP64 shadow of the pre-step state, by particle index, and (shadow step) of the
+  // post-step state
+  const double4* x64;
+  const double4* v64;
+  double4* x64_next;
+  double4* v64_next;
+  uint32_t* flag;           // shadow drift check: bit 3 when the shadow strays beyond slack/2
+  double slack;
 };
 
 hipError_t launch_hist(const HistArgs& a, int n, hipStream_t st);
@@ -405,6 +419,12 @@ hipError_t launch_reduce_rows(const double* in, long stride, int n, int rows, do
 // (and computes the density-factor terms); without, the lists in AdjArgs::hits came from
 // the recompute's force kernel.
 hipError_t launch_adj_step(const AdjArgs& a, int n, hipStream_t st, bool walk);
+// The rollout's FP64 shadow: the pre-step FP32 state promoted (index order); one FP64 step
+// with the decisions of the FP32 pre-step state; the drift check of a shadow state against
+// the FP32 state it shadows.
+hipError_t launch_shadow_init(const AdjArgs& a, int n, hipStream_t st);
+hipError_t launch_shadow_step(const AdjArgs& a, int n, hipStream_t st);
+hipError_t launch_shadow_drift(const AdjArgs& a, int n, hipStream_t st);
 int adj_pair_blocks(int n);  // blocks (columns of AdjArgs::partial) of one backward step
 
 // host-side launchers (cf_kernels.hip)

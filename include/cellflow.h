@@ -261,7 +261,12 @@ int cf_pair_histogram_noalloc(cf_sim* sim, int bins, float r_max, double* hist);
  * and returns the loss L = sum_b (h_b - target_b)^2 of the final histogram and its gradient
  * with respect to the num_types^2 force-table entries (row-major) and ratio_with_lfo, by
  * a hand-written adjoint with state checkpoints every `checkpoint_every` steps.  The
- * rollout's discrete decisions (which pairs interact, neighbour counts) are held fixed.
+ * rollout's discrete decisions (which pairs interact, neighbour counts) are those of the
+ * FP32 steps and are held fixed; the loss, the histogram and the gradient are of the FP64
+ * rollout along those decisions (an FP64 shadow of the state, stepped beside the FP32
+ * one), so the gradient is that of one smooth FP64 function of the table and the ratio.
+ * CF_ERR_STATE when the shadow strays from the FP32 state by more than 1/2000 of the
+ * largest box side (a rollout too chaotic for a gradient with frozen decisions).
  * The call's device buffers (checkpoints, one segment's states, adjoint state) stay
  * allocated for the next call of the same shape and are freed with the handle. */
 int cf_rollout_grad(cf_sim* sim, const cf_params* params, int nsteps, int checkpoint_every,
