@@ -342,8 +342,10 @@ __global__ void __launch_bounds__(256) k_shadow_init(AdjArgs a) {
   a.v64_next[id] = make_double4((double)v.x, (double)v.y, (double)v.z, 0.0);
 }
 
-// One FP64 shadow step, G lanes per particle over the FP32 state's stencil; lane sums are
-// reduced in a fixed order.
+// One FP64 shadow step, G lanes per particle; lane sums are reduced in a fixed order.  The
+// step's in-range pairs come from the lists the FP32 step's force kernel wrote (a.hits, at
+// most kAdjHits per particle, in slot order), or from a walk of the FP32 state's stencil with
+// the forward's FP32 decision (no lists, or a particle with more pairs than a list holds).
 __global__ void __launch_bounds__(kAdjBlock) k_shadow_step(AdjArgs a) {
   constexpr int G = kAdjG;
   const SimDesc sd = a.sims[0];
@@ -356,27 +358,38 @@ __global__ void __launch_bounds__(kAdjBlock) k_shadow_step(AdjArgs a) {
   int cnt = 0;
   uint32_t idi = 0;
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
+  const double R = a.c.radius, rho = a.c.ratio;
+  const double effi = active ? R * (1.0 + a.c.rbt[c.ti] * rho) : 0.0;
+  // the FP64 force term of one in-range pair (the oracle's op order)
+  auto pair = [&](uint32_t q) {
+    const int tj = (int)__float_as_uint(a.pos[q].w);
+    const double4 xj = a.x64[a.id[q]];
+    const double dx = min_image_d(xj.x - xi.x, sd.Lx);
+    const double dy = min_image_d(xj.y - xi.y, sd.Ly);
+    const double dz = min_image_d(xj.z - xi.z, sd.Lz);
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz + (double)1e-4f);
+    const double eff = (effi + R * (1.0 + a.c.rbt[tj] * rho)) * 0.5;
+    const double r = dist / eff;
+    const double net = (a.c.rep * exp(-a.c.k * r * r) - a.c.att * r) * a.c.table[c.ti * T + tj];
+    fx += dx / dist * net;
+    fy += dy / dist * net;
+    fz += dz / dist * net;
+  };
   if (active) {
     idi = a.id[c.p];
     xi = a.x64[idi];
-    const double R = a.c.radius, rho = a.c.ratio;
-    const double effi = R * (1.0 + a.c.rbt[c.ti] * rho);
-    for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
-      if (!adj_in_range(a, sd, c, q)) return;
-      ++cnt;
-      const int tj = (int)__float_as_uint(a.pos[q].w);
-      const double4 xj = a.x64[a.id[q]];
-      const double dx = min_image_d(xj.x - xi.x, sd.Lx);
-      const double dy = min_image_d(xj.y - xi.y, sd.Ly);
-      const double dz = min_image_d(xj.z - xi.z, sd.Lz);
-      const double dist = sqrt(dx * dx + dy * dy + dz * dz + (double)1e-4f);
-      const double eff = (effi + R * (1.0 + a.c.rbt[tj] * rho)) * 0.5;
-      const double r = dist / eff;
-      const double net = (a.c.rep * exp(-a.c.k * r * r) - a.c.att * r) * a.c.table[c.ti * T + tj];
-      fx += dx / dist * net;
-      fy += dy / dist * net;
-      fz += dz / dist * net;
-    });
+    const uint32_t nh = a.hits ? a.nhits[k] : 0u;
+    if (a.hits && nh <= (uint32_t)kAdjHits) {
+      if (g == 0) cnt = (int)nh;
+#pragma nounroll
+      for (uint32_t e = g; e < nh; e += G) pair(a.hits[(long)k * kAdjHits + e]);
+    } else {
+      for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
+        if (!adj_in_range(a, sd, c, q)) return;
+        ++cnt;
+        pair(q);
+      });
+    }
   }
 #pragma unroll
   for (int m = G / 2; m > 0; m >>= 1) {

@@ -1714,12 +1714,16 @@ struct Engine {
         if (s % ckpt == 0) return vel ? ck[s / ckpt].v64 : ck[s / ckpt].x64;
         return vel ? adj.fwv[s & 1] : adj.fwx[s & 1];
       };
-      // one shadow step from the engine's current (FP32, pre-step) state
-      auto shadow_step = [&](double4* x, double4* v, double4* xn, double4* vn) -> int {
-        sa.pos = b.pos[cur];
-        sa.vel = b.vel[cur];
-        sa.id = b.id[cur];
-        sa.start = b.start[cur];
+      // one shadow step of an FP32 pre-step state, after the FP32 step whose force kernel
+      // listed the state's in-range pairs into list buffer `li` (no lists: the shadow walks)
+      auto shadow_step = [&](const StateBuf& pre, int li, const double4* x, const double4* v,
+                             double4* xn, double4* vn) -> int {
+        sa.pos = pre.pos;
+        sa.vel = pre.vel;
+        sa.id = pre.id;
+        sa.start = pre.start;
+        sa.hits = lists ? adj.hits[li] : nullptr;
+        sa.nhits = lists ? adj.nhits[li] : nullptr;
         sa.x64 = x;
         sa.v64 = v;
         sa.x64_next = xn;
@@ -1733,17 +1737,24 @@ struct Engine {
       sa.x64_next = shadow(0, false);
       sa.v64_next = shadow(0, true);
       HIPR(launch_shadow_init(sa, n, st));
-      // forward with checkpoints; the last segment's states are kept as they are made
+      // forward with checkpoints; the last segment's states are kept as they are made.
+      // Every step lists its in-range pairs (for the shadow; the last segment's lists are
+      // also the backward's), into list buffer s mod ckpt.
       for (int s = 0; s < nsteps; ++s) {
         if (s % ckpt == 0) CFR(save_state(ck[s / ckpt]));
-        CFR(shadow_step(shadow(s, false), shadow(s, true), shadow(s + 1, false),
-                        shadow(s + 1, true)));
+        StateBuf pre;
+        pre.pos = b.pos[cur];
+        pre.vel = b.vel[cur];
+        pre.id = b.id[cur];
+        pre.start = b.start[cur];
         if (s >= last0) {
           point(1 - cur, seg[s - last0 + 1]);
           b.tvel = tv[s - last0];
-          listing(s - last0);
         }
+        listing(s % ckpt);
         CFR(step(false, 0));
+        CFR(shadow_step(pre, s % ckpt, shadow(s, false), shadow(s, true), shadow(s + 1, false),
+                        shadow(s + 1, true)));
       }
       fhits = fnhits = nullptr;
       CFR(save_state(fin));
@@ -1794,11 +1805,12 @@ struct Engine {
           point(cur, ck[sg]);
           for (int s = k0; s < k1; ++s) {
             const StateBuf& sp = (s == k0) ? ck[sg] : seg[s - k0];
-            CFR(shadow_step(sp.x64, sp.v64, seg[s - k0 + 1].x64, seg[s - k0 + 1].v64));
             point(1 - cur, seg[s - k0 + 1]);
             b.tvel = tv[s - k0];
             listing(s - k0);
             CFR(step(false, 0));
+            CFR(shadow_step(sp, s - k0, sp.x64, sp.v64, seg[s - k0 + 1].x64,
+                            seg[s - k0 + 1].v64));
           }
           fhits = fnhits = nullptr;
           b = own;

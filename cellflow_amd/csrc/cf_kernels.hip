@@ -1492,8 +1492,9 @@ __device__ __forceinline__ bool edge_row(const SimDesc& sd, int cx, int cy, int 
 }
 
 // Runs a lane's stencil rows r = r0, r0 + G, ... (kEdgeRowBatch of them at a time, their
-// cell-start loads issued together) and calls f(q, pj) for each slot q, positions loaded
-// kEdgeLoadBatch at a time.
+// cell-start loads issued together) and calls f(q, pj, jid) for each slot q, positions and
+// indices loaded kEdgeLoadBatch at a time (an index is loaded with its position, so a
+// qualifying candidate waits for no further load).
 constexpr int kEdgeRowBatch = 3;
 constexpr int kEdgeLoadBatch = 4;
 
@@ -1519,12 +1520,16 @@ __device__ __forceinline__ void edge_walk(const EdgeArgs& a, const SimDesc& sd, 
     for (int i = 0; i < kEdgeRowBatch; ++i)
       for (int q = qa[i]; q < qb[i]; q += kEdgeLoadBatch) {
         float4 pj[kEdgeLoadBatch];
+        uint32_t jid[kEdgeLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kEdgeLoadBatch; ++u) {
+          const bool in = q + u < qb[i];
+          pj[u] = in ? a.pos[q + u] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+          jid[u] = in ? a.id[q + u] : 0u;
+        }
 #pragma unroll
         for (int u = 0; u < kEdgeLoadBatch; ++u)
-          pj[u] = (q + u < qb[i]) ? a.pos[q + u] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-#pragma unroll
-        for (int u = 0; u < kEdgeLoadBatch; ++u)
-          if (q + u < qb[i]) f(q + u, pj[u]);
+          if (q + u < qb[i]) f(q + u, pj[u], jid[u]);
       }
   }
 }
@@ -1565,12 +1570,11 @@ __global__ void __launch_bounds__(256) k_edges_g(EdgeArgs a) {
   wave_lds_sync();
   if (active)
     edge_walk(a, sd, cx, cy, cz, fxr, fyr, fzr, cut2, g, G, nrows,
-              [&](int q, const float4& pj) {
+              [&](int, const float4& pj, uint32_t jid) {
                 if (__float_as_uint(pj.w) != ti) return;        // :219
                 const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                 const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                 if (!(s2 < sd.edge_d2)) return;                  // :227
-                const uint32_t jid = a.id[q];
                 if (jid <= idi) return;                          // j > i, :215
                 const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
                 if (kk < (uint32_t)kListEdges) {
@@ -1616,12 +1620,11 @@ __global__ void __launch_bounds__(256) k_edges_g(EdgeArgs a) {
     ea.maxid = 0;
     ea.maxk = 0;
     edge_walk(a, sd, cx, cy, cz, fxr, fyr, fzr, cut2, 0, 1, nrows,
-              [&](int q, const float4& pj) {
+              [&](int, const float4& pj, uint32_t jid) {
                 if (__float_as_uint(pj.w) != ti) return;
                 const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                 const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                 if (!(s2 < sd.edge_d2)) return;
-                const uint32_t jid = a.id[q];
                 if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
               });
     e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
@@ -1696,7 +1699,8 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
 #pragma unroll
   for (int i = 0; i < kPer; ++i)  // the next step's force pass writes only non-zero counts
     if (ec[i]) a.ecount[id0 + i] = 0u;
-  const int p0 = (id0 < a.P) ? a.sims[a.wave_sim[id0 / kWave]].p0 : 0;
+  // one simulation starts at slot 0; otherwise the owners' simulation's first slot
+  const int p0 = (a.one_sim || id0 >= a.P) ? 0 : a.sims[a.wave_sim[id0 / kWave]].p0;
   uint32_t part = 0, mine = 0;
   for (int v = tid; v < (u + 1) * kEdgeSub; v += kSortThreads) {
     const uint32_t x = a.etcount[v * kCtrStride];
