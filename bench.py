@@ -42,7 +42,8 @@ def force_kernel_name(t, edges=True):
     r, g = t["cell_divisor"], t["lanes_per_particle"]
     tl = "true" if t.get("sort_variant", 1) == 1 else "false"
     return {0: f"k_force_grp<{e}, {r}, {g}, {tl}>", 1: f"k_force<{e}>",
-            2: f"k_force_lds<{e}, {r}>", 3: f"k_force_list<{e}, {r}, {g}, {tl}>"}[t["force_variant"]]
+            2: f"k_force_lds<{e}, {r}>", 3: f"k_force_list<{e}, {r}, {g}, {tl}, false>",
+            4: f"k_force_list<{e}, {r}, {g}, {tl}, true>"}[t["force_variant"]]
 
 
 def read_traffic(path, kernel):
@@ -174,9 +175,11 @@ def parity_sample(cfg, params, kept):
                     "FP32 rounding away diverges by a similar amount)"}
 
 
-def ensemble_leg(steps, warm=3):
+def ensemble_leg(steps, warm=3, tuning="", sequential=True):
     """Batched ensemble (256 simulations, uneven sizes) vs the same simulations run one after
-    another as plain runs.  Both sides are warmed first (grids built, graphs captured)."""
+    another as plain runs.  Both sides are warmed first (grids built, graphs captured).
+    `tuning` (force_variant,lanes,cell_divisor,sort_variant) replaces the ensemble's autotuning
+    and `sequential` False skips the plain runs (profiling runs of the ensemble alone)."""
     from cellflow_amd import Ensemble, ParticleSimulation, SimulationParams, particles_from_state
     from cellflow_amd import configs
     cfg = configs.load("c4_ensemble_256")
@@ -185,7 +188,12 @@ def ensemble_leg(steps, warm=3):
     log(f"[bench] ensemble: {len(sims)} simulations, {total} particles, {steps} steps")
     cache = os.path.join(tempfile.mkdtemp(prefix="cf_tune_"), "tuning.json")
     ens = Ensemble(sims)
-    tuned = ens.autotune(cache=cache)
+    if tuning:
+        v, g, d, so = (int(x) for x in tuning.split(","))
+        ens.setTuning(cell_divisor=d, force_variant=v, lanes_per_particle=g, sort_variant=so)
+        tuned = {"tuning": ens.getTuning(), "us_per_step": float("nan")}
+    else:
+        tuned = ens.autotune(cache=cache)
     log(f"[bench] ensemble tuning {tuned['tuning']} ({tuned['us_per_step']:.1f} us/step)")
     ens.simulate(warm)
     ens.getNeighborCounts(0)
@@ -194,6 +202,10 @@ def ensemble_leg(steps, warm=3):
     ens.getNeighborCounts(0)
     t_ens = time.perf_counter() - t0
     ens.close()
+    if not sequential:
+        return {"sims": len(sims), "particles": total, "steps": steps, "ensemble_s": t_ens,
+                "ensemble_particle_steps_per_s": total * steps / t_ens,
+                "workload": cfg["name"], "tuning": tuned["tuning"]}
     plain = []
     for ph, st in sims:
         p = ParticleSimulation(len(st["type"]), ph["num_types"])
@@ -409,6 +421,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--ensemble-tuning", default="",
+                    help="force_variant,lanes,cell_divisor,sort_variant for the ensemble instead "
+                         "of autotuning it (profiling runs)")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="skip the ensemble's sequential plain runs (profiling runs)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--no-clustered", action="store_true", help="skip the 1M clustered leg")
     ap.add_argument("--no-frame", action="store_true", help="skip the host-inclusive frame leg")
@@ -534,7 +551,8 @@ def main():
         out["mesh_and_init"] = mesh_init_leg(sim, cfg["num_types"])
     sim.close()
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_leg(args.ensemble_steps)
+        out["ensemble"] = ensemble_leg(args.ensemble_steps, tuning=args.ensemble_tuning,
+                                       sequential=not args.no_sequential)
     if not args.no_adjoint:
         log("[bench] differentiable rollout")
         out["adjoint"] = adjoint_leg()

@@ -1972,6 +1972,8 @@ struct Engine {
       for (int g : {16, 8, 4, 32})
         if (g >= 4 * d)
           cands.push_back(cf_tuning{d, 3, g, orig.block_size, orig.steps_per_graph, 1});
+      for (int g : {8, 16})
+        cands.push_back(cf_tuning{d, 4, g, orig.block_size, orig.steps_per_graph, 1});
     }
     const size_t nforce = cands.size();  // then the best two again with the three-pass sort
     const int span = orig.steps_per_graph >= 2 ? (orig.steps_per_graph + 1) / 2 * 2 : 2;

@@ -33,8 +33,10 @@ int check_tuning(const cf_tuning* t) {
     return fail(CF_ERR_ARG, "cell_divisor must be 1 or 2");
   if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256)
     return fail(CF_ERR_ARG, "block_size must be 64, 128 or 256");
-  if (t->force_variant < 0 || t->force_variant > 3)
-    return fail(CF_ERR_ARG, "force_variant must be 0, 1, 2 or 3");
+  if (t->force_variant < 0 || t->force_variant > 4)
+    return fail(CF_ERR_ARG, "force_variant must be 0, 1, 2, 3 or 4");
+  if (t->force_variant == 4 && t->lanes_per_particle != 8 && t->lanes_per_particle != 16)
+    return fail(CF_ERR_ARG, "force_variant 4 needs lanes_per_particle 8 or 16");
   if (t->lanes_per_particle != 4 && t->lanes_per_particle != 8 && t->lanes_per_particle != 16 &&
       t->lanes_per_particle != 32)
     return fail(CF_ERR_ARG, "lanes_per_particle must be 4, 8, 16 or 32");

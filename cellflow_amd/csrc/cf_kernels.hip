@@ -1208,7 +1208,41 @@ __device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, 
   }
 }
 
-template <bool EDGES, int R, int G, bool TILE>
+// ParticleSimulation.cu:136-161 for one particle: density factor, friction-Euler step and
+// wrap; the new state by pre-step slot p, the acceleration (when stored) and the new cell
+// (commit_cell).  cyo, czo: the pre-step cell's row and plane.
+template <bool TILE>
+__device__ __forceinline__ void integrate_commit(const ForceArgs& a, const SimDesc& sd, int p,
+                                                 const float4& pi, const float4& vi,
+                                                 uint32_t idi, float fx, float fy, float fz,
+                                                 int cnt, int cyo, int czo) {
+  const int prev = __float_as_int(vi.w);
+  const float avg = (float)(cnt + prev) * 0.5f;
+  const float dens = fminf(avg / sd.max_exp, 1.0f);
+  const float fac = 1.0f - sd.one_minus_balance * dens;
+  const float m = sd.fm * fac;
+  fx *= m;
+  fy *= m;
+  fz *= m;
+  const float vx = vi.x * sd.friction + fx * sd.dt;
+  const float vy = vi.y * sd.friction + fy * sd.dt;
+  const float vz = vi.z * sd.friction + fz * sd.dt;
+  const float x = wrap_pos(pi.x + vx * sd.dt, sd.Lx);
+  const float y = wrap_pos(pi.y + vy * sd.dt, sd.Ly);
+  const float z = wrap_pos(pi.z + vz * sd.dt, sd.Lz);
+  const float fin = fabsf(x) + fabsf(y) + fabsf(z) + fabsf(vx) + fabsf(vy) + fabsf(vz);
+  if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
+  a.tpos[p] = make_float4(x, y, z, pi.w);
+  a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
+  if (a.acc) a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
+  commit_cell<TILE>(a, sd, p, cyo, czo, x, y, z);
+}
+
+// CONS (variant 4): the integration tail of the block's 4 * kOwn particles runs once, in
+// wave 0 with one lane per particle, instead of once per wave with one lane of each group
+// active: each wave leaves its particles' sums in LDS, and after a block barrier wave 0
+// reloads their (cache-resident) state and finishes them.  Same arithmetic, same results.
+template <bool EDGES, int R, int G, bool TILE, bool CONS = false>
 // 27-cell stencil at G = 8 fits 7 waves per SIMD (72 VGPRs, under 22.8 KB of LDS per block)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 1 && G == 8 ? 7 : 1, 8)))
 k_force_list(ForceArgs a) {
@@ -1221,6 +1255,8 @@ k_force_list(ForceArgs a) {
   __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_n[4][kOwn];
+  __shared__ float4 cres[CONS ? 4 * kOwn : 1];    // force sums and count bits (CONS)
+  __shared__ uint32_t cedge[CONS ? 4 * kOwn : 1];  // kept edges | active << 30 | edges << 31
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int gw = xcd_block() * (blockDim.x >> 6) + wv;
@@ -1355,6 +1391,7 @@ k_force_list(ForceArgs a) {
   cnt = dpp_group_sum_i<G>(cnt);
   if (active && 1e-4f < row[ti].thr) cnt -= 1;  // the particle itself
 
+  uint32_t ekept = 0u;  // kept edges of the particle (lane g == 0), passed on by CONS
   if (EDGES) {
     // the wave's candidate lists are complete
     wave_lds_sync();
@@ -1416,33 +1453,42 @@ k_force_list(ForceArgs a) {
         e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
                                   TILE ? idi : (uint32_t)p);
       }
-      if (e != 0u) a.ecount[idi] = e;  // zero already (the sorts clear it)
-      count_edges<TILE>(a, idi, e);
+      ekept = e;
+      if (!CONS) {
+        if (e != 0u) a.ecount[idi] = e;  // zero already (the sorts clear it)
+        count_edges<TILE>(a, idi, e);
+      }
     }
   }
 
+  if constexpr (CONS) {
+    const bool de = EDGES && a.edges && sd.edge_on && active;
+    if (g == 0) {
+      cres[wv * kOwn + o] = make_float4(fx, fy, fz, __int_as_float(cnt));
+      cedge[wv * kOwn + o] = ekept | (active ? (1u << 30) : 0u) | (de ? (1u << 31) : 0u);
+    }
+    __syncthreads();
+    if (wv != 0 || lane >= 4 * kOwn) return;
+    // wave 0: lane L finishes the block's particle L
+    const uint32_t fl = cedge[lane];
+    if (!(fl & (1u << 30))) return;
+    const int pl = (int)(xcd_block() * 4 * kOwn) + lane;
+    const float4 r = cres[lane];
+    const float4 pl_pos = a.pos[pl];
+    const float4 pl_vel = a.vel[pl];
+    const uint32_t pl_id = a.id[pl];
+    if (fl & (1u << 31)) {
+      const uint32_t e = fl & 0x3FFFFFFFu;
+      if (e != 0u) a.ecount[pl_id] = e;  // zero already (the sorts clear it)
+      count_edges<TILE>(a, pl_id, e);
+    }
+    integrate_commit<TILE>(a, sd, pl, pl_pos, pl_vel, pl_id, r.x, r.y, r.z,
+                           __float_as_int(r.w), bin_axis(pl_pos.y, sd.ihy, sd.ny),
+                           bin_axis(pl_pos.z, sd.ihz, sd.nz));
+    return;
+  }
   if (!active || g != 0) return;
-  // ParticleSimulation.cu:136-161
-  const int prev = __float_as_int(vi.w);
-  const float avg = (float)(cnt + prev) * 0.5f;
-  const float dens = fminf(avg / sd.max_exp, 1.0f);
-  const float fac = 1.0f - sd.one_minus_balance * dens;
-  const float m = sd.fm * fac;
-  fx *= m;
-  fy *= m;
-  fz *= m;
-  const float vx = vi.x * sd.friction + fx * sd.dt;
-  const float vy = vi.y * sd.friction + fy * sd.dt;
-  const float vz = vi.z * sd.friction + fz * sd.dt;
-  const float x = wrap_pos(pi.x + vx * sd.dt, sd.Lx);
-  const float y = wrap_pos(pi.y + vy * sd.dt, sd.Ly);
-  const float z = wrap_pos(pi.z + vz * sd.dt, sd.Lz);
-  const float fin = fabsf(x) + fabsf(y) + fabsf(z) + fabsf(vx) + fabsf(vy) + fabsf(vz);
-  if (!(fin <= 3.402823466e38f)) atomicOr(a.flag, 1u);
-  a.tpos[p] = make_float4(x, y, z, pi.w);
-  a.tvel[p] = make_float4(vx, vy, vz, __int_as_float(cnt));
-  if (a.acc) a.acc[idi] = make_float4(fx, fy, fz, 0.0f);
-  commit_cell<TILE>(a, sd, p, c.cy, c.cz, x, y, z);
+  integrate_commit<TILE>(a, sd, p, pi, vi, idi, fx, fy, fz, cnt, c.cy, c.cz);
 }
 
 // ------------------------------------------------------------------------------------
@@ -2488,6 +2534,19 @@ static void launch_list(const ForceArgs& a, bool edges, hipStream_t st) {
   }
 }
 
+template <int R, int G>
+static void launch_cons(const ForceArgs& a, bool edges, hipStream_t st) {
+  const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
+  const dim3 g((unsigned)((waves + 3) / 4)), b(256);
+  if (a.tile_sort) {
+    if (edges) hipLaunchKernelGGL((k_force_list<true, R, G, true, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_force_list<false, R, G, true, true>), g, b, 0, st, a);
+  } else {
+    if (edges) hipLaunchKernelGGL((k_force_list<true, R, G, false, true>), g, b, 0, st, a);
+    else hipLaunchKernelGGL((k_force_list<false, R, G, false, true>), g, b, 0, st, a);
+  }
+}
+
 hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
                         int lanes, hipStream_t st) {
   if (a.hits && (variant != 0 || edges)) return hipErrorInvalidValue;  // lists: variant 0 only
@@ -2503,6 +2562,18 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
       else if (lanes == 8) launch_grp<2, 8>(a, edges, st);
       else if (lanes == 16) launch_grp<2, 16>(a, edges, st);
       else if (lanes == 32) launch_grp<2, 32>(a, edges, st);
+      else return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+  if (variant == 4) {  // variant 3 with the block-consolidated tail
+    if (reach == 1) {
+      if (lanes == 8) launch_cons<1, 8>(a, edges, st);
+      else if (lanes == 16) launch_cons<1, 16>(a, edges, st);
+      else return hipErrorInvalidValue;
+    } else {
+      if (lanes == 8) launch_cons<2, 8>(a, edges, st);
+      else if (lanes == 16) launch_cons<2, 16>(a, edges, st);
       else return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -57,10 +57,12 @@ static inline bool int_field(const JVal& e, const char* k, int* out) {
 static inline bool valid(const cf_tuning& t) {
   const bool lanes_ok = t.lanes_per_particle == 4 || t.lanes_per_particle == 8 ||
                         t.lanes_per_particle == 16 || t.lanes_per_particle == 32;
-  const bool list_ok = t.force_variant != 3 || t.lanes_per_particle >= 4 * t.cell_divisor;
+  const bool list_ok = (t.force_variant != 3 || t.lanes_per_particle >= 4 * t.cell_divisor) &&
+                       (t.force_variant != 4 || t.lanes_per_particle == 8 ||
+                        t.lanes_per_particle == 16);
   return (t.cell_divisor == 1 || t.cell_divisor == 2) &&
          (t.block_size == 64 || t.block_size == 128 || t.block_size == 256) &&
-         (t.force_variant >= 0 && t.force_variant <= 3) && lanes_ok && list_ok &&
+         (t.force_variant >= 0 && t.force_variant <= 4) && lanes_ok && list_ok &&
          t.steps_per_graph >= 1 && (t.sort_variant == 0 || t.sort_variant == 1);
 }
 

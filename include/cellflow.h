@@ -86,8 +86,10 @@ typedef struct cf_tuning {
   int force_variant;            /* 0: G lanes per particle walking the stencil's runs;
                                    1: one lane per particle; 2: LDS-staged stencil, one wave
                                    per workgroup; 3: G lanes per particle over a per-particle
-                                   candidate list in LDS (G >= 8 with cell_divisor 2) */
-  int lanes_per_particle;       /* G of variants 0 and 3: 4, 8, 16 or 32 */
+                                   candidate list in LDS (G >= 8 with cell_divisor 2); 4: 3
+                                   with each workgroup's integration done by one wave (G 8
+                                   or 16) */
+  int lanes_per_particle;       /* G of variants 0, 3 and 4: 4, 8, 16 or 32 */
   int block_size;               /* threads per workgroup of variant 1: 64, 128, 256 */
   int steps_per_graph;          /* steps per captured HIP graph (rounded up to even);
                                    1 launches every kernel directly */

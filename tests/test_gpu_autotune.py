@@ -27,9 +27,9 @@ def test_autotune_leaves_state_and_matches_direct_tuning(tmp_path, edges):
     res = sim.autotune(params, edges=edges, trial_steps=8)
     assert snapshot(sim) == before
     trials = res["trials"]
-    # 23 force-kernel candidates with the tile sort, then the best two with the counting sort
-    assert len(trials) == 25 and not res["cached"]
-    assert [t["sort_variant"] for t in trials] == [1] * 23 + [0] * 2
+    # 27 force-kernel candidates with the tile sort, then the best two with the counting sort
+    assert len(trials) == 29 and not res["cached"]
+    assert [t["sort_variant"] for t in trials] == [1] * 27 + [0] * 2
     timed = [t for t in trials if not t["pruned"]]
     best = min(timed, key=lambda t: t["us_per_step"])
     assert res["us_per_step"] == pytest.approx(best["us_per_step"], rel=1e-3)
@@ -79,7 +79,7 @@ def test_cache_hit_miss_and_force(tmp_path):
     expect = dict(r1["tuning"], steps_per_graph=8)  # the caller's graph length is kept
     assert r2["tuning"] == expect and b.getTuning() == expect
     r3 = b.autotune(params, trial_steps=4, cache=path, force=True)
-    assert not r3["cached"] and len(r3["trials"]) == 25
+    assert not r3["cached"] and len(r3["trials"]) == 29
 
 
 def test_ensemble_autotune_leaves_state():

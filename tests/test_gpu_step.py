@@ -353,7 +353,8 @@ def test_edge_capacity_error():
 # every combination the autotuner times: (variant, lanes, divisor, block size)
 VARIANTS = [(0, g, d, 128) for d in (1, 2) for g in (4, 8, 16, 32)] + \
     [(1, 16, d, b) for d in (1, 2) for b in (64, 128, 256)] + [(2, 16, 1, 128), (2, 16, 2, 128)] + \
-    [(3, g, 1, 128) for g in (4, 8, 16, 32)] + [(3, g, 2, 128) for g in (8, 16, 32)]
+    [(3, g, 1, 128) for g in (4, 8, 16, 32)] + [(3, g, 2, 128) for g in (8, 16, 32)] + \
+    [(4, g, d, 128) for d in (1, 2) for g in (8, 16)]
 
 
 @pytest.mark.parametrize("sort", [1, 0])
@@ -384,7 +385,24 @@ def test_variants_match_oracle(orc, variant, lanes, divisor, block, name, sort):
     check_step_edges(orc, sim, twin, params, (0.02, 6))
 
 
-@pytest.mark.parametrize("variant,lanes", [(0, 16), (3, 8)])
+def test_consolidated_tail_bit_identical_to_list_kernel():
+    """Variant 4 differs from variant 3 only in which lanes integrate: the rollouts (state,
+    counts, both graphs) are bit for bit the same."""
+    st, phys, params = state_of(small_cfg(n=5000, side=0.6))
+    out = []
+    for variant in (3, 4):
+        sim = make_sim(st, phys)
+        sim.setTuning(force_variant=variant, lanes_per_particle=8)
+        sim.simulate(params, 12, edges=(0.03, 8))
+        pos, vel, ids = sim.getState()
+        out.append((pos.tobytes(), vel.tobytes(), ids.tobytes(), sim.getEdges().tobytes(),
+                    sim.getStepEdges().tobytes(), sim.getNeighborCounts().tobytes(),
+                    sim.getParticleData().tobytes()))
+        sim.close()
+    assert out[0] == out[1]
+
+
+@pytest.mark.parametrize("variant,lanes", [(0, 16), (3, 8), (4, 8)])
 def test_both_sorts_bit_identical(variant, lanes):
     """The two step sorts produce the same canonical order, so a rollout is bit for bit the
     same with either (here with fused edges every step)."""
