@@ -189,7 +189,6 @@ __device__ __forceinline__ void commit_cell(const ForceArgs& a, const SimDesc& s
     }
   }
   a.key[p] = k;
-  atomicAdd(&a.hist[c], 1u);
   // one tile-count atomic per distinct tile among the wave's particles (they are
   // consecutive slots, so nearly always one)
   const int lane = threadIdx.x & (kWave - 1);
@@ -1129,14 +1128,43 @@ struct Own {
   float cut2;           // pruning radius^2 of the particle's type
   bool gen;             // grid too small for image-shifted runs
   int nyo, nzo;         // stencil rows per axis
+  // cell_gap(-k, f) * h and cell_gap(k, f) * h per axis, k = 1..2 (grids with image-shifted
+  // runs only): the gaps to the k-th cells below and above
+  float gxl[2], gxr[2], gyl[2], gyr[2], gzl[2], gzr[2];
 };
+
+// The gaps of Own for reach R (the same float operations as cell_gap).
+template <int R>
+__device__ __forceinline__ void own_gaps(Own& c, const SimDesc& sd) {
+#pragma unroll
+  for (int k = 1; k <= R; ++k) {
+    c.gxl[k - 1] = cell_gap(-k, c.fxr) * sd.hx;
+    c.gxr[k - 1] = cell_gap(k, c.fxr) * sd.hx;
+    c.gyl[k - 1] = cell_gap(-k, c.fyr) * sd.hy;
+    c.gyr[k - 1] = cell_gap(k, c.fyr) * sd.hy;
+    c.gzl[k - 1] = cell_gap(-k, c.fzr) * sd.hz;
+    c.gzr[k - 1] = cell_gap(k, c.fzr) * sd.hz;
+  }
+}
+
+// cell_gap(d, f) * h for |d| <= R from the precomputed gaps (d = 0: 0).
+template <int R>
+__device__ __forceinline__ float gap_of(int d, const float* lo, const float* hi) {
+  float g = 0.0f;
+#pragma unroll
+  for (int k = 1; k <= R; ++k) {
+    g = (d == -k) ? lo[k - 1] : g;
+    g = (d == k) ? hi[k - 1] : g;
+  }
+  return g;
+}
 
 // The up to two runs of stencil row r (x-neighbour cells are adjacent slots): the
 // unwrapped part and the part wrapped across the x boundary, each with its image-shift
 // code (x: bits 0-1, y: 2-3, z: 4-5; 1 = -L, 2 = +L).  Rows and x ends farther than the
 // type's reach (with margins) are dropped.  Length 0 means no run.
 template <int R>
-__device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, const Own& c,
+__device__ __forceinline__ void row_runs_gen(const ForceArgs& a, const SimDesc& sd, const Own& c,
                                          int r, uint32_t& s0, uint32_t& l0, uint32_t& k0,
                                          uint32_t& s1, uint32_t& l1, uint32_t& k1) {
   s0 = l0 = k0 = s1 = l1 = k1 = 0u;
@@ -1206,6 +1234,57 @@ __device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, 
     l1 = a.start[rowc + b1 + 1] - s1;
     k1 = yzc | xc1;
   }
+}
+
+// row_runs on a grid with at least 2R+1 cells per axis, straight-line: the row's y and z
+// gaps from Own's precomputed ones, and the x ends by counting the x offsets within reach
+// (the gap grows with the offset, so those offsets are 1..K on each side and the count is
+// where cell-by-cell trimming from the far end stops).  Same runs as row_runs_gen.
+template <int R>
+__device__ __forceinline__ void row_runs_fast(const ForceArgs& a, const SimDesc& sd,
+                                              const Own& c, int r, uint32_t& s0, uint32_t& l0,
+                                              uint32_t& k0, uint32_t& s1, uint32_t& l1,
+                                              uint32_t& k1) {
+  s0 = l0 = k0 = s1 = l1 = k1 = 0u;
+  constexpr int D = 2 * R + 1;
+  const int dy = r % D - R, dz = r / D - R;
+  const float gy = gap_of<R>(dy, c.gyl, c.gyr), gz = gap_of<R>(dz, c.gzl, c.gzr);
+  const float gyz = fmaf(gy, gy, gz * gz);
+  if (!(gyz < c.cut2)) return;
+  int y = c.cy + dy, z = c.cz + dz;
+  uint32_t cyc = 0u, czc = 0u;
+  if (y < 0) { y += sd.ny; cyc = 1u; }
+  else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
+  if (z < 0) { z += sd.nz; czc = 1u; }
+  else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
+  const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+  int xlo = c.cx, xhi = c.cx;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    xlo -= (fmaf(c.gxl[k], c.gxl[k], gyz) < c.cut2) ? 1 : 0;
+    xhi += (fmaf(c.gxr[k], c.gxr[k], gyz) < c.cut2) ? 1 : 0;
+  }
+  const uint32_t yzc = (cyc << 2) | (czc << 4);
+  int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
+  uint32_t xc1 = 0u;
+  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
+  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
+  s0 = a.start[rowc + a0];
+  l0 = a.start[rowc + b0 + 1] - s0;
+  k0 = yzc;
+  if (a1 <= b1) {
+    s1 = a.start[rowc + a1];
+    l1 = a.start[rowc + b1 + 1] - s1;
+    k1 = yzc | xc1;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, const Own& c,
+                                         int r, uint32_t& s0, uint32_t& l0, uint32_t& k0,
+                                         uint32_t& s1, uint32_t& l1, uint32_t& k1) {
+  if (c.gen) row_runs_gen<R>(a, sd, c, r, s0, l0, k0, s1, l1, k1);
+  else row_runs_fast<R>(a, sd, c, r, s0, l0, k0, s1, l1, k1);
 }
 
 // ParticleSimulation.cu:136-161 for one particle: density factor, friction-Euler step and
@@ -1293,6 +1372,7 @@ k_force_list(ForceArgs a) {
   c.fxr = fminf(fmaxf(pi.x * sd.ihx - (float)c.cx, 0.0f), 1.0f);
   c.fyr = fminf(fmaxf(pi.y * sd.ihy - (float)c.cy, 0.0f), 1.0f);
   c.fzr = fminf(fmaxf(pi.z * sd.ihz - (float)c.cz, 0.0f), 1.0f);
+  own_gaps<R>(c, sd);
   if (EDGES && g == 0) el_n[wv][o] = 0u;
   // the pruning radius straight from the pair table in memory, so the stencil rows (and
   // their cell-start loads) overlap the copy of the table into LDS
@@ -1538,9 +1618,10 @@ __device__ __forceinline__ bool edge_row(const SimDesc& sd, int cx, int cy, int 
 }
 
 // Runs a lane's stencil rows r = r0, r0 + G, ... (kEdgeRowBatch of them at a time, their
-// cell-start loads issued together) and calls f(q, pj, jid) for each slot q, positions and
-// indices loaded kEdgeLoadBatch at a time (an index is loaded with its position, so a
-// qualifying candidate waits for no further load).
+// cell-start loads issued together) and calls f(q, pj, jid) for each slot q, in row order.
+// The batch's slot runs are walked as one sequence, kEdgeLoadBatch slots at a time, so the
+// loads of several short rows are in flight together; an index is loaded with its
+// position, so a qualifying candidate waits for no further load.
 constexpr int kEdgeRowBatch = 3;
 constexpr int kEdgeLoadBatch = 4;
 
@@ -1556,27 +1637,40 @@ __device__ __forceinline__ void edge_walk(const EdgeArgs& a, const SimDesc& sd, 
       const int r = rb + i * step;
       on[i] = r < nrows && edge_row(sd, cx, cy, cz, fxr, fyr, fzr, cut2, r, c0[i], c1[i]);
     }
-    int qa[kEdgeRowBatch], qb[kEdgeRowBatch];
+    int qa[kEdgeRowBatch], len[kEdgeRowBatch];
 #pragma unroll
     for (int i = 0; i < kEdgeRowBatch; ++i) {
       qa[i] = on[i] ? (int)a.start[c0[i]] : 0;
-      qb[i] = on[i] ? (int)a.start[c1[i] + 1] : 0;
+      len[i] = on[i] ? (int)a.start[c1[i] + 1] : 0;
     }
+    int total = 0;
 #pragma unroll
-    for (int i = 0; i < kEdgeRowBatch; ++i)
-      for (int q = qa[i]; q < qb[i]; q += kEdgeLoadBatch) {
-        float4 pj[kEdgeLoadBatch];
-        uint32_t jid[kEdgeLoadBatch];
+    for (int i = 0; i < kEdgeRowBatch; ++i) {
+      len[i] -= qa[i];
+      total += len[i];
+    }
+    for (int x0 = 0; x0 < total; x0 += kEdgeLoadBatch) {
+      int qs[kEdgeLoadBatch];
+      float4 pj[kEdgeLoadBatch];
+      uint32_t jid[kEdgeLoadBatch];
 #pragma unroll
-        for (int u = 0; u < kEdgeLoadBatch; ++u) {
-          const bool in = q + u < qb[i];
-          pj[u] = in ? a.pos[q + u] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-          jid[u] = in ? a.id[q + u] : 0u;
+      for (int u = 0; u < kEdgeLoadBatch; ++u) {
+        const int x = x0 + u;
+        int q = qa[0] + x, run = len[0];  // x's row: the first whose runs reach past x
+#pragma unroll
+        for (int i = 1; i < kEdgeRowBatch; ++i) {
+          if (x >= run) q = qa[i] + (x - run);
+          run += len[i];
         }
-#pragma unroll
-        for (int u = 0; u < kEdgeLoadBatch; ++u)
-          if (q + u < qb[i]) f(q + u, pj[u], jid[u]);
+        qs[u] = q;
+        const bool in = x < total;
+        pj[u] = in ? a.pos[q] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        jid[u] = in ? a.id[q] : 0u;
       }
+#pragma unroll
+      for (int u = 0; u < kEdgeLoadBatch; ++u)
+        if (x0 + u < total) f(qs[u], pj[u], jid[u]);
+    }
   }
 }
 
@@ -1732,7 +1826,14 @@ __device__ __forceinline__ void block_sums_excl_u32(uint32_t v, uint32_t u, uint
   wpre = pre + incl - w;
 }
 
-__device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
+// An edge tile's owners' edges: the counts and the other tiles' totals give the tile's
+// place in the list, then the block writes the tile's edges e = tid, tid + 256, ...: each
+// finds its owner in LDS (the last thread whose first edge is at or before e, then that
+// thread's owners in order) and its rank k = e - the owner's first edge, so a thread's
+// staged-partner loads are all in flight together and the stores are contiguous.
+// `toff`, `p0s` (kSortThreads words) and `ecs` (kEdgeTile words) are LDS scratch.
+__device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red, uint32_t* toff,
+                          uint32_t* p0s, uint32_t* ecs) {
   const int tid = threadIdx.x;
   if (tid < kEdgeSub) a.etcount_next[(u * kEdgeSub + tid) * kCtrStride] = 0u;
   // the owners' edge counts and their simulation's first slot (a thread's kPer owners share
@@ -1758,20 +1859,57 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red) {
   for (int i = 0; i < kPer; ++i) run += ec[i];
   uint32_t own, base, pre, total;
   block_sums_excl_u32(mine, part, run, red, own, base, pre, total);
-  uint32_t off = base + pre;
   if (total != own || (long)base + total > a.edge_room) {
     if (tid == 0) atomicOr(a.flag, 4u);
     return;
   }
   if (tid == 0 && u == a.netiles - 1) *a.etotal = base + total;  // the whole list's size
+  toff[tid] = pre;
+  p0s[tid] = (uint32_t)p0;
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    if (ec[i]) {
-      const long id = id0 + i;
-      for (uint32_t k = 0; k < ec[i]; ++k)
-        a.edges[off + k] = make_int2((int)(id - p0), (int)(a.estage[(long)k * a.P + id] - p0));
+  for (int i = 0; i < kPer; ++i) ecs[tid * kPer + i] = ec[i];
+  __syncthreads();
+  constexpr int kB = 4;  // edges per thread with their loads in flight together
+  for (uint32_t e0 = 0; e0 < total; e0 += kSortThreads * kB) {
+    long src[kB];
+    int own[kB];
+    uint32_t ob[kB];
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+      const uint32_t e = e0 + (uint32_t)(b * kSortThreads + tid);
+      src[b] = 0;
+      own[b] = 0;
+      ob[b] = 0u;
+      if (e < total) {
+        int t = 0;  // toff[0] = 0 <= e; the steps reach at most index 255
+#pragma unroll
+        for (int st = kSortThreads / 2; st > 0; st >>= 1)
+          if (toff[t + st] <= e) t += st;
+        uint32_t r = e - toff[t];
+        int i = 0;
+#pragma unroll
+        for (; i < kPer - 1; ++i) {
+          const uint32_t c = ecs[t * kPer + i];
+          if (r < c) break;
+          r -= c;
+        }
+        const long id = (long)u * kEdgeTile + (long)t * kPer + i;
+        src[b] = (long)r * a.P + id;
+        own[b] = (int)(id - (long)p0s[t]);
+        ob[b] = p0s[t];
+      }
     }
-    off += ec[i];
+    uint32_t jv[kB];
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+      const uint32_t e = e0 + (uint32_t)(b * kSortThreads + tid);
+      jv[b] = (e < total) ? a.estage[src[b]] : 0u;
+    }
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+      const uint32_t e = e0 + (uint32_t)(b * kSortThreads + tid);
+      if (e < total) a.edges[(long)base + e] = make_int2(own[b], (int)(jv[b] - ob[b]));
+    }
   }
 }
 
@@ -1782,12 +1920,18 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   __shared__ uint32_t mslot[kTileCap];
   __shared__ uint32_t mid[kTileCap];
   __shared__ uint16_t mlc[kTileCap];
+  __shared__ uint32_t aq[kTileCap];  // members in visit order (slot, index, local cell)
+  __shared__ uint32_t aid[kTileCap];
+  __shared__ uint16_t alc[kTileCap];
   __shared__ uint32_t red[3 * (kSortThreads / kWave)];
   __shared__ uint32_t wq0[6], wq1[6];  // the window's slot runs
   __shared__ int s_next;
+  __shared__ uint32_t s_app;
   const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
   if ((int)blockIdx.x >= a.ntiles) {
-    edge_tile(a, (int)blockIdx.x - a.ntiles, red);
+    static_assert(kTileCells >= kEdgeTile && kTileCells >= kSortThreads, "edge tile scratch");
+    edge_tile(a, (int)blockIdx.x - a.ntiles, red, cofs, fill_s, cnt_s);
     return;
   }
   // consecutive tiles (consecutive slots) share an XCD, as in the force pass
@@ -1825,48 +1969,13 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   const uint32_t nfar_t = min(a.ftn[t * kCtrStride], (uint32_t)kFarCap);
   const uint32_t nfar = min(*a.far_n, (uint32_t)a.P);
 
-  // first slot of the tile, then its cells' starts (counts cleared for the next step)
-  uint32_t part = 0;
-  for (int u = sd.t0 + tid; u < t; u += kSortThreads) part += a.tcount[u * kCtrStride];
-  const int per = (nc + kSortThreads - 1) / kSortThreads;
-  const int i0 = min(tid * per, nc), i1 = min(i0 + per, nc);
-  uint32_t run = 0;
-  for (int i = i0; i < i1; ++i) {
-    const uint32_t v = a.hist[cA + i];
-    cnt_s[i] = v;
-    run += v;
-  }
-  uint32_t before, unused, pre, M;
-  block_sums_excl_u32(part, 0u, run, red, before, unused, pre, M);
-  const uint32_t base = (uint32_t)sd.p0 + before;
-  // The counts must agree (and the simulation's last tile must end at its last particle);
-  // if they do not, the tile writes nothing but clamped starts, and the call fails.
-  const uint32_t lim = (uint32_t)(sd.p0 + sd.n);
-  const bool last = lt == sd.nz * sd.nyt - 1;
-  const bool ok = M == a.tcount[t * kCtrStride] && base + M <= lim && (!last || base + M == lim);
-  for (int i = i0; i < i1; ++i) {
-    cofs[i] = pre;
-    a.start_new[cA + i] = min(base + pre, lim);
-    a.hist[cA + i] = 0u;
-    pre += cnt_s[i];
-  }
-  if (tid == 0) {
-    cofs[nc] = M;
-    if (!ok) atomicOr(a.flag, 4u);
-    if (last) {  // the simulation's pad cell, and the start of what follows it
-      a.start_new[sd.c0 + sd.ncells] = lim;
-      a.start_new[sd.c0 + sd.ncells + 1] =
-          (uint32_t)(sd.p0 + (sd.n + kSlotAlign - 1) / kSlotAlign * kSlotAlign);
-    }
-  }
-  __syncthreads();
-  if (!ok) return;
-
-  // fn(q, key, id) for every slot of the window and of the far lists (the index is loaded
-  // with the key: most slots are not members, but a dependent load per member costs more).
-  // The window's six runs are taken as one sequence, kVisitBatch slots per thread with all
-  // their loads in flight before any is used; fn's order does not matter (ranks are by
-  // index).
+  // fn(valid, q, key, id) for every slot of the window and of the far lists, called by
+  // every lane of the block the same number of times (valid: a slot whose key is not a far
+  // mover's; far movers come from the lists, key without the far bit).  The index is
+  // loaded with the key: most slots are not members, but a dependent load per member costs
+  // more.  The window's six runs are taken as one sequence, kVisitBatch slots per thread
+  // with all their loads in flight before any is used; fn's order does not matter (ranks
+  // are by index).
   auto visit = [&](auto&& fn) {
     uint32_t wp[7];
     wp[0] = 0u;
@@ -1892,18 +2001,110 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
         }
       }
 #pragma unroll
-      for (int u = 0; u < kVisitBatch; ++u)
-        if (!(kv[u] & kFarBit)) fn(qv[u], kv[u], iv[u]);
+      for (int u = 0; u < kVisitBatch; ++u) fn(!(kv[u] & kFarBit), qv[u], kv[u], iv[u]);
     }
-    for (uint32_t j = tid; j < nfar_t; j += kSortThreads) {
-      const uint32_t q = a.ftl[(long)t * kFarCap + j];
-      fn(q, a.key[q] & ~kFarBit, a.src_id[q]);
+    for (uint32_t j0 = 0; j0 < nfar_t; j0 += kSortThreads) {
+      const uint32_t j = j0 + (uint32_t)tid;
+      uint32_t q = 0u, k = 0u, id = 0u;
+      if (j < nfar_t) {
+        q = a.ftl[(long)t * kFarCap + j];
+        k = a.key[q] & ~kFarBit;
+        id = a.src_id[q];
+      }
+      fn(j < nfar_t, q, k, id);
     }
-    for (uint32_t j = tid; j < nfar; j += kSortThreads) {
-      const uint32_t q = a.far_slot[j];
-      fn(q, a.key[q] & ~kFarBit, a.src_id[q]);
+    for (uint32_t j0 = 0; j0 < nfar; j0 += kSortThreads) {
+      const uint32_t j = j0 + (uint32_t)tid;
+      uint32_t q = 0u, k = 0u, id = 0u;
+      if (j < nfar) {
+        q = a.far_slot[j];
+        k = a.key[q] & ~kFarBit;
+        id = a.src_id[q];
+      }
+      fn(j < nfar, q, k, id);
     }
   };
+
+  // The tile's cell counts come from its members.  Normally (the force pass counted at most
+  // kTileCap particles into the tile) one visit appends the members to LDS, a wave's
+  // members at one LDS atomic, and counts them per cell; else a visit only counts, and the
+  // cells are then sorted a chunk at a time below.
+  uint32_t part = 0;  // the particles of the simulation's tiles before this one
+  for (int u = sd.t0 + tid; u < t; u += kSortThreads) part += a.tcount[u * kCtrStride];
+  const uint32_t expect = a.tcount[t * kCtrStride];
+  for (int i = tid; i < nc; i += kSortThreads) {
+    cnt_s[i] = 0u;
+    fill_s[i] = 0u;
+  }
+  if (tid == 0) s_app = 0u;
+  __syncthreads();
+  const bool placed = expect <= (uint32_t)kTileCap;  // the same for the whole block
+  if (placed) {
+    visit([&](bool valid, uint32_t q, uint32_t k, uint32_t id) {
+      const bool mem = valid && k >= cA && k - cA < (uint32_t)nc;
+      const unsigned long long bal = __ballot(mem);
+      if (bal == 0ull) return;
+      const int lead = __ffsll((long long)bal) - 1;
+      uint32_t b0 = 0u;
+      if (lane == lead) b0 = atomicAdd(&s_app, (uint32_t)__popcll(bal));
+      b0 = (uint32_t)__shfl((int)b0, lead, kWave);
+      if (mem) {
+        const uint32_t j = b0 + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        const uint32_t lc = k - cA;
+        atomicAdd(&cnt_s[lc], 1u);
+        if (j < (uint32_t)kTileCap) {
+          aq[j] = q;
+          aid[j] = id;
+          alc[j] = (uint16_t)lc;
+        }
+      }
+    });
+  } else {
+    visit([&](bool valid, uint32_t, uint32_t k, uint32_t) {
+      if (valid && k >= cA && k - cA < (uint32_t)nc) atomicAdd(&cnt_s[k - cA], 1u);
+    });
+  }
+  __syncthreads();
+
+  // first slot of the tile, then its cells' starts
+  const int per = (nc + kSortThreads - 1) / kSortThreads;
+  const int i0 = min(tid * per, nc), i1 = min(i0 + per, nc);
+  uint32_t run = 0;
+  for (int i = i0; i < i1; ++i) run += cnt_s[i];
+  uint32_t before, unused, pre, M;
+  block_sums_excl_u32(part, 0u, run, red, before, unused, pre, M);
+  const uint32_t base = (uint32_t)sd.p0 + before;
+  // The members must be what the force pass counted into the tile (and the simulation's
+  // last tile must end at its last particle); if not, the tile writes nothing but clamped
+  // starts, and the call fails.
+  const uint32_t lim = (uint32_t)(sd.p0 + sd.n);
+  const bool last = lt == sd.nz * sd.nyt - 1;
+  const bool ok = M == expect && base + M <= lim && (!last || base + M == lim);
+  for (int i = i0; i < i1; ++i) {
+    cofs[i] = pre;
+    a.start_new[cA + i] = min(base + pre, lim);
+    pre += cnt_s[i];
+  }
+  if (tid == 0) {
+    cofs[nc] = M;
+    if (!ok) atomicOr(a.flag, 4u);
+    if (last) {  // the simulation's pad cell, and the start of what follows it
+      a.start_new[sd.c0 + sd.ncells] = lim;
+      a.start_new[sd.c0 + sd.ncells + 1] =
+          (uint32_t)(sd.p0 + (sd.n + kSlotAlign - 1) / kSlotAlign * kSlotAlign);
+    }
+  }
+  __syncthreads();
+  if (!ok) return;
+  if (placed) {  // group the appended members by cell (M == expect <= kTileCap: all kept)
+    for (uint32_t m = tid; m < M; m += kSortThreads) {
+      const uint32_t lc = alc[m];
+      const uint32_t at = cofs[lc] + atomicAdd(&fill_s[lc], 1u);
+      mslot[at] = aq[m];
+      mid[at] = aid[m];
+      mlc[at] = (uint16_t)lc;
+    }
+  }
 
   // cells [ia, ib) at a time, as many as kTileCap members allow (all of them, normally)
   int ia = 0;
@@ -1924,23 +2125,25 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
     }
     const uint32_t m0 = cofs[ia], mn = cofs[ib] - m0;
     if (mn <= (uint32_t)kTileCap) {
-      for (int i = ia + tid; i < ib; i += kSortThreads) fill_s[i] = 0u;
-      for (uint32_t m = tid; m < mn; m += kSortThreads) mlc[m] = 0xFFFFu;
-      __syncthreads();
-      const uint32_t ca = cA + (uint32_t)ia, cb = cA + (uint32_t)ib;
-      visit([&](uint32_t q, uint32_t k, uint32_t id) {
-        if (k < ca || k >= cb) return;
-        const uint32_t lc = k - cA;
-        const uint32_t at = atomicAdd(&fill_s[lc], 1u);
-        if (at >= cnt_s[lc]) {
-          atomicOr(a.flag, 4u);
-          return;
-        }
-        const uint32_t m = cofs[lc] - m0 + at;
-        mslot[m] = q;
-        mid[m] = id;
-        mlc[m] = (uint16_t)lc;
-      });
+      if (!placed) {
+        for (int i = ia + tid; i < ib; i += kSortThreads) fill_s[i] = 0u;
+        for (uint32_t m = tid; m < mn; m += kSortThreads) mlc[m] = 0xFFFFu;
+        __syncthreads();
+        const uint32_t ca = cA + (uint32_t)ia, cb = cA + (uint32_t)ib;
+        visit([&](bool valid, uint32_t q, uint32_t k, uint32_t id) {
+          if (!valid || k < ca || k >= cb) return;
+          const uint32_t lc = k - cA;
+          const uint32_t at = atomicAdd(&fill_s[lc], 1u);
+          if (at >= cnt_s[lc]) {
+            atomicOr(a.flag, 4u);
+            return;
+          }
+          const uint32_t m = cofs[lc] - m0 + at;
+          mslot[m] = q;
+          mid[m] = id;
+          mlc[m] = (uint16_t)lc;
+        });
+      }
       __syncthreads();
       // kMoveBatch members per thread: their state loads are in flight during the ranking
       for (uint32_t mb = 0; mb < mn; mb += kMoveBatch * kSortThreads) {
@@ -1987,8 +2190,8 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
         atomicOr(a.flag, 2u);
       }
       __syncthreads();
-      visit([&](uint32_t q, uint32_t k, uint32_t id) {
-        if (k != cg) return;
+      visit([&](bool valid, uint32_t q, uint32_t k, uint32_t id) {
+        if (!valid || k != cg) return;
         const uint32_t at = atomicAdd(&fill_s[ia], 1u);
         if (at >= K) {
           atomicOr(a.flag, 4u);
