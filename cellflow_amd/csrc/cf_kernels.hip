@@ -721,6 +721,174 @@ __device__ __forceinline__ float shift_of(uint32_t c, float L) {
   return (c == 1u) ? -L : ((c == 2u) ? L : 0.0f);
 }
 
+// Per-particle context of the stencil walk.
+struct Own {
+  float4 pi;
+  uint32_t ti;
+  int cx, cy, cz;
+  float fxr, fyr, fzr;  // fractional position inside the cell, per axis
+  float cut2;           // pruning radius^2 of the particle's type
+  bool gen;             // grid too small for image-shifted runs
+  int nyo, nzo;         // stencil rows per axis
+  // cell_gap(-k, f) * h and cell_gap(k, f) * h per axis, k = 1..2 (grids with image-shifted
+  // runs only): the gaps to the k-th cells below and above
+  float gxl[2], gxr[2], gyl[2], gyr[2], gzl[2], gzr[2];
+};
+
+// The gaps of Own for reach R (the same float operations as cell_gap).
+template <int R>
+__device__ __forceinline__ void own_gaps(Own& c, const SimDesc& sd) {
+#pragma unroll
+  for (int k = 1; k <= R; ++k) {
+    c.gxl[k - 1] = cell_gap(-k, c.fxr) * sd.hx;
+    c.gxr[k - 1] = cell_gap(k, c.fxr) * sd.hx;
+    c.gyl[k - 1] = cell_gap(-k, c.fyr) * sd.hy;
+    c.gyr[k - 1] = cell_gap(k, c.fyr) * sd.hy;
+    c.gzl[k - 1] = cell_gap(-k, c.fzr) * sd.hz;
+    c.gzr[k - 1] = cell_gap(k, c.fzr) * sd.hz;
+  }
+}
+
+// cell_gap(d, f) * h for |d| <= R from the precomputed gaps (d = 0: 0).
+template <int R>
+__device__ __forceinline__ float gap_of(int d, const float* lo, const float* hi) {
+  float g = 0.0f;
+#pragma unroll
+  for (int k = 1; k <= R; ++k) {
+    g = (d == -k) ? lo[k - 1] : g;
+    g = (d == k) ? hi[k - 1] : g;
+  }
+  return g;
+}
+
+// The up to two runs of stencil row r (x-neighbour cells are adjacent slots): the
+// unwrapped part and the part wrapped across the x boundary, each with its image-shift
+// code (x: bits 0-1, y: 2-3, z: 4-5; 1 = -L, 2 = +L).  Rows and x ends farther than the
+// type's reach (with margins) are dropped.  Length 0 means no run.
+template <int R>
+__device__ __forceinline__ void row_runs_gen(const ForceArgs& a, const SimDesc& sd, const Own& c,
+                                         int r, uint32_t& s0, uint32_t& l0, uint32_t& k0,
+                                         uint32_t& s1, uint32_t& l1, uint32_t& k1) {
+  s0 = l0 = k0 = s1 = l1 = k1 = 0u;
+  int iy, iz;
+  if (c.gen) {  // uniform: the simulation's grid decides
+    iy = r % c.nyo;
+    iz = r / c.nyo;
+  } else {  // nyo == 2R+1: division by a constant
+    iy = r % (2 * R + 1);
+    iz = r / (2 * R + 1);
+  }
+  int y, z, dy = 0, dz = 0;
+  uint32_t cyc = 0u, czc = 0u;
+  if (c.gen && sd.ny < 2 * R + 1) {
+    y = iy;
+  } else {
+    dy = iy - R;
+    y = c.cy + dy;
+    if (y < 0) { y += sd.ny; cyc = 1u; }
+    else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
+  }
+  if (c.gen && sd.nz < 2 * R + 1) {
+    z = iz;
+  } else {
+    dz = iz - R;
+    z = c.cz + dz;
+    if (z < 0) { z += sd.nz; czc = 1u; }
+    else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
+  }
+  float gyz = 0.0f;
+  if (!c.gen) {
+    const float gy = cell_gap(dy, c.fyr) * sd.hy, gz = cell_gap(dz, c.fzr) * sd.hz;
+    gyz = fmaf(gy, gy, gz * gz);
+    if (!(gyz < c.cut2)) return;
+  }
+  const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+  int xlo, xhi;
+  if (c.gen && sd.nx < 2 * R + 1) {
+    xlo = 0;
+    xhi = sd.nx - 1;
+  } else {
+    xlo = c.cx - R;
+    xhi = c.cx + R;
+    if (!c.gen) {
+      while (xlo < c.cx) {
+        const float gx = cell_gap(xlo - c.cx, c.fxr) * sd.hx;
+        if (fmaf(gx, gx, gyz) < c.cut2) break;
+        ++xlo;
+      }
+      while (xhi > c.cx) {
+        const float gx = cell_gap(xhi - c.cx, c.fxr) * sd.hx;
+        if (fmaf(gx, gx, gyz) < c.cut2) break;
+        --xhi;
+      }
+    }
+  }
+  const uint32_t yzc = (cyc << 2) | (czc << 4);
+  int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
+  uint32_t xc1 = 0u;
+  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
+  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
+  s0 = a.start[rowc + a0];
+  l0 = a.start[rowc + b0 + 1] - s0;
+  k0 = yzc;
+  if (a1 <= b1) {
+    s1 = a.start[rowc + a1];
+    l1 = a.start[rowc + b1 + 1] - s1;
+    k1 = yzc | xc1;
+  }
+}
+
+// row_runs on a grid with at least 2R+1 cells per axis, straight-line: the row's y and z
+// gaps from Own's precomputed ones, and the x ends by counting the x offsets within reach
+// (the gap grows with the offset, so those offsets are 1..K on each side and the count is
+// where cell-by-cell trimming from the far end stops).  Same runs as row_runs_gen.
+template <int R>
+__device__ __forceinline__ void row_runs_fast(const ForceArgs& a, const SimDesc& sd,
+                                              const Own& c, int r, uint32_t& s0, uint32_t& l0,
+                                              uint32_t& k0, uint32_t& s1, uint32_t& l1,
+                                              uint32_t& k1) {
+  s0 = l0 = k0 = s1 = l1 = k1 = 0u;
+  constexpr int D = 2 * R + 1;
+  const int dy = r % D - R, dz = r / D - R;
+  const float gy = gap_of<R>(dy, c.gyl, c.gyr), gz = gap_of<R>(dz, c.gzl, c.gzr);
+  const float gyz = fmaf(gy, gy, gz * gz);
+  if (!(gyz < c.cut2)) return;
+  int y = c.cy + dy, z = c.cz + dz;
+  uint32_t cyc = 0u, czc = 0u;
+  if (y < 0) { y += sd.ny; cyc = 1u; }
+  else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
+  if (z < 0) { z += sd.nz; czc = 1u; }
+  else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
+  const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+  int xlo = c.cx, xhi = c.cx;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    xlo -= (fmaf(c.gxl[k], c.gxl[k], gyz) < c.cut2) ? 1 : 0;
+    xhi += (fmaf(c.gxr[k], c.gxr[k], gyz) < c.cut2) ? 1 : 0;
+  }
+  const uint32_t yzc = (cyc << 2) | (czc << 4);
+  int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
+  uint32_t xc1 = 0u;
+  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
+  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
+  s0 = a.start[rowc + a0];
+  l0 = a.start[rowc + b0 + 1] - s0;
+  k0 = yzc;
+  if (a1 <= b1) {
+    s1 = a.start[rowc + a1];
+    l1 = a.start[rowc + b1 + 1] - s1;
+    k1 = yzc | xc1;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, const Own& c,
+                                         int r, uint32_t& s0, uint32_t& l0, uint32_t& k0,
+                                         uint32_t& s1, uint32_t& l1, uint32_t& k1) {
+  if (c.gen) row_runs_gen<R>(a, sd, c, r, s0, l0, k0, s1, l1, k1);
+  else row_runs_fast<R>(a, sd, c, r, s0, l0, k0, s1, l1, k1);
+}
+
 template <bool EDGES>
 struct GrpAcc {
   float fx, fy, fz;
@@ -773,76 +941,27 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   wave_lds_sync();  // pair table ready
 
   const PairConst* row = &tab[wv][ti * sd.T];
-  // ---- run table ------------------------------------------------------------------
+  // ---- run table: row_runs of each stencil row, as (start slot, length | code << 26) ----
   {
-    const float cut2 = row[0].pad;  // largest squared reach of this type, with margins
-    const float fxr = fminf(fmaxf(pi.x * sd.ihx - (float)cx, 0.0f), 1.0f);
-    const float fyr = fminf(fmaxf(pi.y * sd.ihy - (float)cy, 0.0f), 1.0f);
-    const float fzr = fminf(fmaxf(pi.z * sd.ihz - (float)cz, 0.0f), 1.0f);
+    Own c;
+    c.pi = pi;
+    c.ti = ti;
+    c.cx = cx;
+    c.cy = cy;
+    c.cz = cz;
+    c.gen = gen;
+    c.nyo = nyo;
+    c.nzo = nzo;
+    c.cut2 = row[0].pad;  // largest squared reach of this type, with margins
+    c.fxr = fminf(fmaxf(pi.x * sd.ihx - (float)cx, 0.0f), 1.0f);
+    c.fyr = fminf(fmaxf(pi.y * sd.ihy - (float)cy, 0.0f), 1.0f);
+    c.fzr = fminf(fmaxf(pi.z * sd.ihz - (float)cz, 0.0f), 1.0f);
+    own_gaps<R>(c, sd);
     for (int r = g; r < kRows; r += G) {
-      uint2 e0 = make_uint2(0u, 0u), e1 = make_uint2(0u, 0u);
-      if (active && r < nrows) {
-        const int iy = r % nyo, iz = r / nyo;
-        int y, z, dy = 0, dz = 0;
-        uint32_t cyc = 0u, czc = 0u;
-        if (gen && sd.ny < 2 * R + 1) {
-          y = iy;
-        } else {
-          dy = iy - R;
-          y = cy + dy;
-          if (y < 0) { y += sd.ny; cyc = 1u; }
-          else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
-        }
-        if (gen && sd.nz < 2 * R + 1) {
-          z = iz;
-        } else {
-          dz = iz - R;
-          z = cz + dz;
-          if (z < 0) { z += sd.nz; czc = 1u; }
-          else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
-        }
-        float gyz = 0.0f;
-        if (!gen) {
-          const float gy = cell_gap(dy, fyr) * sd.hy, gz = cell_gap(dz, fzr) * sd.hz;
-          gyz = fmaf(gy, gy, gz * gz);
-        }
-        if (gen || gyz < cut2) {
-          const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
-          int xlo, xhi;
-          if (gen && sd.nx < 2 * R + 1) {
-            xlo = 0;
-            xhi = sd.nx - 1;
-          } else {
-            xlo = cx - R;
-            xhi = cx + R;
-            if (!gen) {
-              while (xlo < cx) {
-                const float gx = cell_gap(xlo - cx, fxr) * sd.hx;
-                if (fmaf(gx, gx, gyz) < cut2) break;
-                ++xlo;
-              }
-              while (xhi > cx) {
-                const float gx = cell_gap(xhi - cx, fxr) * sd.hx;
-                if (fmaf(gx, gx, gyz) < cut2) break;
-                --xhi;
-              }
-            }
-          }
-          const uint32_t yzc = (cyc << 2) | (czc << 4);
-          int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
-          uint32_t xc1 = 0u;
-          if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
-          else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
-          const uint32_t s0 = a.start[rowc + a0];
-          e0 = make_uint2(s0, (a.start[rowc + b0 + 1] - s0) | (yzc << 26));
-          if (a1 <= b1) {
-            const uint32_t s1 = a.start[rowc + a1];
-            e1 = make_uint2(s1, (a.start[rowc + b1 + 1] - s1) | ((yzc | xc1) << 26));
-          }
-        }
-      }
-      runs[wv][o][2 * r] = e0;
-      runs[wv][o][2 * r + 1] = e1;
+      uint32_t s0 = 0u, l0 = 0u, k0 = 0u, s1 = 0u, l1 = 0u, k1 = 0u;
+      if (active && r < nrows) row_runs<R>(a, sd, c, r, s0, l0, k0, s1, l1, k1);
+      runs[wv][o][2 * r] = make_uint2(s0, l0 | (k0 << 26));
+      runs[wv][o][2 * r + 1] = make_uint2(s1, l1 | (k1 << 26));
     }
   }
   wave_lds_sync();
@@ -1117,174 +1236,6 @@ __device__ __forceinline__ uint32_t group_excl_scan(uint32_t v, int g) {
     if (g >= d) x += y;
   }
   return x - v;
-}
-
-// Per-particle context of the stencil walk.
-struct Own {
-  float4 pi;
-  uint32_t ti;
-  int cx, cy, cz;
-  float fxr, fyr, fzr;  // fractional position inside the cell, per axis
-  float cut2;           // pruning radius^2 of the particle's type
-  bool gen;             // grid too small for image-shifted runs
-  int nyo, nzo;         // stencil rows per axis
-  // cell_gap(-k, f) * h and cell_gap(k, f) * h per axis, k = 1..2 (grids with image-shifted
-  // runs only): the gaps to the k-th cells below and above
-  float gxl[2], gxr[2], gyl[2], gyr[2], gzl[2], gzr[2];
-};
-
-// The gaps of Own for reach R (the same float operations as cell_gap).
-template <int R>
-__device__ __forceinline__ void own_gaps(Own& c, const SimDesc& sd) {
-#pragma unroll
-  for (int k = 1; k <= R; ++k) {
-    c.gxl[k - 1] = cell_gap(-k, c.fxr) * sd.hx;
-    c.gxr[k - 1] = cell_gap(k, c.fxr) * sd.hx;
-    c.gyl[k - 1] = cell_gap(-k, c.fyr) * sd.hy;
-    c.gyr[k - 1] = cell_gap(k, c.fyr) * sd.hy;
-    c.gzl[k - 1] = cell_gap(-k, c.fzr) * sd.hz;
-    c.gzr[k - 1] = cell_gap(k, c.fzr) * sd.hz;
-  }
-}
-
-// cell_gap(d, f) * h for |d| <= R from the precomputed gaps (d = 0: 0).
-template <int R>
-__device__ __forceinline__ float gap_of(int d, const float* lo, const float* hi) {
-  float g = 0.0f;
-#pragma unroll
-  for (int k = 1; k <= R; ++k) {
-    g = (d == -k) ? lo[k - 1] : g;
-    g = (d == k) ? hi[k - 1] : g;
-  }
-  return g;
-}
-
-// The up to two runs of stencil row r (x-neighbour cells are adjacent slots): the
-// unwrapped part and the part wrapped across the x boundary, each with its image-shift
-// code (x: bits 0-1, y: 2-3, z: 4-5; 1 = -L, 2 = +L).  Rows and x ends farther than the
-// type's reach (with margins) are dropped.  Length 0 means no run.
-template <int R>
-__device__ __forceinline__ void row_runs_gen(const ForceArgs& a, const SimDesc& sd, const Own& c,
-                                         int r, uint32_t& s0, uint32_t& l0, uint32_t& k0,
-                                         uint32_t& s1, uint32_t& l1, uint32_t& k1) {
-  s0 = l0 = k0 = s1 = l1 = k1 = 0u;
-  int iy, iz;
-  if (c.gen) {  // uniform: the simulation's grid decides
-    iy = r % c.nyo;
-    iz = r / c.nyo;
-  } else {  // nyo == 2R+1: division by a constant
-    iy = r % (2 * R + 1);
-    iz = r / (2 * R + 1);
-  }
-  int y, z, dy = 0, dz = 0;
-  uint32_t cyc = 0u, czc = 0u;
-  if (c.gen && sd.ny < 2 * R + 1) {
-    y = iy;
-  } else {
-    dy = iy - R;
-    y = c.cy + dy;
-    if (y < 0) { y += sd.ny; cyc = 1u; }
-    else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
-  }
-  if (c.gen && sd.nz < 2 * R + 1) {
-    z = iz;
-  } else {
-    dz = iz - R;
-    z = c.cz + dz;
-    if (z < 0) { z += sd.nz; czc = 1u; }
-    else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
-  }
-  float gyz = 0.0f;
-  if (!c.gen) {
-    const float gy = cell_gap(dy, c.fyr) * sd.hy, gz = cell_gap(dz, c.fzr) * sd.hz;
-    gyz = fmaf(gy, gy, gz * gz);
-    if (!(gyz < c.cut2)) return;
-  }
-  const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
-  int xlo, xhi;
-  if (c.gen && sd.nx < 2 * R + 1) {
-    xlo = 0;
-    xhi = sd.nx - 1;
-  } else {
-    xlo = c.cx - R;
-    xhi = c.cx + R;
-    if (!c.gen) {
-      while (xlo < c.cx) {
-        const float gx = cell_gap(xlo - c.cx, c.fxr) * sd.hx;
-        if (fmaf(gx, gx, gyz) < c.cut2) break;
-        ++xlo;
-      }
-      while (xhi > c.cx) {
-        const float gx = cell_gap(xhi - c.cx, c.fxr) * sd.hx;
-        if (fmaf(gx, gx, gyz) < c.cut2) break;
-        --xhi;
-      }
-    }
-  }
-  const uint32_t yzc = (cyc << 2) | (czc << 4);
-  int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
-  uint32_t xc1 = 0u;
-  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
-  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
-  s0 = a.start[rowc + a0];
-  l0 = a.start[rowc + b0 + 1] - s0;
-  k0 = yzc;
-  if (a1 <= b1) {
-    s1 = a.start[rowc + a1];
-    l1 = a.start[rowc + b1 + 1] - s1;
-    k1 = yzc | xc1;
-  }
-}
-
-// row_runs on a grid with at least 2R+1 cells per axis, straight-line: the row's y and z
-// gaps from Own's precomputed ones, and the x ends by counting the x offsets within reach
-// (the gap grows with the offset, so those offsets are 1..K on each side and the count is
-// where cell-by-cell trimming from the far end stops).  Same runs as row_runs_gen.
-template <int R>
-__device__ __forceinline__ void row_runs_fast(const ForceArgs& a, const SimDesc& sd,
-                                              const Own& c, int r, uint32_t& s0, uint32_t& l0,
-                                              uint32_t& k0, uint32_t& s1, uint32_t& l1,
-                                              uint32_t& k1) {
-  s0 = l0 = k0 = s1 = l1 = k1 = 0u;
-  constexpr int D = 2 * R + 1;
-  const int dy = r % D - R, dz = r / D - R;
-  const float gy = gap_of<R>(dy, c.gyl, c.gyr), gz = gap_of<R>(dz, c.gzl, c.gzr);
-  const float gyz = fmaf(gy, gy, gz * gz);
-  if (!(gyz < c.cut2)) return;
-  int y = c.cy + dy, z = c.cz + dz;
-  uint32_t cyc = 0u, czc = 0u;
-  if (y < 0) { y += sd.ny; cyc = 1u; }
-  else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
-  if (z < 0) { z += sd.nz; czc = 1u; }
-  else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
-  const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
-  int xlo = c.cx, xhi = c.cx;
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    xlo -= (fmaf(c.gxl[k], c.gxl[k], gyz) < c.cut2) ? 1 : 0;
-    xhi += (fmaf(c.gxr[k], c.gxr[k], gyz) < c.cut2) ? 1 : 0;
-  }
-  const uint32_t yzc = (cyc << 2) | (czc << 4);
-  int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
-  uint32_t xc1 = 0u;
-  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
-  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
-  s0 = a.start[rowc + a0];
-  l0 = a.start[rowc + b0 + 1] - s0;
-  k0 = yzc;
-  if (a1 <= b1) {
-    s1 = a.start[rowc + a1];
-    l1 = a.start[rowc + b1 + 1] - s1;
-    k1 = yzc | xc1;
-  }
-}
-
-template <int R>
-__device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, const Own& c,
-                                         int r, uint32_t& s0, uint32_t& l0, uint32_t& k0,
-                                         uint32_t& s1, uint32_t& l1, uint32_t& k1) {
-  if (c.gen) row_runs_gen<R>(a, sd, c, r, s0, l0, k0, s1, l1, k1);
-  else row_runs_fast<R>(a, sd, c, r, s0, l0, k0, s1, l1, k1);
 }
 
 // ParticleSimulation.cu:136-161 for one particle: density factor, friction-Euler step and
@@ -1617,6 +1568,42 @@ __device__ __forceinline__ bool edge_row(const SimDesc& sd, int cx, int cy, int 
   return true;
 }
 
+// An owner of the standalone graph pass: its cell, and for reach ER (> 0) the gaps to
+// the k-th cells below and above per axis, cell_gap(-k, f) * h and cell_gap(k, f) * h.
+template <int ER>
+struct EdgeOwner {
+  int cx, cy, cz;
+  float fxr, fyr, fzr;
+  float gxl[ER > 0 ? ER : 1], gxr[ER > 0 ? ER : 1], gyl[ER > 0 ? ER : 1],
+      gyr[ER > 0 ? ER : 1], gzl[ER > 0 ? ER : 1], gzr[ER > 0 ? ER : 1];
+};
+
+// edge_row for reach ER on every axis, straight-line: the row's gaps from the owner's
+// precomputed ones, and the x ends by counting the x offsets within reach (the gap grows
+// with the offset, so those offsets are 1..K on each side: where edge_row's trimming from
+// the far end stops).  Same result as edge_row.
+template <int ER>
+__device__ __forceinline__ bool edge_row_fast(const SimDesc& sd, const EdgeOwner<ER>& w,
+                                              float cut2, int r, int& cell0, int& cell1) {
+  constexpr int D = 2 * ER + 1;
+  const int dy = r % D - ER, dz = r / D - ER;
+  const int y = w.cy + dy, z = w.cz + dz;
+  if (y < 0 || y >= sd.ny || z < 0 || z >= sd.nz) return false;
+  const float gy = gap_of<ER>(dy, w.gyl, w.gyr), gz = gap_of<ER>(dz, w.gzl, w.gzr);
+  const float gyz = fmaf(gy, gy, gz * gz);
+  if (!(gyz < cut2)) return false;
+  int xlo = w.cx, xhi = w.cx;
+#pragma unroll
+  for (int k = 0; k < ER; ++k) {
+    xlo -= (fmaf(w.gxl[k], w.gxl[k], gyz) < cut2) ? 1 : 0;
+    xhi += (fmaf(w.gxr[k], w.gxr[k], gyz) < cut2) ? 1 : 0;
+  }
+  const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
+  cell0 = rowc + max(xlo, 0);
+  cell1 = rowc + min(xhi, sd.nx - 1);
+  return true;
+}
+
 // Runs a lane's stencil rows r = r0, r0 + G, ... (kEdgeRowBatch of them at a time, their
 // cell-start loads issued together) and calls f(q, pj, jid) for each slot q, in row order.
 // The batch's slot runs are walked as one sequence, kEdgeLoadBatch slots at a time, so the
@@ -1625,17 +1612,22 @@ __device__ __forceinline__ bool edge_row(const SimDesc& sd, int cx, int cy, int 
 constexpr int kEdgeRowBatch = 3;
 constexpr int kEdgeLoadBatch = 4;
 
-template <class F>
-__device__ __forceinline__ void edge_walk(const EdgeArgs& a, const SimDesc& sd, int cx, int cy,
-                                          int cz, float fxr, float fyr, float fzr, float cut2,
-                                          int r0, int step, int nrows, F&& f) {
+// ER > 0: reach ER on every axis (edge_row_fast); ER = 0: any reach (edge_row).
+template <int ER, class F>
+__device__ __forceinline__ void edge_walk(const EdgeArgs& a, const SimDesc& sd,
+                                          const EdgeOwner<ER>& w, float cut2, int r0, int step,
+                                          int nrows, F&& f) {
   for (int rb = r0; rb < nrows; rb += kEdgeRowBatch * step) {
     int c0[kEdgeRowBatch], c1[kEdgeRowBatch];
     bool on[kEdgeRowBatch];
 #pragma unroll
     for (int i = 0; i < kEdgeRowBatch; ++i) {
       const int r = rb + i * step;
-      on[i] = r < nrows && edge_row(sd, cx, cy, cz, fxr, fyr, fzr, cut2, r, c0[i], c1[i]);
+      if constexpr (ER > 0)
+        on[i] = r < nrows && edge_row_fast<ER>(sd, w, cut2, r, c0[i], c1[i]);
+      else
+        on[i] = r < nrows && edge_row(sd, w.cx, w.cy, w.cz, w.fxr, w.fyr, w.fzr, cut2, r,
+                                      c0[i], c1[i]);
     }
     int qa[kEdgeRowBatch], len[kEdgeRowBatch];
 #pragma unroll
@@ -1674,6 +1666,102 @@ __device__ __forceinline__ void edge_walk(const EdgeArgs& a, const SimDesc& sd, 
   }
 }
 
+template <int ER>
+__device__ __forceinline__ void edges_owner(const EdgeArgs& a, const SimDesc& sd, int wv, int o,
+                                            int g, int p, bool active, const float4& pi,
+                                            uint32_t idi, float2 (*el_d)[kListEdges],
+                                            uint32_t (*el_i)[kListEdges], uint32_t* el_n) {
+  constexpr int G = kEdgeLanes;
+  const uint32_t ti = __float_as_uint(pi.w);
+  EdgeOwner<ER> w;
+  w.cx = bin_axis(pi.x, sd.ihx, sd.nx);
+  w.cy = bin_axis(pi.y, sd.ihy, sd.ny);
+  w.cz = bin_axis(pi.z, sd.ihz, sd.nz);
+  w.fxr = fminf(fmaxf(pi.x * sd.ihx - (float)w.cx, 0.0f), 1.0f);
+  w.fyr = fminf(fmaxf(pi.y * sd.ihy - (float)w.cy, 0.0f), 1.0f);
+  w.fzr = fminf(fmaxf(pi.z * sd.ihz - (float)w.cz, 0.0f), 1.0f);
+#pragma unroll
+  for (int k = 1; k <= ER; ++k) {
+    w.gxl[k - 1] = cell_gap(-k, w.fxr) * sd.hx;
+    w.gxr[k - 1] = cell_gap(k, w.fxr) * sd.hx;
+    w.gyl[k - 1] = cell_gap(-k, w.fyr) * sd.hy;
+    w.gyr[k - 1] = cell_gap(k, w.fyr) * sd.hy;
+    w.gzl[k - 1] = cell_gap(-k, w.fzr) * sd.hz;
+    w.gzr[k - 1] = cell_gap(k, w.fzr) * sd.hz;
+  }
+  // pruning radius: the edge distance with the force stencil's margins
+  const float cut2 = sd.edge_d2 * 1.001f + 1e-12f;
+  const int nrows = (2 * sd.edge_ry + 1) * (2 * sd.edge_rz + 1);
+  if (g == 0) el_n[o] = 0u;
+  wave_lds_sync();
+  if (active)
+    edge_walk<ER>(a, sd, w, cut2, g, G, nrows,
+              [&](int, const float4& pj, uint32_t jid) {
+                if (__float_as_uint(pj.w) != ti) return;        // :219
+                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (!(s2 < sd.edge_d2)) return;                  // :227
+                if (jid <= idi) return;                          // j > i, :215
+                const uint32_t kk = atomicAdd(&el_n[o], 1u);
+                if (kk < (uint32_t)kListEdges) {
+                  el_d[o][kk] = make_float2(s2, 0.0f);
+                  el_i[o][kk] = jid;
+                }
+              });
+  wave_lds_sync();
+  const uint32_t n = active ? el_n[o] : 0u;
+  const bool fits = n <= (uint32_t)kListEdges;
+  const int cap = a.edge_cap, maxc = sd.edge_maxc;
+  // phase 1: keep the `cap` smallest indices (the reference's first 2*maxC by index, :215)
+  if (fits)
+    for (uint32_t k = g; k < n; k += G) {
+      const uint32_t ik = el_i[o][k];
+      uint32_t below = 0;
+      for (uint32_t k2 = 0; k2 < n; ++k2) below += (el_i[o][k2] < ik) ? 1u : 0u;
+      el_d[o][k].y = (below < (uint32_t)cap) ? 1.0f : 0.0f;
+    }
+  wave_lds_sync();
+  // phase 2: rank the kept ones by (d^2, index), keep the first maxC (:235-246)
+  if (fits)
+    for (uint32_t k = g; k < n; k += G) {
+      const float2 dk = el_d[o][k];
+      if (dk.y == 0.0f) continue;
+      const uint32_t ik = el_i[o][k];
+      uint32_t rank = 0;
+      for (uint32_t k2 = 0; k2 < n; ++k2) {
+        const float2 d2 = el_d[o][k2];
+        const uint32_t i2 = el_i[o][k2];
+        rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
+      }
+      if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + idi] = ik;
+    }
+  if (!active || g != 0) return;
+  uint32_t e;
+  if (fits) {
+    e = (uint32_t)min(min((int)n, cap), maxc);
+  } else {
+    // more candidates than the LDS list holds: exact streaming selection by this lane
+    EdgeAcc ea;
+    ea.q = 0;
+    ea.maxid = 0;
+    ea.maxk = 0;
+    edge_walk<ER>(a, sd, w, cut2, 0, 1, nrows,
+              [&](int, const float4& pj, uint32_t jid) {
+                if (__float_as_uint(pj.w) != ti) return;
+                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (!(s2 < sd.edge_d2)) return;
+                if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
+              });
+    e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
+  }
+  if (e) {  // zero already: the edge tiles that read the counts clear them
+    a.ecount[idi] = e;
+    atomicAdd(&a.etcount[((idi / kEdgeTile) * kEdgeSub + (blockIdx.x & (kEdgeSub - 1))) *
+                         kCtrStride], e);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_edges_g(EdgeArgs a) {
   constexpr int G = kEdgeLanes;
   constexpr int kOwn = kWave / G;
@@ -1696,84 +1784,10 @@ __global__ void __launch_bounds__(256) k_edges_g(EdgeArgs a) {
     pi = a.pos[p];
     idi = a.id[p];
   }
-  const uint32_t ti = __float_as_uint(pi.w);
-  const int cx = bin_axis(pi.x, sd.ihx, sd.nx);
-  const int cy = bin_axis(pi.y, sd.ihy, sd.ny);
-  const int cz = bin_axis(pi.z, sd.ihz, sd.nz);
-  const float fxr = fminf(fmaxf(pi.x * sd.ihx - (float)cx, 0.0f), 1.0f);
-  const float fyr = fminf(fmaxf(pi.y * sd.ihy - (float)cy, 0.0f), 1.0f);
-  const float fzr = fminf(fmaxf(pi.z * sd.ihz - (float)cz, 0.0f), 1.0f);
-  // pruning radius: the edge distance with the force stencil's margins
-  const float cut2 = sd.edge_d2 * 1.001f + 1e-12f;
-  const int nrows = (2 * sd.edge_ry + 1) * (2 * sd.edge_rz + 1);
-  if (g == 0) el_n[wv][o] = 0u;
-  wave_lds_sync();
-  if (active)
-    edge_walk(a, sd, cx, cy, cz, fxr, fyr, fzr, cut2, g, G, nrows,
-              [&](int, const float4& pj, uint32_t jid) {
-                if (__float_as_uint(pj.w) != ti) return;        // :219
-                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (!(s2 < sd.edge_d2)) return;                  // :227
-                if (jid <= idi) return;                          // j > i, :215
-                const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
-                if (kk < (uint32_t)kListEdges) {
-                  el_d[wv][o][kk] = make_float2(s2, 0.0f);
-                  el_i[wv][o][kk] = jid;
-                }
-              });
-  wave_lds_sync();
-  const uint32_t n = active ? el_n[wv][o] : 0u;
-  const bool fits = n <= (uint32_t)kListEdges;
-  const int cap = a.edge_cap, maxc = sd.edge_maxc;
-  // phase 1: keep the `cap` smallest indices (the reference's first 2*maxC by index, :215)
-  if (fits)
-    for (uint32_t k = g; k < n; k += G) {
-      const uint32_t ik = el_i[wv][o][k];
-      uint32_t below = 0;
-      for (uint32_t k2 = 0; k2 < n; ++k2) below += (el_i[wv][o][k2] < ik) ? 1u : 0u;
-      el_d[wv][o][k].y = (below < (uint32_t)cap) ? 1.0f : 0.0f;
-    }
-  wave_lds_sync();
-  // phase 2: rank the kept ones by (d^2, index), keep the first maxC (:235-246)
-  if (fits)
-    for (uint32_t k = g; k < n; k += G) {
-      const float2 dk = el_d[wv][o][k];
-      if (dk.y == 0.0f) continue;
-      const uint32_t ik = el_i[wv][o][k];
-      uint32_t rank = 0;
-      for (uint32_t k2 = 0; k2 < n; ++k2) {
-        const float2 d2 = el_d[wv][o][k2];
-        const uint32_t i2 = el_i[wv][o][k2];
-        rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
-      }
-      if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + idi] = ik;
-    }
-  if (!active || g != 0) return;
-  uint32_t e;
-  if (fits) {
-    e = (uint32_t)min(min((int)n, cap), maxc);
-  } else {
-    // more candidates than the LDS list holds: exact streaming selection by this lane
-    EdgeAcc ea;
-    ea.q = 0;
-    ea.maxid = 0;
-    ea.maxk = 0;
-    edge_walk(a, sd, cx, cy, cz, fxr, fyr, fzr, cut2, 0, 1, nrows,
-              [&](int, const float4& pj, uint32_t jid) {
-                if (__float_as_uint(pj.w) != ti) return;
-                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (!(s2 < sd.edge_d2)) return;
-                if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
-              });
-    e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
-  }
-  if (e) {  // zero already: the edge tiles that read the counts clear them
-    a.ecount[idi] = e;
-    atomicAdd(&a.etcount[((idi / kEdgeTile) * kEdgeSub + (blockIdx.x & (kEdgeSub - 1))) *
-                         kCtrStride], e);
-  }
+  if (sd.edge_rx == 1 && sd.edge_ry == 1 && sd.edge_rz == 1)
+    edges_owner<1>(a, sd, wv, o, g, p, active, pi, idi, el_d[wv], el_i[wv], el_n[wv]);
+  else
+    edges_owner<0>(a, sd, wv, o, g, p, active, pi, idi, el_d[wv], el_i[wv], el_n[wv]);
 }
 
 // ------------------------------------------------------------------------------------
