@@ -785,28 +785,44 @@ struct Engine {
     return ta;
   }
 
+  // Times the next kernel launch into ev[2i] (start) and ev[2i+1] (end) when `ev` is given.
+  static void timed(hipEvent_t* ev, int i) {
+    if (ev) arm_kernel_timer(ev[2 * i], ev[2 * i + 1]);
+  }
+
   // Kernel sequence of one step: the force pass, then either the tile sort (which also
   // writes the step's canonical edge list) or the three-pass counting sort; `ev`
-  // (optional) receives an event after each kernel.  `write_acc` false leaves the
-  // accelerations unstored: they are output-only (ParticleSimulation.cu:146), so only the
-  // last step of a call stores them.
+  // (optional, two events per kernel) receives each kernel's start and end.  `write_acc`
+  // false leaves the accelerations unstored: they are output-only
+  // (ParticleSimulation.cu:146), so only the last step of a call stores them.
   int step(bool fused_edges, int cap, hipEvent_t* ev = nullptr, bool write_acc = true) {
     const int B = 1 - cur;
     ForceArgs fa = force_args(fused_edges, cap, write_acc);
-    HIPR(launch_force(fa, fused_edges, tun.block_size, tun.force_variant,
-                      std::max(1, std::min(2, tun.cell_divisor)), tun.lanes_per_particle, st));
-    if (ev) HIPR(hipEventRecord(ev[0], st));
+    timed(ev, 0);
+    const hipError_t e0 = launch_force(fa, fused_edges, tun.block_size, tun.force_variant,
+                                       std::max(1, std::min(2, tun.cell_divisor)),
+                                       tun.lanes_per_particle, st);
+    arm_kernel_timer(nullptr, nullptr);
+    HIPR(e0);
     if (tun.sort_variant == 1) {
-      HIPR(launch_tile_sort(tile_sort_args(fused_edges), st));
-      if (ev) HIPR(hipEventRecord(ev[1], st));
+      timed(ev, 1);
+      const hipError_t e1 = launch_tile_sort(tile_sort_args(fused_edges), st);
+      arm_kernel_timer(nullptr, nullptr);
+      HIPR(e1);
     } else {
-      HIPR(launch_scan(plan(fused_edges ? 1 : 0, b.start[B]), st));
-      if (ev) HIPR(hipEventRecord(ev[1], st));
+      timed(ev, 1);
+      const hipError_t e1 = launch_scan(plan(fused_edges ? 1 : 0, b.start[B]), st);
+      arm_kernel_timer(nullptr, nullptr);
+      HIPR(e1);
       SortArgs sa = step_sort_args(fused_edges);
-      HIPR(launch_fill(sa, st));
-      if (ev) HIPR(hipEventRecord(ev[2], st));
-      HIPR(launch_rank_scatter(sa, st));
-      if (ev) HIPR(hipEventRecord(ev[3], st));
+      timed(ev, 2);
+      const hipError_t e2 = launch_fill(sa, st);
+      arm_kernel_timer(nullptr, nullptr);
+      HIPR(e2);
+      timed(ev, 3);
+      const hipError_t e3 = launch_rank_scatter(sa, st);
+      arm_kernel_timer(nullptr, nullptr);
+      HIPR(e3);
     }
     cur = B;
     return CF_OK;
@@ -829,8 +845,10 @@ struct Engine {
     ea.P = P;
     ea.nwaves = nwaves;
     ea.edge_cap = cap;
-    HIPR(launch_edges(ea, st));
-    if (ev) HIPR(hipEventRecord(ev[0], st));
+    timed(ev, 0);
+    const hipError_t e0 = launch_edges(ea, st);
+    arm_kernel_timer(nullptr, nullptr);
+    HIPR(e0);
     TileSortArgs ta{};
     ta.sims = b.sims;
     ta.wave_sim = b.wave_sim;
@@ -846,8 +864,10 @@ struct Engine {
     ta.netiles = netiles;
     ta.P = P;
     ta.one_sim = sims.size() == 1 ? 1 : 0;
-    HIPR(launch_tile_sort(ta, st));
-    if (ev) HIPR(hipEventRecord(ev[1], st));
+    timed(ev, 1);
+    const hipError_t e1 = launch_tile_sort(ta, st);
+    arm_kernel_timer(nullptr, nullptr);
+    HIPR(e1);
     epar ^= 1;
     return CF_OK;
   }
@@ -1358,11 +1378,11 @@ struct Engine {
     const int K = ns + 2;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     int cnt[6] = {0, 0, 0, 0, 0, 0};
-    // Every step is queued before any is waited for, each with its own events, so the GPU
-    // runs the kernels back to back: an interval holds one kernel and the event marker
-    // after it, never a host launch.  The first step's intervals are left out when there
-    // are others (its first event may reach the GPU before the kernel behind it is queued).
-    const int per = K + 1;
+    // Every step is queued before any is waited for, so the GPU runs the kernels back to
+    // back; each kernel's interval comes from its own dispatch (start and end events of
+    // hipExtLaunchKernel), so it holds that kernel alone.  The first step's kernels are
+    // left out when there are others (they start on a cold state).
+    const int per = 2 * K;
     std::vector<hipEvent_t> ev((size_t)(nsteps + 1) * per, nullptr);
     int rc = CF_OK;
     for (auto& e : ev)
@@ -1371,28 +1391,21 @@ struct Engine {
     std::vector<int> nev(nsteps, 0);
     for (int k = 0; k < nsteps && rc == CF_OK; ++k) {
       hipEvent_t* e = &ev[(size_t)k * per];
-      if (hipEventRecord(e[0], st) != hipSuccess) {
-        rc = fail(CF_ERR_DEVICE, "hipEventRecord failed");
-        break;
-      }
-      rc = step(er.on && fuse && k > 0, cap, e + 1, k == nsteps - 1);
+      rc = step(er.on && fuse && k > 0, cap, e, k == nsteps - 1);
       nev[k] = ns;
       if (rc == CF_OK && er.on && !fuse) {
-        rc = edge_pass(cap, e + ns + 1);
+        rc = edge_pass(cap, e + 2 * ns);
         nev[k] = ns + 2;
       }
     }
     hipEvent_t* fe = &ev[(size_t)nsteps * per];  // the final standalone edge pass (fused case)
-    if (rc == CF_OK && er.on && fuse) {
-      if (hipEventRecord(fe[ns], st) != hipSuccess) rc = fail(CF_ERR_DEVICE, "hipEventRecord failed");
-      if (rc == CF_OK) rc = edge_pass(cap, fe + ns + 1);
-    }
+    if (rc == CF_OK && er.on && fuse) rc = edge_pass(cap, fe + 2 * ns);
     if (rc == CF_OK && hipStreamSynchronize(st) != hipSuccess)
       rc = fail(CF_ERR_DEVICE, "the profiled steps failed");
     auto add = [&](const hipEvent_t* e, int i) {
       float ms = 0;
-      if (hipEventElapsedTime(&ms, e[i], e[i + 1]) != hipSuccess)
-        return fail(CF_ERR_DEVICE, "hipEventElapsedTime failed");
+      if (hipEventElapsedTime(&ms, e[2 * i], e[2 * i + 1]) != hipSuccess)
+        return fail(CF_ERR_DEVICE, "hipEventElapsedTime failed (a kernel was not timed)");
       acc[i] += ms;
       cnt[i]++;
       return CF_OK;

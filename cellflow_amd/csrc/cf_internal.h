@@ -427,6 +427,12 @@ hipError_t launch_shadow_step(const AdjArgs& a, int n, hipStream_t st);
 hipError_t launch_shadow_drift(const AdjArgs& a, int n, hipStream_t st);
 int adj_pair_blocks(int n);  // blocks (columns of AdjArgs::partial) of one backward step
 
+// Per-kernel timing (cf_profile_steps): the next kernel launch of this thread through the
+// launchers below records its own start and end into `start` and `stop`
+// (hipExtLaunchKernel, timestamps of the dispatch itself: no marker between kernels), and
+// the pair is then cleared.  Never armed during graph capture.
+void arm_kernel_timer(hipEvent_t start, hipEvent_t stop);
+
 // host-side launchers (cf_kernels.hip)
 hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, int reach,
                         int lanes, hipStream_t st);

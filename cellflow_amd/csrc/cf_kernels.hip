@@ -25,9 +25,33 @@
 // oracle evaluates the same FP32 expressions, so counts, cells, order and edges match it
 // bit for bit.
 
+#include <hip/hip_ext.h>
+
 #include "cf_internal.h"
 
 namespace cf {
+
+namespace {
+thread_local hipEvent_t t_timer_start = nullptr, t_timer_stop = nullptr;
+}
+
+void arm_kernel_timer(hipEvent_t start, hipEvent_t stop) {
+  t_timer_start = start;
+  t_timer_stop = stop;
+}
+
+// hipLaunchKernelGGL, or, when arm_kernel_timer armed this thread's timer, the same launch
+// through hipExtLaunchKernelGGL with the armed events (then disarmed).
+#define CF_LAUNCH(kernel, grid, block, shm, st, ...)                                   \
+  do {                                                                                 \
+    if (t_timer_start) {                                                               \
+      hipEvent_t cf_s0_ = t_timer_start, cf_s1_ = t_timer_stop;                        \
+      t_timer_start = t_timer_stop = nullptr;                                          \
+      hipExtLaunchKernelGGL(kernel, grid, block, shm, st, cf_s0_, cf_s1_, 0u, __VA_ARGS__); \
+    } else {                                                                           \
+      hipLaunchKernelGGL(kernel, grid, block, shm, st, __VA_ARGS__);                   \
+    }                                                                                  \
+  } while (0)
 
 // ------------------------------------------------------------------------------------
 // small device helpers
@@ -2725,16 +2749,16 @@ static void launch_grp(const ForceArgs& a, bool edges, hipStream_t st) {
   const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
   const dim3 g((unsigned)((waves + 3) / 4)), b(256);
   if (a.hits) {  // the gradient rollout's lists (never with edges)
-    if (a.tile_sort) hipLaunchKernelGGL((k_force_grp<false, R, G, true, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_force_grp<false, R, G, false, true>), g, b, 0, st, a);
+    if (a.tile_sort) CF_LAUNCH((k_force_grp<false, R, G, true, true>), g, b, 0, st, a);
+    else CF_LAUNCH((k_force_grp<false, R, G, false, true>), g, b, 0, st, a);
     return;
   }
   if (a.tile_sort) {
-    if (edges) hipLaunchKernelGGL((k_force_grp<true, R, G, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_force_grp<false, R, G, true>), g, b, 0, st, a);
+    if (edges) CF_LAUNCH((k_force_grp<true, R, G, true>), g, b, 0, st, a);
+    else CF_LAUNCH((k_force_grp<false, R, G, true>), g, b, 0, st, a);
   } else {
-    if (edges) hipLaunchKernelGGL((k_force_grp<true, R, G, false>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_force_grp<false, R, G, false>), g, b, 0, st, a);
+    if (edges) CF_LAUNCH((k_force_grp<true, R, G, false>), g, b, 0, st, a);
+    else CF_LAUNCH((k_force_grp<false, R, G, false>), g, b, 0, st, a);
   }
 }
 
@@ -2743,11 +2767,11 @@ static void launch_list(const ForceArgs& a, bool edges, hipStream_t st) {
   const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
   const dim3 g((unsigned)((waves + 3) / 4)), b(256);
   if (a.tile_sort) {
-    if (edges) hipLaunchKernelGGL((k_force_list<true, R, G, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_force_list<false, R, G, true>), g, b, 0, st, a);
+    if (edges) CF_LAUNCH((k_force_list<true, R, G, true>), g, b, 0, st, a);
+    else CF_LAUNCH((k_force_list<false, R, G, true>), g, b, 0, st, a);
   } else {
-    if (edges) hipLaunchKernelGGL((k_force_list<true, R, G, false>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_force_list<false, R, G, false>), g, b, 0, st, a);
+    if (edges) CF_LAUNCH((k_force_list<true, R, G, false>), g, b, 0, st, a);
+    else CF_LAUNCH((k_force_list<false, R, G, false>), g, b, 0, st, a);
   }
 }
 
@@ -2756,11 +2780,11 @@ static void launch_cons(const ForceArgs& a, bool edges, hipStream_t st) {
   const long waves = (long)a.nwaves * G;  // one wave per 64/G particle slots
   const dim3 g((unsigned)((waves + 3) / 4)), b(256);
   if (a.tile_sort) {
-    if (edges) hipLaunchKernelGGL((k_force_list<true, R, G, true, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_force_list<false, R, G, true, true>), g, b, 0, st, a);
+    if (edges) CF_LAUNCH((k_force_list<true, R, G, true, true>), g, b, 0, st, a);
+    else CF_LAUNCH((k_force_list<false, R, G, true, true>), g, b, 0, st, a);
   } else {
-    if (edges) hipLaunchKernelGGL((k_force_list<true, R, G, false, true>), g, b, 0, st, a);
-    else hipLaunchKernelGGL((k_force_list<false, R, G, false, true>), g, b, 0, st, a);
+    if (edges) CF_LAUNCH((k_force_list<true, R, G, false, true>), g, b, 0, st, a);
+    else CF_LAUNCH((k_force_list<false, R, G, false, true>), g, b, 0, st, a);
   }
 }
 
@@ -2813,68 +2837,68 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
   if (variant == 2) {
     const dim3 g(a.nwaves), b(kWave);
     if (reach == 1) {
-      if (edges) hipLaunchKernelGGL((k_force_lds<true, 1>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_force_lds<false, 1>), g, b, 0, st, a);
+      if (edges) CF_LAUNCH((k_force_lds<true, 1>), g, b, 0, st, a);
+      else CF_LAUNCH((k_force_lds<false, 1>), g, b, 0, st, a);
     } else {
-      if (edges) hipLaunchKernelGGL((k_force_lds<true, 2>), g, b, 0, st, a);
-      else hipLaunchKernelGGL((k_force_lds<false, 2>), g, b, 0, st, a);
+      if (edges) CF_LAUNCH((k_force_lds<true, 2>), g, b, 0, st, a);
+      else CF_LAUNCH((k_force_lds<false, 2>), g, b, 0, st, a);
     }
     return hipGetLastError();
   }
   if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
   const int g = grid_for(a.nwaves, block);
   if (edges)
-    hipLaunchKernelGGL(k_force<true>, dim3(g), dim3(block), 0, st, a);
+    CF_LAUNCH(k_force<true>, dim3(g), dim3(block), 0, st, a);
   else
-    hipLaunchKernelGGL(k_force<false>, dim3(g), dim3(block), 0, st, a);
+    CF_LAUNCH(k_force<false>, dim3(g), dim3(block), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_edges(const EdgeArgs& a, hipStream_t st) {
   // kWave / kEdgeLanes owners per wave, four waves per block: a.nwaves blocks
-  hipLaunchKernelGGL(k_edges_g, dim3(a.nwaves), dim3(256), 0, st, a);
+  CF_LAUNCH(k_edges_g, dim3(a.nwaves), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_tile_sort(const TileSortArgs& a, hipStream_t st) {
   const int g = a.ntiles + a.netiles;
-  if (g > 0) hipLaunchKernelGGL(k_tile_sort, dim3(g), dim3(kSortThreads), 0, st, a);
+  if (g > 0) CF_LAUNCH(k_tile_sort, dim3(g), dim3(kSortThreads), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_scan(const ScanPlan& p, hipStream_t st) {
-  hipLaunchKernelGGL(k_scan, dim3(p.tiles_total), dim3(kScanThreads), 0, st, p);
+  CF_LAUNCH(k_scan, dim3(p.tiles_total), dim3(kScanThreads), 0, st, p);
   return hipGetLastError();
 }
 
 hipError_t launch_fill(const SortArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  CF_LAUNCH(k_fill, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_rank_scatter(const SortArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_rank_scatter, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  CF_LAUNCH(k_rank_scatter, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_tri_count(const TriArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_tri_count, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  CF_LAUNCH(k_tri_count, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_tri_write(const TriArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_tri_write, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  CF_LAUNCH(k_tri_write, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_tri_vertices(const TriVertArgs& a, hipStream_t st) {
   const long g = (a.ntris + 255) / 256;
-  if (g > 0) hipLaunchKernelGGL(k_tri_vertices, dim3((unsigned)g), dim3(256), 0, st, a);
+  if (g > 0) CF_LAUNCH(k_tri_vertices, dim3((unsigned)g), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_bin(const BinArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_bin, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
+  CF_LAUNCH(k_bin, dim3(grid_for(a.nwaves, 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
@@ -2882,12 +2906,12 @@ hipError_t launch_io(int which, const IoArgs& a, int n, hipStream_t st) {
   const int g = (n + 255) / 256;
   if (g == 0) return hipSuccess;
   switch (which) {
-    case 0: hipLaunchKernelGGL(k_load_aos, dim3(g), dim3(256), 0, st, a); break;
-    case 1: hipLaunchKernelGGL(k_unload_aos, dim3(g), dim3(256), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_get_counts, dim3(g), dim3(256), 0, st, a); break;
-    case 3: hipLaunchKernelGGL(k_set_counts, dim3(g), dim3(256), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_sort_order, dim3(g), dim3(256), 0, st, a); break;
-    case 5: hipLaunchKernelGGL(k_move, dim3(g), dim3(256), 0, st, a); break;
+    case 0: CF_LAUNCH(k_load_aos, dim3(g), dim3(256), 0, st, a); break;
+    case 1: CF_LAUNCH(k_unload_aos, dim3(g), dim3(256), 0, st, a); break;
+    case 2: CF_LAUNCH(k_get_counts, dim3(g), dim3(256), 0, st, a); break;
+    case 3: CF_LAUNCH(k_set_counts, dim3(g), dim3(256), 0, st, a); break;
+    case 4: CF_LAUNCH(k_sort_order, dim3(g), dim3(256), 0, st, a); break;
+    case 5: CF_LAUNCH(k_move, dim3(g), dim3(256), 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -2896,7 +2920,7 @@ hipError_t launch_io(int which, const IoArgs& a, int n, hipStream_t st) {
 hipError_t launch_init(const InitArgs& a, int n, hipStream_t st) {
   const int g = (n + 255) / 256;
   if (g == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_init, dim3(g), dim3(256), 0, st, a);
+  CF_LAUNCH(k_init, dim3(g), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
@@ -2904,14 +2928,14 @@ hipError_t launch_slot_of(const uint32_t* id, uint32_t* slot_of, const SimDesc* 
                           hipStream_t st) {
   const int g = (n + 255) / 256;
   if (g == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_slot_of, dim3(g), dim3(256), 0, st, id, slot_of, sims, sim);
+  CF_LAUNCH(k_slot_of, dim3(g), dim3(256), 0, st, id, slot_of, sims, sim);
   return hipGetLastError();
 }
 
 hipError_t launch_line_vertices(const LineArgs& a, hipStream_t st) {
   const long g = (a.nedges + 255) / 256;
   if (g == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_line_vertices, dim3((unsigned)g), dim3(256), 0, st, a);
+  CF_LAUNCH(k_line_vertices, dim3((unsigned)g), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
