@@ -1297,8 +1297,11 @@ __device__ __forceinline__ void integrate_commit(const ForceArgs& a, const SimDe
 // active: each wave leaves its particles' sums in LDS, and after a block barrier wave 0
 // reloads their (cache-resident) state and finishes them.  Same arithmetic, same results.
 template <bool EDGES, int R, int G, bool TILE, bool CONS = false>
-// 27-cell stencil at G = 8 fits 7 waves per SIMD (72 VGPRs, under 22.8 KB of LDS per block)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 1 && G == 8 ? 7 : 1, 8)))
+// With the consolidated tail, the 27-cell stencil at G = 8 fits 7 waves per SIMD (72 VGPRs,
+// under 22.8 KB of LDS per block).  Without it the velocity stays live to the end and the
+// kernel needs a few more registers: held to 72 it spills, and a spilling kernel's scratch
+// allocation cuts its occupancy far below 7 waves (a 6x slower launch), so no hint there.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 1 && G == 8 && CONS ? 7 : 1, 8)))
 k_force_list(ForceArgs a) {
   constexpr int kRows = (2 * R + 1) * (2 * R + 1);
   constexpr int kOwn = kWave / G;
