@@ -172,16 +172,19 @@ def test_adjoint_dense_state_pairs_beyond_the_lds_list():
 
 
 def test_checkpoint_interval_and_repeat_bit_identical():
+    """The checkpoint interval changes nothing, repeats are identical.  Intervals 1 and 2
+    reuse the per-step pair lists soonest (the shadow steps run on their own stream while
+    the next FP32 step lists into a buffer), and 7 leaves a short last segment."""
     cfg = small_cfg(n=2000)
     st, ph = setup(cfg)
     tgt = np.linspace(0.5, 3.0, 16)
     prm = SimulationParams.from_physics(ph)
     outs = []
-    for ckpt in (5, 30, 5):
+    for ckpt in (5, 30, 1, 2, 7, 5):
         sim = make_sim(st, ph)
         loss, h, gt, gr = sim.rolloutGrad(prm, 30, ckpt, 16, 0.1, tgt)
         outs.append((loss, gt.tobytes(), gr))
-    assert outs[0] == outs[1] == outs[2]
+    assert all(o == outs[0] for o in outs), [o[0] for o in outs]
 
 
 def test_autograd_function_wraps_adjoint():

@@ -6,7 +6,8 @@
   teacher-forced step 1,001; the graphs of the states after steps 999 and 1,000 (the one
   built inside step 1,000's force pass and sort, and the standalone one) against the
   oracle's; binning and order canonical.  Run with the tuning the autotuner picks for this
-  workload (list kernel, 8 lanes per particle, tile sort) and with the default kernel.
+  workload (list kernel with the consolidated tail, 8 lanes per particle, tile sort) and
+  with the default kernel.
 * 1M clustered (200 Gaussian blobs, edges): all 500 steps, then the same checks.
 * 256-simulation ensemble (2k-50k particles each): every member's first step against the
   oracle, then 10 more steps with members equal to their plain runs bit for bit.
@@ -25,8 +26,9 @@ from helpers import check_step, make_sim
 
 pytestmark = pytest.mark.gpu
 
-# The autotuner's pick for the 100k workload with edges, and the engine's default tuning.
-LIST_TILE = dict(cell_divisor=1, force_variant=3, lanes_per_particle=8, sort_variant=1)
+# The autotuner's pick for the 100k workload with edges (the list kernel with its
+# integration tail consolidated per block), and the engine's default tuning.
+LIST_TILE = dict(cell_divisor=1, force_variant=4, lanes_per_particle=8, sort_variant=1)
 DEFAULT = dict(cell_divisor=1, force_variant=0, lanes_per_particle=16, sort_variant=1)
 
 
