@@ -796,6 +796,9 @@ struct Engine {
   // false leaves the accelerations unstored: they are output-only
   // (ParticleSimulation.cu:146), so only the last step of a call stores them.
   int step(bool fused_edges, int cap, hipEvent_t* ev = nullptr, bool write_acc = true) {
+    if ((tun.force_variant == 3 || tun.force_variant == 4) && P > (1L << kCodeShift))
+      return fail(CF_ERR_ARG, "force variants 3 and 4 take at most 2^%d particle slots",
+                  kCodeShift);
     const int B = 1 - cur;
     ForceArgs fa = force_args(fused_edges, cap, write_acc);
     timed(ev, 0);
@@ -1982,6 +1985,7 @@ struct Engine {
                                                               orig.steps_per_graph, 1});
       cands.push_back(cf_tuning{d, 2, orig.lanes_per_particle, orig.block_size,
                                 orig.steps_per_graph, 1});
+      if (P > (1L << kCodeShift)) continue;  // the list kernels' slot limit
       for (int g : {16, 8, 4, 32})
         if (g >= 4 * d)
           cands.push_back(cf_tuning{d, 3, g, orig.block_size, orig.steps_per_graph, 1});

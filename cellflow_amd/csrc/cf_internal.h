@@ -24,6 +24,9 @@ constexpr int kTileCells = 2048;       // most cells in one tile (tr * nx)
 constexpr int kTileCap = 2048;         // members a tile sorts in LDS at once
 constexpr int kTileTarget = 384;       // expected members of a tile (chooses tr)
 constexpr int kEdgeTile = 1024;        // owner ids per edge tile
+// The list force kernels (variants 3 and 4) keep each candidate as slot | image-shift code
+// << kCodeShift in LDS, so they take at most 2^kCodeShift slots.
+constexpr int kCodeShift = 26;
 constexpr uint32_t kFarBit = 0x80000000u;
 // Tile and edge-tile counters sit one per 128-byte line: a line takes every atomic of the
 // tiles it holds, one after another.  An edge tile's count is further split over kEdgeSub

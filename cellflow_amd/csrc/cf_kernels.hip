@@ -1307,8 +1307,8 @@ k_force_list(ForceArgs a) {
   constexpr int kOwn = kWave / G;
   constexpr int kRpl = (kRows + G - 1) / G;  // stencil rows per lane
   __shared__ PairConst tab[4][kMaxTypes * kMaxTypes];
+  // list entries: slot | image-shift code << kCodeShift (slots stay below 2^kCodeShift)
   __shared__ uint32_t lslot[4][kOwn][kList];
-  __shared__ uint8_t lcode[4][kOwn][kList];
   __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_n[4][kOwn];
@@ -1383,17 +1383,15 @@ k_force_list(ForceArgs a) {
 #pragma unroll
     for (int k = 0; k < 2 * kRpl; ++k) {
       const uint32_t lo = max(f0, b0), hi = min(f0 + rl[k], b1);
-      for (uint32_t f = lo; f < hi; ++f) {
-        lslot[wv][o][f - b0] = rs[k] + (f - f0);
-        lcode[wv][o][f - b0] = (uint8_t)rk[k];
-      }
+      const uint32_t e0 = (rs[k] | (rk[k] << kCodeShift)) - f0;  // entry f: e0 + f (no carry)
+      for (uint32_t f = lo; f < hi; ++f) lslot[wv][o][f - b0] = e0 + f;
       f0 += rl[k];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const uint32_t nb = min(total - b0, (uint32_t)kList);
     for (uint32_t k = (uint32_t)g; k < nb; k += (uint32_t)G) {
-      const uint32_t q = lslot[wv][o][k];
-      const uint32_t code = lcode[wv][o][k];
+      const uint32_t e = lslot[wv][o][k];
+      const uint32_t q = e & ((1u << kCodeShift) - 1u), code = e >> kCodeShift;
       const float4 pj = a.pos[q];
       // the index travels with the position: an edge candidate then waits for no load
       const uint32_t jq = (EDGES && do_edges) ? a.id[q] : 0u;
@@ -2810,6 +2808,7 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
     }
     return hipGetLastError();
   }
+  if ((variant == 3 || variant == 4) && a.P > (1L << kCodeShift)) return hipErrorInvalidValue;
   if (variant == 4) {  // variant 3 with the block-consolidated tail
     if (reach == 1) {
       if (lanes == 8) launch_cons<1, 8>(a, edges, st);
