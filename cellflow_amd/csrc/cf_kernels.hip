@@ -1383,8 +1383,18 @@ k_force_list(ForceArgs a) {
 #pragma unroll
     for (int k = 0; k < 2 * kRpl; ++k) {
       const uint32_t lo = max(f0, b0), hi = min(f0 + rl[k], b1);
-      const uint32_t e0 = (rs[k] | (rk[k] << kCodeShift)) - f0;  // entry f: e0 + f (no carry)
-      for (uint32_t f = lo; f < hi; ++f) lslot[wv][o][f - b0] = e0 + f;
+      // entries lo..hi-1 of run k: (slot | code << kCodeShift) + f - f0 (no carry), two per
+      // store instruction
+      if (lo < hi) {
+        uint32_t* d = &lslot[wv][o][lo - b0];
+        uint32_t v = (rs[k] | (rk[k] << kCodeShift)) + (lo - f0);
+        uint32_t m = hi - lo;
+        for (; m >= 2u; m -= 2u, d += 2, v += 2u) {
+          d[0] = v;
+          d[1] = v + 1u;
+        }
+        if (m) d[0] = v;
+      }
       f0 += rl[k];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
