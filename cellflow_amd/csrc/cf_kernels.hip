@@ -27,6 +27,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "cf_internal.h"
 
 namespace cf {
@@ -1399,56 +1401,63 @@ k_force_list(ForceArgs a) {
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const uint32_t nb = min(total - b0, (uint32_t)kList);
-    for (uint32_t k = (uint32_t)g; k < nb; k += (uint32_t)G) {
-      const uint32_t e = lslot[wv][o][k];
-      const uint32_t q = e & ((1u << kCodeShift) - 1u), code = e >> kCodeShift;
-      const float4 pj = a.pos[q];
-      // the index travels with the position: an edge candidate then waits for no load
-      const uint32_t jq = (EDGES && do_edges) ? a.id[q] : 0u;
-      float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-      bool raw;
-      if (!c.gen) {
-        raw = code == 0u;
-        if (!raw) {  // image-shifted run
-          dx += shift_of(code & 3u, sd.Lx);
-          dy += shift_of((code >> 2) & 3u, sd.Ly);
-          dz += shift_of((code >> 4) & 3u, sd.Lz);
+    // one copy of the loop per grid kind (the choice is the same for the whole wave), so
+    // the common kind carries no per-pair minimum-image code
+    auto pairs = [&](auto gen_kind) {
+      constexpr bool GEN = decltype(gen_kind)::value;
+      for (uint32_t k = (uint32_t)g; k < nb; k += (uint32_t)G) {
+        const uint32_t e = lslot[wv][o][k];
+        const uint32_t q = e & ((1u << kCodeShift) - 1u), code = e >> kCodeShift;
+        const float4 pj = a.pos[q];
+        // the index travels with the position: an edge candidate then waits for no load
+        const uint32_t jq = (EDGES && do_edges) ? a.id[q] : 0u;
+        float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        bool raw;
+        if constexpr (!GEN) {
+          raw = code == 0u;
+          if (!raw) {  // image-shifted run
+            dx += shift_of(code & 3u, sd.Lx);
+            dy += shift_of((code >> 2) & 3u, sd.Ly);
+            dz += shift_of((code >> 4) & 3u, sd.Lz);
+          }
+        } else {  // per-pair minimum image
+          const float ux = dx, uy = dy, uz = dz;
+          dx = min_image(ux, sd.Lx);
+          dy = min_image(uy, sd.Ly);
+          dz = min_image(uz, sd.Lz);
+          raw = (dx == ux) && (dy == uy) && (dz == uz);
         }
-      } else {  // per-pair minimum image
-        const float ux = dx, uy = dy, uz = dz;
-        dx = min_image(ux, sd.Lx);
-        dy = min_image(uy, sd.Ly);
-        dz = min_image(uz, sd.Lz);
-        raw = (dx == ux) && (dy == uy) && (dz == uz);
-      }
-      const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      const uint32_t tj = __float_as_uint(pj.w);
-      const PairConst pc = row[tj];
-      const float sp = s2 + 1e-4f;
-      if (sp < pc.thr) {
-        cnt++;
-        const float inv = __builtin_amdgcn_rsqf(sp);
-        const float rr = (sp * inv) * pc.inv_eff;
-        const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
-        const float w = (sd.rep * ex - sd.att * rr) * pc.fv * inv;
-        fx += dx * w;
-        fy += dy * w;
-        fz += dz * w;
-      }
-      if (EDGES) {
-        // edges are non-periodic (ParticleSimulation.cu:222-227): unshifted pairs only
-        if (do_edges && raw && tj == ti && s2 < sd.edge_d2) {
-          const uint32_t jid = jq;
-          if (jid > idi) {
-            const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
-            if (kk < (uint32_t)kListEdges) {
-              el_d[wv][o][kk] = make_float2(s2, 0.0f);
-              el_i[wv][o][kk] = jid;
+        const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const uint32_t tj = __float_as_uint(pj.w);
+        const PairConst pc = row[tj];
+        const float sp = s2 + 1e-4f;
+        if (sp < pc.thr) {
+          cnt++;
+          const float inv = __builtin_amdgcn_rsqf(sp);
+          const float rr = (sp * inv) * pc.inv_eff;
+          const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
+          const float w = (sd.rep * ex - sd.att * rr) * pc.fv * inv;
+          fx += dx * w;
+          fy += dy * w;
+          fz += dz * w;
+        }
+        if (EDGES) {
+          // edges are non-periodic (ParticleSimulation.cu:222-227): unshifted pairs only
+          if (do_edges && raw && tj == ti && s2 < sd.edge_d2) {
+            const uint32_t jid = jq;
+            if (jid > idi) {
+              const uint32_t kk = atomicAdd(&el_n[wv][o], 1u);
+              if (kk < (uint32_t)kListEdges) {
+                el_d[wv][o][kk] = make_float2(s2, 0.0f);
+                el_i[wv][o][kk] = jid;
+              }
             }
           }
         }
       }
-    }
+    };
+    if (c.gen) pairs(std::true_type{});
+    else pairs(std::false_type{});
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // list reused by the next chunk
   }
   fx = dpp_group_sum_f<G>(fx);
