@@ -25,15 +25,27 @@ OCCUPANCY = {"_ZN2cf12k_force_listILb1ELi1ELi8ELb1ELb1EEEvNS_9ForceArgsE": 72,
              "_ZN2cf12k_force_listILb0ELi1ELi8ELb1ELb1EEEvNS_9ForceArgsE": 72}
 
 
+MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
+
+
 def kernel_metadata(tmp):
-    fat, co = os.path.join(tmp, "fat.bin"), os.path.join(tmp, "co.o")
+    """Kernel name -> resource fields, over every offload bundle of the library (one per
+    source file with kernels)."""
+    fat = os.path.join(tmp, "fat.bin")
     subprocess.run([os.path.join(LLVM, "llvm-objcopy"), f"--dump-section=.hip_fatbin={fat}",
                     LIB, os.path.join(tmp, "lib.so")], check=True)
-    subprocess.run([os.path.join(LLVM, "clang-offload-bundler"), "--unbundle", "--type=o",
-                    "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--input={fat}",
-                    f"--output={co}"], check=True)
-    notes = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "--notes", co], check=True,
-                           capture_output=True, text=True).stdout
+    data = open(fat, "rb").read()
+    starts = [i for i in range(len(data)) if data.startswith(MAGIC, i)]
+    notes = ""
+    for n, (b0, b1) in enumerate(zip(starts, starts[1:] + [len(data)])):
+        part, co = os.path.join(tmp, f"bundle{n}.bin"), os.path.join(tmp, f"co{n}.o")
+        with open(part, "wb") as f:
+            f.write(data[b0:b1])
+        subprocess.run([os.path.join(LLVM, "clang-offload-bundler"), "--unbundle", "--type=o",
+                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--input={part}",
+                        f"--output={co}"], check=True)
+        notes += subprocess.run([os.path.join(LLVM, "llvm-readelf"), "--notes", co],
+                                check=True, capture_output=True, text=True).stdout
     out = {}
     for block in re.split(r"\n\s+- \.", notes):
         m = re.search(r"\.name:\s+(\S+)", block)
@@ -59,7 +71,9 @@ def meta(tmp_path_factory):
 
 
 def test_no_kernel_spills_or_uses_scratch(meta):
-    assert len(meta) > 50, "expected the library's kernels in the code object"
+    assert len(meta) > 50, "expected the library's kernels in the code objects"
+    for k in ("_ZN2cf13k_shadow_stepENS_7AdjArgsE", "_ZN2cf9k_edges_gENS_8EdgeArgsE"):
+        assert k in meta, k  # kernels of each source file are covered
     bad = {k: v for k, v in meta.items()
            if v["vgpr_spill_count"] or v["private_segment_fixed_size"]}
     assert not bad, bad
