@@ -46,6 +46,13 @@ __device__ __forceinline__ double bspline3_d(double u) {  // dB3/du
   return (-2.0 * a + 1.5 * a * a) * s;
 }
 
+// fmod(y, L) for L > 0, exactly: for y in [0, 2L) it is y or y - L (the subtraction is exact
+// there), else fmod itself.
+__device__ __forceinline__ double wrap_d(double y, double L) {
+  if (y >= 0.0 && y < 2.0 * L) return y >= L ? y - L : y;
+  return fmod(y, L);
+}
+
 __device__ __forceinline__ double min_image_d(double d, double L) {
   if (d > L * 0.5) d -= L;
   if (d < -L * 0.5) d += L;
@@ -360,10 +367,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_shadow_step(AdjArgs a) {
   double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
   const double R = a.c.radius, rho = a.c.ratio;
   const double effi = active ? R * (1.0 + a.c.rbt[c.ti] * rho) : 0.0;
-  // the FP64 force term of one in-range pair (the oracle's op order)
-  auto pair = [&](uint32_t q) {
-    const int tj = (int)__float_as_uint(a.pos[q].w);
-    const double4 xj = a.x64[a.id[q]];
+  // the FP64 force term of one in-range pair (the oracle's op order), partner type tj at
+  // shadow position xj
+  auto pair = [&](int tj, const double4& xj) {
     const double dx = min_image_d(xj.x - xi.x, sd.Lx);
     const double dy = min_image_d(xj.y - xi.y, sd.Ly);
     const double dz = min_image_d(xj.z - xi.z, sd.Lz);
@@ -375,19 +381,36 @@ __global__ void __launch_bounds__(kAdjBlock) k_shadow_step(AdjArgs a) {
     fy += dy / dist * net;
     fz += dz / dist * net;
   };
+  int prev = 0;
+  double4 vi = make_double4(0.0, 0.0, 0.0, 0.0);
   if (active) {
     idi = a.id[c.p];
     xi = a.x64[idi];
+    if (g == 0) {  // the integration's inputs, loaded before the pairs are summed
+      prev = __float_as_int(a.vel[c.p].w);
+      vi = a.v64[idi];
+    }
     const uint32_t nh = a.hits ? a.nhits[k] : 0u;
     if (a.hits && nh <= (uint32_t)kAdjHits) {
       if (g == 0) cnt = (int)nh;
+      // the lane's entries two at a time, each stage's loads issued together (list entry,
+      // then type and index, then shadow position); pairs summed in entry order
+      const uint32_t* lst = a.hits + (long)k * kAdjHits;
 #pragma nounroll
-      for (uint32_t e = g; e < nh; e += G) pair(a.hits[(long)k * kAdjHits + e]);
+      for (uint32_t e = g; e < nh; e += 2 * G) {
+        const bool two = e + G < nh;
+        const uint32_t q0 = lst[e], q1 = two ? lst[e + G] : q0;
+        const int t0 = (int)__float_as_uint(a.pos[q0].w), t1 = (int)__float_as_uint(a.pos[q1].w);
+        const uint32_t j0 = a.id[q0], j1 = a.id[q1];
+        const double4 x0 = a.x64[j0], x1 = a.x64[j1];
+        pair(t0, x0);
+        if (two) pair(t1, x1);
+      }
     } else {
       for_cells_split(sd, a.start, c.cx, c.cy, c.cz, sd.reach, g, G, [&](uint32_t q) {
         if (!adj_in_range(a, sd, c, q)) return;
         ++cnt;
-        pair(q);
+        pair((int)__float_as_uint(a.pos[q].w), a.x64[a.id[q]]);
       });
     }
   }
@@ -399,16 +422,14 @@ __global__ void __launch_bounds__(kAdjBlock) k_shadow_step(AdjArgs a) {
     cnt += __shfl_xor(cnt, m, 64);
   }
   if (!active || g != 0) return;
-  const int prev = __float_as_int(a.vel[c.p].w);
   const double m = a.c.fm * shadow_fac(a.c, cnt, prev) * a.c.dt;
-  const double4 vi = a.v64[idi];
   const double vx = vi.x * a.c.friction + fx * m;
   const double vy = vi.y * a.c.friction + fy * m;
   const double vz = vi.z * a.c.friction + fz * m;
   a.v64_next[idi] = make_double4(vx, vy, vz, 0.0);
-  a.x64_next[idi] = make_double4(fmod(xi.x + vx * a.c.dt + (double)sd.Lx, (double)sd.Lx),
-                                 fmod(xi.y + vy * a.c.dt + (double)sd.Ly, (double)sd.Ly),
-                                 fmod(xi.z + vz * a.c.dt + (double)sd.Lz, (double)sd.Lz), 0.0);
+  a.x64_next[idi] = make_double4(wrap_d(xi.x + vx * a.c.dt + (double)sd.Lx, (double)sd.Lx),
+                                 wrap_d(xi.y + vy * a.c.dt + (double)sd.Ly, (double)sd.Ly),
+                                 wrap_d(xi.z + vz * a.c.dt + (double)sd.Lz, (double)sd.Lz), 0.0);
 }
 
 // Sets flag bit 3 when a shadow position lies more than slack/2 (periodic, per axis) from
@@ -496,12 +517,10 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
     nh = a.nhits[k];
     xi = a.x64[idi];
   }
-  // the FP64 terms of one in-range pair
-  auto pair = [&](uint32_t q) {
+  // the FP64 terms of one in-range pair: partner type tj, shadow position xj, factor phi_j
+  auto pair = [&](int tj, const double4& xj, const double4& phi_j) {
     const double R = a.c.radius, rho = a.c.ratio, rep = a.c.rep, att = a.c.att, kk = a.c.k;
     const double eps = (double)1e-4f;
-    const int tj = (int)__float_as_uint(a.pos[q].w);
-    const double4 xj = a.x64[a.id[q]];  // the shadow positions
     const double dx = min_image_d(xj.x - xi.x, sd.Lx);
     const double dy = min_image_d(xj.y - xi.y, sd.Ly);
     const double dz = min_image_d(xj.z - xi.z, sd.Lz);
@@ -514,7 +533,6 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
     const double gp = -2.0 * kk * r * rep * ex - att;
     const double h = gr / dist;
     const double hp = gp / (eff * dist) - gr / (dist * dist);
-    const double4 phi_j = a.phi[q];
     const double fij = a.c.table[ti * T + tj], fji = a.c.table[tj * T + ti];
     const double dpi = (dx * phi_i.x + dy * phi_i.y + dz * phi_i.z) / dist;
     const double dpj = (dx * phi_j.x + dy * phi_j.y + dz * phi_j.z) / dist;
@@ -534,9 +552,24 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
       if (adj_in_range(a, sd, c, q)) ++nmine;
     });
   const uint32_t first = group_excl_scan<G>(nmine, g);
+  auto pair_q = [&](uint32_t q) {
+    pair((int)__float_as_uint(a.pos[q].w), a.x64[a.id[q]], a.phi[q]);
+  };
   if (listed) {
+    // the lane's entries two at a time, each stage's loads issued together (list entry, then
+    // type, index and factor, then shadow position); pairs summed in entry order
+    const uint32_t* lst = a.hits + (long)k * kAdjHits;
 #pragma nounroll
-    for (uint32_t e = g; e < nh; e += G) pair(a.hits[(long)k * kAdjHits + e]);
+    for (uint32_t e = g; e < nh; e += 2 * G) {
+      const bool two = e + G < nh;
+      const uint32_t q0 = lst[e], q1 = two ? lst[e + G] : q0;
+      const int t0 = (int)__float_as_uint(a.pos[q0].w), t1 = (int)__float_as_uint(a.pos[q1].w);
+      const uint32_t j0 = a.id[q0], j1 = a.id[q1];
+      const double4 f0 = a.phi[q0], f1 = a.phi[q1];
+      const double4 x0 = a.x64[j0], x1 = a.x64[j1];
+      pair(t0, x0, f0);
+      if (two) pair(t1, x1, f1);
+    }
   } else {
     // a group's lanes share one wave: LDS writes and reads between them are ordered by
     // wavefront fences, so the windows need no block barrier
@@ -550,7 +583,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_pair_g(AdjArgs a) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       const uint32_t nb = min(nh - w0, (uint32_t)kAdjHits);
 #pragma nounroll
-      for (uint32_t e = g; e < nb; e += G) pair(hsorted[o][e]);
+      for (uint32_t e = g; e < nb; e += G) pair_q(hsorted[o][e]);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // window reused by the next
     }
   }
