@@ -2124,6 +2124,22 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   }
   __syncthreads();
 
+  // A tile of at most kMoveBatch * kSortThreads appended members loads their states now,
+  // entry m = u * kSortThreads + tid by thread tid: the loads are in flight during the
+  // scan and the grouping below, and the same thread then moves them.
+  const uint32_t napp = placed ? s_app : 0u;
+  const bool preload = placed && napp <= (uint32_t)(kMoveBatch * kSortThreads);
+  float4 psp[kMoveBatch], psv[kMoveBatch];
+#pragma unroll
+  for (int u = 0; u < kMoveBatch; ++u) {
+    const uint32_t m = (uint32_t)(u * kSortThreads + tid);
+    psp[u] = psv[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (preload && m < napp) {
+      psp[u] = a.src_pos[aq[m]];
+      psv[u] = a.src_vel[aq[m]];
+    }
+  }
+
   // first slot of the tile, then its cells' starts
   const int per = (nc + kSortThreads - 1) / kSortThreads;
   const int i0 = min(tid * per, nc), i1 = min(i0 + per, nc);
@@ -2162,6 +2178,24 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
       mid[at] = aid[m];
       mlc[at] = (uint16_t)lc;
     }
+  }
+  if (preload) {  // rank each preloaded member by index inside its cell and move it
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kMoveBatch; ++u) {
+      const uint32_t m = (uint32_t)(u * kSortThreads + tid);
+      if (m < M) {
+        const uint32_t lc = alc[m], me = aid[m];
+        const uint32_t s0 = cofs[lc], sn = cnt_s[lc];
+        uint32_t rank = 0;
+        for (uint32_t e = 0; e < sn; ++e) rank += (mid[s0 + e] < me) ? 1u : 0u;
+        const uint32_t dst = base + s0 + rank;
+        a.dst_pos[dst] = psp[u];
+        a.dst_vel[dst] = psv[u];
+        a.dst_id[dst] = me;
+      }
+    }
+    return;
   }
 
   // cells [ia, ib) at a time, as many as kTileCap members allow (all of them, normally)
