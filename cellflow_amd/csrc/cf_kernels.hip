@@ -1580,7 +1580,7 @@ k_force_list(ForceArgs a) {
 // (estage[rank][index]), the non-zero per-owner counts go to `ecount` and to the edge-tile
 // counters, and k_tile_sort's edge tiles write the canonical list.
 
-constexpr int kEdgeLanes = 4;
+constexpr int kEdgeLanes = 8;
 
 // Row r of an owner's non-periodic edge stencil: false when it lies outside the grid or
 // farther than the pruning radius; else its cell range [cell0, cell1] (one row, x-adjacent
@@ -2910,8 +2910,8 @@ hipError_t launch_force(const ForceArgs& a, bool edges, int block, int variant, 
 }
 
 hipError_t launch_edges(const EdgeArgs& a, hipStream_t st) {
-  // kWave / kEdgeLanes owners per wave, four waves per block: a.nwaves blocks
-  CF_LAUNCH(k_edges_g, dim3(a.nwaves), dim3(256), 0, st, a);
+  // kWave / kEdgeLanes owners per wave, four waves per block
+  CF_LAUNCH(k_edges_g, dim3((a.nwaves * kEdgeLanes + 3) / 4), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
