@@ -697,6 +697,7 @@ struct Engine {
     fa.ecand_d2 = b.ecand_d2;
     fa.ecand_id = b.ecand_id;
     fa.estage = b.estage;
+    fa.estride = edge_maxc_alloc;
     fa.ecount = b.fcount;
     fa.P = P;
     fa.nwaves = nwaves;
@@ -732,6 +733,7 @@ struct Engine {
     sa.wave_sim = b.wave_sim;
     if (edges) {
       sa.estage = b.estage;
+      sa.estride = edge_maxc_alloc;
       sa.eoff = b.eoff;
       sa.edges = b.sedges;
       sa.etotal = b.stotal;
@@ -774,6 +776,7 @@ struct Engine {
       ta.etcount = etcnt[A];
       ta.etcount_next = etcnt[B];
       ta.estage = b.estage;
+      ta.estride = edge_maxc_alloc;
       ta.edges = b.sedges;
       ta.etotal = b.stotal;
       ta.edge_room = (long)edge_maxc_alloc * P;
@@ -843,6 +846,7 @@ struct Engine {
     ea.ecand_d2 = b.ecand_d2;
     ea.ecand_id = b.ecand_id;
     ea.estage = b.estage;
+    ea.estride = edge_maxc_alloc;
     ea.ecount = b.fcount;
     ea.etcount = etcs[epar];
     ea.P = P;
@@ -861,6 +865,7 @@ struct Engine {
     ta.etcount = etcs[epar];
     ta.etcount_next = etcs[1 - epar];
     ta.estage = b.estage;
+    ta.estride = edge_maxc_alloc;
     ta.edges = b.edges;
     ta.etotal = b.gtotal;
     ta.edge_room = (long)edge_maxc_alloc * P;

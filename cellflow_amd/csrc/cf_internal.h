@@ -113,7 +113,7 @@ struct DevBufs {
   // edges
   float* ecand_d2;      // [cap][P]
   uint32_t* ecand_id;   // [cap][P]
-  uint32_t* estage;     // [maxc][P] kept neighbour index by rank
+  uint32_t* estage;     // [P][maxc] kept neighbour index by owner and rank
   uint32_t* ecount;     // [P + 1] kept edges per index (standalone pass, mesh)
   uint32_t* fcount;     // [P + 1] kept edges per index of the graph a step builds inside
                         // its force pass: zero between steps (the force pass writes the
@@ -155,9 +155,10 @@ struct ForceArgs {
   uint32_t* flag;
   float* ecand_d2;
   uint32_t* ecand_id;
-  uint32_t* estage;
+  uint32_t* estage;  // kept partners: owner o's rank-r partner at o * estride + r
   uint32_t* ecount;
   long P;
+  long estride;     // words per owner record in estage (the largest maxc allocated)
   int nwaves;
   int edges;        // fused edge candidates requested for this launch
   int edge_cap;
@@ -189,11 +190,12 @@ struct EdgeArgs {
   const uint32_t* __restrict__ wave_sim;
   float* ecand_d2;      // [cap][P] by slot: scratch of the streaming selection
   uint32_t* ecand_id;
-  uint32_t* estage;     // [maxc][P] kept partners by owner index and rank
+  uint32_t* estage;     // kept partners: owner index o's rank r at o * estride + r
   uint32_t* ecount;     // kept partners per owner index: non-zero counts only (zero between
                         // passes: the edge tiles clear what they read)
   uint32_t* etcount;    // per edge tile, in kEdgeSub parts (as ForceArgs::etcount)
   long P;
+  long estride;
   int nwaves;
   int edge_cap;
 };
@@ -213,12 +215,13 @@ struct SortArgs {
   const SimDesc* __restrict__ sims;
   const uint32_t* __restrict__ wave_sim;
   // canonical edge output (k_fill only; null when no edges this step)
-  const uint32_t* __restrict__ estage;
+  const uint32_t* __restrict__ estage;  // slot p's rank-r partner at p * estride + r
   const uint32_t* __restrict__ eoff;
   int2* __restrict__ edges;
   uint32_t* etotal;     // k_fill: receives the number of edges written (eoff[P])
   uint32_t* fcount;     // k_fill: the step's per-owner edge counts, cleared after use
   long P;
+  long estride;
   int nwaves;
 };
 
@@ -254,7 +257,8 @@ struct TileSortArgs {
   uint32_t* ecount;                     // per-owner counts: read, then cleared
   const uint32_t* etcount;
   uint32_t* etcount_next;
-  const uint32_t* estage;               // [rank][owner index]
+  const uint32_t* estage;               // owner index o's rank r at o * estride + r
+  long estride;
   int2* edges;
   uint32_t* etotal;                     // the last edge tile writes the edge count here
   long edge_room;                       // entries `edges` holds

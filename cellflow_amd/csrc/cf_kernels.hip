@@ -154,11 +154,11 @@ __device__ __forceinline__ void edge_push(EdgeAcc& e, float d2, uint32_t jid, in
 
 // Stable sort of the kept candidates by squared distance (ties by index, which is the
 // order the reference's insertion sort leaves, ParticleSimulation.cu:235-243), truncated
-// to maxc (:246).  Writes estage[rank][o] and returns the kept count.  o is the owner's
-// slot, or its index when the tile sort (which writes edges in index order) follows.
+// to maxc (:246).  Writes estage[o * es + rank] and returns the kept count.  o is the
+// owner's slot, or its index when the tile sort (which writes edges in index order) follows.
 __device__ __forceinline__ int edge_finish(const EdgeAcc& e, int cap, int maxc, const float* cd2,
                                            const uint32_t* cid, uint32_t* estage, long P, int p,
-                                           uint32_t o) {
+                                           uint32_t o, long es) {
   const int m = min(e.q, cap);
   for (int k = 0; k < m; ++k) {
     const float dk = cd2[(long)k * P + p];
@@ -169,7 +169,7 @@ __device__ __forceinline__ int edge_finish(const EdgeAcc& e, int cap, int maxc, 
       const uint32_t i2 = cid[(long)k2 * P + p];
       rank += (d2 < dk || (d2 == dk && i2 < ik)) ? 1 : 0;
     }
-    if (rank < maxc) estage[(long)rank * P + o] = ik;
+    if (rank < maxc) estage[(long)o * es + rank] = ik;
   }
   return min(m, maxc);
 }
@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs a) {
   if (EDGES) {
     if (do_edges) {
       const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage,
-                                a.P, p, a.tile_sort ? idi : (uint32_t)p);
+                                a.P, p, a.tile_sort ? idi : (uint32_t)p, a.estride);
       if (e != 0) a.ecount[idi] = (uint32_t)e;  // zero already (the sorts clear it)
       if (a.tile_sort) count_edges<true>(a, idi, (uint32_t)e);
     }
@@ -694,7 +694,7 @@ __global__ void __launch_bounds__(64) k_force_lds(ForceArgs a) {
   if (EDGES) {
     if (do_edges) {
       const int e = edge_finish(acc.ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id,
-                                a.estage, a.P, p, a.tile_sort ? idi : (uint32_t)p);
+                                a.estage, a.P, p, a.tile_sort ? idi : (uint32_t)p, a.estride);
       if (e != 0) a.ecount[idi] = (uint32_t)e;  // zero already (the sorts clear it)
       if (a.tile_sort) count_edges<true>(a, idi, (uint32_t)e);
     }
@@ -1150,7 +1150,8 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
           const uint32_t i2 = el_i[wv][o][k2];
           rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
         }
-        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + (TILE ? idi : (uint32_t)p)] = ik;
+        if (rank < (uint32_t)maxc)
+          a.estage[(long)(TILE ? idi : (uint32_t)p) * a.estride + rank] = ik;
       }
     if (do_edges && g == 0) {
       uint32_t e;
@@ -1181,7 +1182,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
           }
         }
         e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
-                                  TILE ? idi : (uint32_t)p);
+                                  TILE ? idi : (uint32_t)p, a.estride);
       }
       if (e != 0u) a.ecount[idi] = e;  // zero already (the sorts clear it)
       count_edges<TILE>(a, idi, e);
@@ -1492,7 +1493,8 @@ k_force_list(ForceArgs a) {
           const uint32_t i2 = el_i[wv][o][k2];
           rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
         }
-        if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + (TILE ? idi : (uint32_t)p)] = ik;
+        if (rank < (uint32_t)maxc)
+          a.estage[(long)(TILE ? idi : (uint32_t)p) * a.estride + rank] = ik;
       }
     if (do_edges && g == 0) {
       uint32_t e;
@@ -1526,7 +1528,7 @@ k_force_list(ForceArgs a) {
           }
         }
         e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p,
-                                  TILE ? idi : (uint32_t)p);
+                                  TILE ? idi : (uint32_t)p, a.estride);
       }
       ekept = e;
       if (!CONS) {
@@ -1577,8 +1579,8 @@ k_force_list(ForceArgs a) {
 // to the owner's LDS list; the two-phase selection then keeps the 2*maxC smallest indices
 // and ranks them by (d^2, index) (:228-246).  An owner with more than kListEdges candidates
 // takes the exact streaming selection of one lane.  Kept partners are staged by owner index
-// (estage[rank][index]), the non-zero per-owner counts go to `ecount` and to the edge-tile
-// counters, and k_tile_sort's edge tiles write the canonical list.
+// (estage[index * estride + rank]), the non-zero per-owner counts go to `ecount` and to the
+// edge-tile counters, and k_tile_sort's edge tiles write the canonical list.
 
 constexpr int kEdgeLanes = 8;
 
@@ -1777,7 +1779,7 @@ __device__ __forceinline__ void edges_owner(const EdgeArgs& a, const SimDesc& sd
         const uint32_t i2 = el_i[o][k2];
         rank += (d2.y != 0.0f && (d2.x < dk.x || (d2.x == dk.x && i2 < ik))) ? 1u : 0u;
       }
-      if (rank < (uint32_t)maxc) a.estage[(long)rank * a.P + idi] = ik;
+      if (rank < (uint32_t)maxc) a.estage[(long)idi * a.estride + rank] = ik;
     }
   if (!active || g != 0) return;
   uint32_t e;
@@ -1797,7 +1799,8 @@ __device__ __forceinline__ void edges_owner(const EdgeArgs& a, const SimDesc& sd
                 if (!(s2 < sd.edge_d2)) return;
                 if (jid > idi) edge_push(ea, s2, jid, cap, a.ecand_d2, a.ecand_id, a.P, p);
               });
-    e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi);
+    e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi,
+                              a.estride);
   }
   if (e) {  // zero already: the edge tiles that read the counts clear them
     a.ecount[idi] = e;
@@ -1952,7 +1955,7 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red, uint32_t*
           r -= c;
         }
         const long id = (long)u * kEdgeTile + (long)t * kPer + i;
-        src[b] = (long)r * a.P + id;
+        src[b] = id * a.estride + r;
         own[b] = (int)(id - (long)p0s[t]);
         ob[b] = p0s[t];
       }
@@ -2455,7 +2458,8 @@ __global__ void __launch_bounds__(256) k_fill(SortArgs a) {
     const int e = (int)(a.eoff[me + 1] - off);
     if (e) a.fcount[me] = 0u;  // the next step's force pass writes only non-zero counts
     for (int k = 0; k < e; ++k)
-      a.edges[off + k] = make_int2((int)(me - sd.p0), (int)(a.estage[(long)k * a.P + p] - sd.p0));
+      a.edges[off + k] =
+          make_int2((int)(me - sd.p0), (int)(a.estage[(long)p * a.estride + k] - sd.p0));
   }
 }
 
