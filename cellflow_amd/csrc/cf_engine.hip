@@ -218,11 +218,10 @@ struct Engine {
   // tile sort of the step: per-parity counters (the step with state buffer A counts into
   // parity A and clears parity B for the next step), far list, simulation of each tile
   uint32_t* tcnt[2] = {nullptr, nullptr};
-  uint32_t* etcnt[2] = {nullptr, nullptr};
-  // edge-tile counters of the standalone graph pass, in two parities (a pass counts into
-  // one and its edge tiles clear the other for the next pass)
-  uint32_t* etcs[2] = {nullptr, nullptr};
-  int epar = 0;
+  // the edge tiles' look-back (every launch with edge tiles): ticket counter and one flag
+  // word per edge tile, zeroed once with the buffers
+  unsigned long long* eticket = nullptr;
+  unsigned long long* eflags = nullptr;
   uint32_t* farn = nullptr;  // two words
   uint32_t* far_slot = nullptr;
   uint32_t* ftn[2] = {nullptr, nullptr};  // far movers per tile
@@ -276,11 +275,9 @@ struct Engine {
     vtx_cap = vcol_cap = iobuf_cap = 0;
     for (int i = 0; i < 2; ++i) {
       dfree(tcnt[i]);
-      dfree(etcnt[i]);
-      dfree(etcs[i]);
       dfree(ftn[i]);
     }
-    dfree(farn); dfree(far_slot); dfree(tile_sim); dfree(ftl);
+    dfree(farn); dfree(far_slot); dfree(tile_sim); dfree(ftl); dfree(eticket); dfree(eflags);
     for (auto& f : flags) dfree(f);
     if (hsmall) (void)hipHostFree(hsmall);
     hsmall = nullptr;
@@ -328,12 +325,9 @@ struct Engine {
     CFR(dalloc(&b.stotal, 1));
     CFR(dalloc(&b.gtotal, 1));
     netiles = (int)((P + kEdgeTile - 1) / kEdgeTile);
-    for (int i = 0; i < 2; ++i) {
-      CFR(dalloc(&etcnt[i], (size_t)netiles * kEdgeSub * kCtrStride));
-      CFR(dalloc(&etcs[i], (size_t)netiles * kEdgeSub * kCtrStride));
-      HIPR(hipMemsetAsync(etcs[i], 0, (size_t)netiles * kEdgeSub * kCtrStride * sizeof(uint32_t),
-                          st));
-    }
+    CFR(dalloc(&eticket, 1)); CFR(dalloc(&eflags, netiles));
+    HIPR(hipMemsetAsync(eticket, 0, sizeof(unsigned long long), st));
+    HIPR(hipMemsetAsync(eflags, 0, (size_t)netiles * sizeof(unsigned long long), st));
     CFR(dalloc(&farn, 2)); CFR(dalloc(&far_slot, P));
     HIPR(hipMemsetAsync(far_slot, 0, P * sizeof(uint32_t), st));
     HIPR(hipMemsetAsync(b.ecount, 0, (P + 1) * sizeof(uint32_t), st));
@@ -498,8 +492,6 @@ struct Engine {
         HIPR(hipMemsetAsync(tcnt[i], 0, (size_t)nt * kCtrStride * sizeof(uint32_t), st));
         HIPR(hipMemsetAsync(ftn[i], 0, (size_t)nt * kCtrStride * sizeof(uint32_t), st));
       }
-      HIPR(hipMemsetAsync(etcnt[i], 0, (size_t)netiles * kEdgeSub * kCtrStride * sizeof(uint32_t),
-                          st));
     }
     HIPR(hipMemsetAsync(farn, 0, 2 * sizeof(uint32_t), st));
     HIPR(hipMemcpyAsync(b.sims, hdesc.data(), hdesc.size() * sizeof(SimDesc),
@@ -705,7 +697,6 @@ struct Engine {
     fa.edge_cap = cap;
     fa.tile_sort = tun.sort_variant == 1 ? 1 : 0;
     fa.tcount = tcnt[A];
-    fa.etcount = etcnt[A];
     fa.far_n = farn + A;
     fa.far_slot = far_slot;
     fa.ftn = ftn[A];
@@ -773,8 +764,8 @@ struct Engine {
     ta.ntiles = ntiles;
     if (edges) {
       ta.ecount = b.fcount;
-      ta.etcount = etcnt[A];
-      ta.etcount_next = etcnt[B];
+      ta.eticket = eticket;
+      ta.eflags = eflags;
       ta.estage = b.estage;
       ta.estride = edge_maxc_alloc;
       ta.edges = b.sedges;
@@ -848,7 +839,6 @@ struct Engine {
     ea.estage = b.estage;
     ea.estride = edge_maxc_alloc;
     ea.ecount = b.fcount;
-    ea.etcount = etcs[epar];
     ea.P = P;
     ea.nwaves = nwaves;
     ea.edge_cap = cap;
@@ -862,8 +852,8 @@ struct Engine {
     ta.flag = b.flag;
     ta.ntiles = 0;
     ta.ecount = b.fcount;
-    ta.etcount = etcs[epar];
-    ta.etcount_next = etcs[1 - epar];
+    ta.eticket = eticket;
+    ta.eflags = eflags;
     ta.estage = b.estage;
     ta.estride = edge_maxc_alloc;
     ta.edges = b.edges;
@@ -876,7 +866,6 @@ struct Engine {
     const hipError_t e1 = launch_tile_sort(ta, st);
     arm_kernel_timer(nullptr, nullptr);
     HIPR(e1);
-    epar ^= 1;
     return CF_OK;
   }
 

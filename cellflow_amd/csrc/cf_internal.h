@@ -28,11 +28,9 @@ constexpr int kEdgeTile = 1024;        // owner ids per edge tile
 // << kCodeShift in LDS, so they take at most 2^kCodeShift slots.
 constexpr int kCodeShift = 26;
 constexpr uint32_t kFarBit = 0x80000000u;
-// Tile and edge-tile counters sit one per 128-byte line: a line takes every atomic of the
-// tiles it holds, one after another.  An edge tile's count is further split over kEdgeSub
-// lines (by the counting block's index), since its owners come from everywhere.
+// Tile counters sit one per 128-byte line: a line takes every atomic of the tiles it holds,
+// one after another.
 constexpr int kCtrStride = 32;
-constexpr int kEdgeSub = 8;
 // Far movers are listed by destination tile (kFarCap per tile); a tile's excess goes to one
 // shared overflow list that every tile reads.
 constexpr int kFarCap = 256;
@@ -167,8 +165,6 @@ struct ForceArgs {
   int tile_sort;
   // tile sort inputs (this step's counters; the sort clears the next step's)
   uint32_t* tcount;     // members per cell tile (tile t at t * kCtrStride)
-  uint32_t* etcount;    // edges per edge tile (owner id / kEdgeTile) in kEdgeSub parts,
-                        // part j of tile u at (u * kEdgeSub + j) * kCtrStride
   uint32_t* far_n;      // far movers in the overflow list
   uint32_t* far_slot;   // their pre-step slots
   uint32_t* ftn;        // far movers per tile (tile t at t * kCtrStride)
@@ -181,7 +177,7 @@ struct ForceArgs {
 };
 
 // The standalone proximity-graph pass (k_edges_g); k_tile_sort's edge tiles then write the
-// canonical list from estage, ecount and etcount.
+// canonical list from estage and ecount.
 struct EdgeArgs {
   const float4* __restrict__ pos;
   const uint32_t* __restrict__ id;
@@ -193,7 +189,6 @@ struct EdgeArgs {
   uint32_t* estage;     // kept partners: owner index o's rank r at o * estride + r
   uint32_t* ecount;     // kept partners per owner index: non-zero counts only (zero between
                         // passes: the edge tiles clear what they read)
-  uint32_t* etcount;    // per edge tile, in kEdgeSub parts (as ForceArgs::etcount)
   long P;
   long estride;
   int nwaves;
@@ -255,8 +250,12 @@ struct TileSortArgs {
   int ntiles;
   // edges (edge tiles; null edges: no edge tiles)
   uint32_t* ecount;                     // per-owner counts: read, then cleared
-  const uint32_t* etcount;
-  uint32_t* etcount_next;
+  // the edge tiles' decoupled look-back: a ticket counter that every edge-tile block of
+  // every launch draws from (tile = ticket % netiles, launch epoch = ticket / netiles) and
+  // one flag word per edge tile (epoch, status, value); both start at zero and are never
+  // reset while netiles stays the same
+  unsigned long long* eticket;
+  unsigned long long* eflags;
   const uint32_t* estage;               // owner index o's rank r at o * estride + r
   long estride;
   int2* edges;

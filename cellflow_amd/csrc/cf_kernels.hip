@@ -231,15 +231,6 @@ __device__ __forceinline__ void commit_cell(const ForceArgs& a, const SimDesc& s
   }
 }
 
-// For the tile sort: an owner's edge count into its edge tile's count (one of kEdgeSub
-// parts).
-template <bool TILE>
-__device__ __forceinline__ void count_edges(const ForceArgs& a, uint32_t idi, uint32_t e) {
-  if (TILE && e > 0u)
-    atomicAdd(&a.etcount[((idi / kEdgeTile) * kEdgeSub + (blockIdx.x & (kEdgeSub - 1))) * kCtrStride],
-              e);
-}
-
 // One axis of the force stencil as up to three runs of cells: the run below cell 0
 // (wrapped, image shift -L), the unwrapped run, and the run above cell n-1 (wrapped,
 // shift +L).  Axes with fewer than 2r+1 cells are one run over every cell (the minimum
@@ -404,7 +395,6 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs a) {
       const int e = edge_finish(ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id, a.estage,
                                 a.P, p, a.tile_sort ? idi : (uint32_t)p, a.estride);
       if (e != 0) a.ecount[idi] = (uint32_t)e;  // zero already (the sorts clear it)
-      if (a.tile_sort) count_edges<true>(a, idi, (uint32_t)e);
     }
   }
 }
@@ -696,7 +686,6 @@ __global__ void __launch_bounds__(64) k_force_lds(ForceArgs a) {
       const int e = edge_finish(acc.ea, a.edge_cap, sd.edge_maxc, a.ecand_d2, a.ecand_id,
                                 a.estage, a.P, p, a.tile_sort ? idi : (uint32_t)p, a.estride);
       if (e != 0) a.ecount[idi] = (uint32_t)e;  // zero already (the sorts clear it)
-      if (a.tile_sort) count_edges<true>(a, idi, (uint32_t)e);
     }
   }
 }
@@ -1185,7 +1174,6 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
                                   TILE ? idi : (uint32_t)p, a.estride);
       }
       if (e != 0u) a.ecount[idi] = e;  // zero already (the sorts clear it)
-      count_edges<TILE>(a, idi, e);
     }
   }
 
@@ -1531,10 +1519,7 @@ k_force_list(ForceArgs a) {
                                   TILE ? idi : (uint32_t)p, a.estride);
       }
       ekept = e;
-      if (!CONS) {
-        if (e != 0u) a.ecount[idi] = e;  // zero already (the sorts clear it)
-        count_edges<TILE>(a, idi, e);
-      }
+      if (!CONS && e != 0u) a.ecount[idi] = e;  // zero already (the sorts clear it)
     }
   }
 
@@ -1557,7 +1542,6 @@ k_force_list(ForceArgs a) {
     if (fl & (1u << 31)) {
       const uint32_t e = fl & 0x3FFFFFFFu;
       if (e != 0u) a.ecount[pl_id] = e;  // zero already (the sorts clear it)
-      count_edges<TILE>(a, pl_id, e);
     }
     integrate_commit<TILE>(a, sd, pl, pl_pos, pl_vel, pl_id, r.x, r.y, r.z,
                            __float_as_int(r.w), bin_axis(pl_pos.y, sd.ihy, sd.ny),
@@ -1802,11 +1786,7 @@ __device__ __forceinline__ void edges_owner(const EdgeArgs& a, const SimDesc& sd
     e = (uint32_t)edge_finish(ea, cap, maxc, a.ecand_d2, a.ecand_id, a.estage, a.P, p, idi,
                               a.estride);
   }
-  if (e) {  // zero already: the edge tiles that read the counts clear them
-    a.ecount[idi] = e;
-    atomicAdd(&a.etcount[((idi / kEdgeTile) * kEdgeSub + (blockIdx.x & (kEdgeSub - 1))) *
-                         kCtrStride], e);
-  }
+  if (e) a.ecount[idi] = e;  // zero already: the edge tiles that read the counts clear them
 }
 
 __global__ void __launch_bounds__(256) k_edges_g(EdgeArgs a) {
@@ -1846,13 +1826,70 @@ __global__ void __launch_bounds__(256) k_edges_g(EdgeArgs a) {
 // before it, its cells' starts follow from the per-cell counts, and its particles are
 // found by reading the keys of its window (and the far list).  Inside a cell they are
 // ranked by index, so the order is the canonical one whatever order the atomics arrived
-// in.  An edge tile writes its owners' edges at the counts of the edge tiles before it
-// plus a local scan.  Each block also clears its own counter for the next step (the next
-// step counts into the other parity).
+// in.  An edge tile writes its owners' edges at the sum of the counts of the edge tiles
+// before it (a decoupled look-back) plus a local scan.  Each cell-tile block also clears its
+// own counters for the next step (the next step counts into the other parity).
 
 constexpr int kSortThreads = 256;
 constexpr int kVisitBatch = 8;  // window slots per thread with their loads in flight together
 constexpr int kMoveBatch = 4;   // members per thread moved together
+
+// Flag word of a look-back tile: launch epoch (30 bits), status (0 none, 1 the tile's own
+// sum, 2 the inclusive prefix), value.
+__device__ __forceinline__ unsigned long long pack_flag(uint32_t epoch, uint32_t status,
+                                                        uint32_t value) {
+  return ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34) |
+         ((unsigned long long)status << 32) | (unsigned long long)value;
+}
+
+// Single-pass decoupled look-back for tile g, in two halves, each run by one whole wave.
+// lookback_publish publishes the tile's sum `total` (as its inclusive prefix when g ==
+// first).  lookback_wait then reads back over tiles g-1, g-2, ... (lane l reads tile
+// hi - l) until an inclusive prefix is found, publishes the tile's own inclusive prefix
+// and returns the exclusive one.  Tiles are numbered by tickets drawn at block start, so
+// every earlier tile's workgroup is already running and publishes its sum without waiting
+// on anything; tiles before `first` count as 0.  Flags of another epoch (an earlier
+// launch) read as status 0.
+__device__ __forceinline__ void lookback_publish(unsigned long long* flags, int g, int first,
+                                                 uint32_t epoch, uint32_t total, int lane) {
+  if (lane == 0)
+    __hip_atomic_store(&flags[g], pack_flag(epoch, g == first ? 2u : 1u, total),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ uint32_t lookback_wait(unsigned long long* flags, int g, int first, uint32_t epoch,
+                                  uint32_t total, int lane) {
+  if (g == first) return 0u;
+  uint32_t prefix = 0;
+  int hi = g - 1;
+  while (true) {
+    const int j = hi - lane;
+    uint32_t st = 2u, val = 0u;
+    if (j >= first) {
+      const unsigned long long w =
+          __hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      st = ((uint32_t)(w >> 34) == epoch) ? ((uint32_t)(w >> 32) & 3u) : 0u;
+      val = (uint32_t)w;
+    }
+    const unsigned long long incl = __ballot(st == 2u);
+    const int stop = incl ? (__ffsll((long long)incl) - 1) : (kWave - 1);
+    const unsigned long long need = (stop == kWave - 1) ? ~0ull : ((1ull << (stop + 1)) - 1ull);
+    if (__ballot(st == 0u) & need) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint32_t v = (lane <= stop) ? val : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, kWave);
+    prefix += v;
+    if (incl) break;
+    hi -= kWave;
+  }
+  if (lane == 0)
+    __hip_atomic_store(&flags[g], pack_flag(epoch, 2u, prefix + total), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return prefix;
+}
 
 // Sums of `v` and `u` over the block, and the exclusive scan of `w` in thread order with
 // its total, behind one pair of barriers.  `red` holds 12 words.
@@ -1887,54 +1924,64 @@ __device__ __forceinline__ void block_sums_excl_u32(uint32_t v, uint32_t u, uint
   wpre = pre + incl - w;
 }
 
-// An edge tile's owners' edges: the counts and the other tiles' totals give the tile's
-// place in the list, then the block writes the tile's edges e = tid, tid + 256, ...: each
-// finds its owner in LDS (the last thread whose first edge is at or before e, then that
-// thread's owners in order) and its rank k = e - the owner's first edge, so a thread's
-// staged-partner loads are all in flight together and the stores are contiguous.
-// `toff`, `p0s` (kSortThreads words) and `ecs` (kEdgeTile words) are LDS scratch.
-__device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red, uint32_t* toff,
-                          uint32_t* p0s, uint32_t* ecs) {
+// An edge tile's owners' edges.  The block draws a ticket (its edge tile, in the order the
+// edge-tile blocks started), sums its owners' counts and finds the tile's place in the list
+// by a decoupled look-back over the tiles before it; then it writes the tile's edges
+// e = tid, tid + 256, ...: each finds its owner in LDS (the last thread whose first edge is
+// at or before e, then that thread's owners in order) and its rank k = e - the owner's
+// first edge, so a thread's staged-partner loads are all in flight together and the stores
+// are contiguous.  `toff`, `p0s` (kSortThreads words) and `ecs` (kEdgeTile words) are LDS
+// scratch.
+__device__ void edge_tile(const TileSortArgs& a, uint32_t* red, uint32_t* toff, uint32_t* p0s,
+                          uint32_t* ecs) {
+  __shared__ uint32_t s_u, s_epoch, s_base;
   const int tid = threadIdx.x;
-  if (tid < kEdgeSub) a.etcount_next[(u * kEdgeSub + tid) * kCtrStride] = 0u;
-  // the owners' edge counts and their simulation's first slot (a thread's kPer owners share
-  // one 64-slot group) depend on nothing below: loaded first
+  const int lane = tid & (kWave - 1);
+  if (tid == 0) {
+    const unsigned long long t = atomicAdd(a.eticket, 1ull);
+    s_epoch = (uint32_t)((t / (unsigned long long)a.netiles) & 0x3FFFFFFFu);
+    s_u = (uint32_t)(t % (unsigned long long)a.netiles);
+  }
+  // the owners' edge counts (a thread's kPer owners share one 64-slot group): loaded for the
+  // tile of the block's own position while the ticket is in flight (blocks nearly always
+  // start in order), and again in the rare case the ticket names another tile
   constexpr int kPer = kEdgeTile / kSortThreads;
-  const long id0 = (long)u * kEdgeTile + (long)tid * kPer;
+  int u = (int)blockIdx.x - a.ntiles;
+  long id0 = (long)u * kEdgeTile + (long)tid * kPer;
   uint32_t ec[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) ec[i] = (id0 + i < a.P) ? a.ecount[id0 + i] : 0u;
+  __syncthreads();
+  if ((int)s_u != u) {
+    u = (int)s_u;
+    id0 = (long)u * kEdgeTile + (long)tid * kPer;
 #pragma unroll
-  for (int i = 0; i < kPer; ++i)  // the next step's force pass writes only non-zero counts
+    for (int i = 0; i < kPer; ++i) ec[i] = (id0 + i < a.P) ? a.ecount[id0 + i] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i)  // the next pass writes only non-zero counts
     if (ec[i]) a.ecount[id0 + i] = 0u;
   // one simulation starts at slot 0; otherwise the owners' simulation's first slot
   const int p0 = (a.one_sim || id0 >= a.P) ? 0 : a.sims[a.wave_sim[id0 / kWave]].p0;
-  uint32_t part = 0, mine = 0;
-  for (int v = tid; v < (u + 1) * kEdgeSub; v += kSortThreads) {
-    const uint32_t x = a.etcount[v * kCtrStride];
-    if (v < u * kEdgeSub) part += x;
-    else mine += x;
-  }
   uint32_t run = 0;
 #pragma unroll
   for (int i = 0; i < kPer; ++i) run += ec[i];
-  uint32_t own, base, pre, total;
-  block_sums_excl_u32(mine, part, run, red, own, base, pre, total);
-  if (total != own || (long)base + total > a.edge_room) {
-    if (tid == 0) atomicOr(a.flag, 4u);
-    return;
-  }
-  if (tid == 0 && u == a.netiles - 1) *a.etotal = base + total;  // the whole list's size
+  uint32_t unused0, unused1, pre, total;
+  block_sums_excl_u32(0u, 0u, run, red, unused0, unused1, pre, total);
+  const uint32_t epoch = s_epoch;
+  if (tid < kWave) lookback_publish(a.eflags, u, 0, epoch, total, lane);
   toff[tid] = pre;
   p0s[tid] = (uint32_t)p0;
 #pragma unroll
   for (int i = 0; i < kPer; ++i) ecs[tid * kPer + i] = ec[i];
   __syncthreads();
-  constexpr int kB = 4;  // edges per thread with their loads in flight together
-  for (uint32_t e0 = 0; e0 < total; e0 += kSortThreads * kB) {
+  // edges e0 + b * kSortThreads + tid, b < kB: the staged partners (loads issued together),
+  // then the pairs stored at base + e
+  constexpr int kB = 4;
+  int own[kB];
+  uint32_t ob[kB], jv[kB];
+  auto gather = [&](uint32_t e0) {
     long src[kB];
-    int own[kB];
-    uint32_t ob[kB];
 #pragma unroll
     for (int b = 0; b < kB; ++b) {
       const uint32_t e = e0 + (uint32_t)(b * kSortThreads + tid);
@@ -1960,17 +2007,36 @@ __device__ void edge_tile(const TileSortArgs& a, int u, uint32_t* red, uint32_t*
         ob[b] = p0s[t];
       }
     }
-    uint32_t jv[kB];
 #pragma unroll
     for (int b = 0; b < kB; ++b) {
       const uint32_t e = e0 + (uint32_t)(b * kSortThreads + tid);
       jv[b] = (e < total) ? a.estage[src[b]] : 0u;
     }
+  };
+  auto put = [&](uint32_t e0, uint32_t base) {
 #pragma unroll
     for (int b = 0; b < kB; ++b) {
       const uint32_t e = e0 + (uint32_t)(b * kSortThreads + tid);
       if (e < total) a.edges[(long)base + e] = make_int2(own[b], (int)(jv[b] - ob[b]));
     }
+  };
+  // the first edges' loads go out before the look-back waits: only the stores need the base
+  gather(0u);
+  if (tid < kWave) {
+    const uint32_t b = lookback_wait(a.eflags, u, 0, epoch, total, lane);
+    if (lane == 0) s_base = b;
+  }
+  __syncthreads();
+  const uint32_t base = s_base;
+  if ((long)base + total > a.edge_room) {
+    if (tid == 0) atomicOr(a.flag, 4u);
+    return;
+  }
+  if (tid == 0 && u == a.netiles - 1) *a.etotal = base + total;  // the whole list's size
+  put(0u, base);
+  for (uint32_t e0 = kSortThreads * kB; e0 < total; e0 += kSortThreads * kB) {
+    gather(e0);
+    put(e0, base);
   }
 }
 
@@ -1992,7 +2058,7 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
   const int lane = tid & (kWave - 1);
   if ((int)blockIdx.x >= a.ntiles) {
     static_assert(kTileCells >= kEdgeTile && kTileCells >= kSortThreads, "edge tile scratch");
-    edge_tile(a, (int)blockIdx.x - a.ntiles, red, cofs, fill_s, cnt_s);
+    edge_tile(a, red, cofs, fill_s, cnt_s);
     return;
   }
   // consecutive tiles (consecutive slots) share an XCD, as in the force pass
@@ -2318,12 +2384,6 @@ __global__ void __launch_bounds__(kSortThreads) k_tile_sort(TileSortArgs a) {
 // ------------------------------------------------------------------------------------
 // k_scan: single-pass decoupled look-back exclusive scan over up to two arrays.
 
-__device__ __forceinline__ unsigned long long pack_flag(uint32_t epoch, uint32_t status,
-                                                        uint32_t value) {
-  return ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34) |
-         ((unsigned long long)status << 32) | (unsigned long long)value;
-}
-
 __global__ void __launch_bounds__(kScanThreads) k_scan(ScanPlan plan) {
   __shared__ uint32_t smem[kScanTile + kScanTile / 32];
   __shared__ uint32_t wsum[kScanThreads / kWave];
@@ -2374,48 +2434,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(ScanPlan plan) {
   const uint32_t texcl = wpre + incl - sum;
 
   if (wv == 0) {
-    uint32_t prefix = 0;
-    if (tile == 0) {
-      if (lane == 0)
-        __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 2u, total), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (lane == 0)
-        __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 1u, total), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      // Wave-parallel look-back: lane l reads the flag of tile g-1-l.  Every earlier tile
-      // holds an earlier ticket, so its workgroup is already running and publishes its
-      // aggregate without waiting; tiles before the segment start count as prefix 0.
-      const int first = (int)g - tile;
-      int hi = (int)g - 1;
-      while (true) {
-        const int j = hi - lane;
-        uint32_t st = 2u, val = 0u;
-        if (j >= first) {
-          const unsigned long long w =
-              __hip_atomic_load(&plan.flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          st = ((uint32_t)(w >> 34) == epoch) ? ((uint32_t)(w >> 32) & 3u) : 0u;
-          val = (uint32_t)w;
-        }
-        const unsigned long long incl = __ballot(st == 2u);
-        const int stop = incl ? (__ffsll((long long)incl) - 1) : (kWave - 1);
-        const unsigned long long need =
-            (stop == kWave - 1) ? ~0ull : ((1ull << (stop + 1)) - 1ull);
-        if (__ballot(st == 0u) & need) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        uint32_t v = (lane <= stop) ? val : 0u;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, kWave);
-        prefix += v;
-        if (incl) break;
-        hi -= kWave;
-      }
-      if (lane == 0)
-        __hip_atomic_store(&plan.flags[g], pack_flag(epoch, 2u, prefix + total), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
+    lookback_publish(plan.flags, (int)g, (int)g - tile, epoch, total, lane);
+    const uint32_t prefix = lookback_wait(plan.flags, (int)g, (int)g - tile, epoch, total, lane);
     if (lane == 0) s_prefix = prefix;
   }
   __syncthreads();
