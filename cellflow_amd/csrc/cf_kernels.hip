@@ -59,10 +59,9 @@ void arm_kernel_timer(hipEvent_t start, hipEvent_t stop) {
 // small device helpers
 
 __device__ __forceinline__ int bin_axis(float x, float ih, int n) {
-  float u = x * ih;
-  if (!(u >= 0.0f)) return 0;
-  if (u >= (float)n) return n - 1;
-  return (int)u;
+  // floor(x * ih) clamped to [0, n - 1], NaN to 0 (fmaxf takes the number); n - 1 < 2^24 is
+  // exact in float, and u in [n - 1, n) truncates to n - 1 either way
+  return (int)fminf(fmaxf(x * ih, 0.0f), (float)(n - 1));
 }
 
 __device__ __forceinline__ float min_image(float d, float L) {
@@ -732,8 +731,10 @@ __device__ __forceinline__ float cell_gap(int d, float f) {
   return fmaxf(g - 1e-3f, 0.0f);
 }
 
+// Image shift of one axis from its 2-bit code (a signed count of periods: 1 = +L, 3 = -L,
+// 0 = none): exactly 0 or +-L.
 __device__ __forceinline__ float shift_of(uint32_t c, float L) {
-  return (c == 1u) ? -L : ((c == 2u) ? L : 0.0f);
+  return (float)(((int)(c << 30)) >> 30) * L;
 }
 
 // Per-particle context of the stencil walk.
@@ -778,7 +779,7 @@ __device__ __forceinline__ float gap_of(int d, const float* lo, const float* hi)
 
 // The up to two runs of stencil row r (x-neighbour cells are adjacent slots): the
 // unwrapped part and the part wrapped across the x boundary, each with its image-shift
-// code (x: bits 0-1, y: 2-3, z: 4-5; 1 = -L, 2 = +L).  Rows and x ends farther than the
+// code (x: bits 0-1, y: 2-3, z: 4-5; 1 = +L, 3 = -L).  Rows and x ends farther than the
 // type's reach (with margins) are dropped.  Length 0 means no run.
 template <int R>
 __device__ __forceinline__ void row_runs_gen(const ForceArgs& a, const SimDesc& sd, const Own& c,
@@ -800,16 +801,16 @@ __device__ __forceinline__ void row_runs_gen(const ForceArgs& a, const SimDesc& 
   } else {
     dy = iy - R;
     y = c.cy + dy;
-    if (y < 0) { y += sd.ny; cyc = 1u; }
-    else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
+    if (y < 0) { y += sd.ny; cyc = 3u; }
+    else if (y >= sd.ny) { y -= sd.ny; cyc = 1u; }
   }
   if (c.gen && sd.nz < 2 * R + 1) {
     z = iz;
   } else {
     dz = iz - R;
     z = c.cz + dz;
-    if (z < 0) { z += sd.nz; czc = 1u; }
-    else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
+    if (z < 0) { z += sd.nz; czc = 3u; }
+    else if (z >= sd.nz) { z -= sd.nz; czc = 1u; }
   }
   float gyz = 0.0f;
   if (!c.gen) {
@@ -841,8 +842,8 @@ __device__ __forceinline__ void row_runs_gen(const ForceArgs& a, const SimDesc& 
   const uint32_t yzc = (cyc << 2) | (czc << 4);
   int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
   uint32_t xc1 = 0u;
-  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
-  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
+  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 3u; }
+  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 1u; }
   s0 = a.start[rowc + a0];
   l0 = a.start[rowc + b0 + 1] - s0;
   k0 = yzc;
@@ -870,10 +871,10 @@ __device__ __forceinline__ void row_runs_fast(const ForceArgs& a, const SimDesc&
   if (!(gyz < c.cut2)) return;
   int y = c.cy + dy, z = c.cz + dz;
   uint32_t cyc = 0u, czc = 0u;
-  if (y < 0) { y += sd.ny; cyc = 1u; }
-  else if (y >= sd.ny) { y -= sd.ny; cyc = 2u; }
-  if (z < 0) { z += sd.nz; czc = 1u; }
-  else if (z >= sd.nz) { z -= sd.nz; czc = 2u; }
+  if (y < 0) { y += sd.ny; cyc = 3u; }
+  else if (y >= sd.ny) { y -= sd.ny; cyc = 1u; }
+  if (z < 0) { z += sd.nz; czc = 3u; }
+  else if (z >= sd.nz) { z -= sd.nz; czc = 1u; }
   const int rowc = sd.c0 + sd.nx * (y + sd.ny * z);
   int xlo = c.cx, xhi = c.cx;
 #pragma unroll
@@ -884,8 +885,8 @@ __device__ __forceinline__ void row_runs_fast(const ForceArgs& a, const SimDesc&
   const uint32_t yzc = (cyc << 2) | (czc << 4);
   int a0 = xlo, b0 = xhi, a1 = 1, b1 = 0;
   uint32_t xc1 = 0u;
-  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 1u; }
-  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 2u; }
+  if (xlo < 0) { a0 = 0; a1 = xlo + sd.nx; b1 = sd.nx - 1; xc1 = 3u; }
+  else if (xhi >= sd.nx) { b0 = sd.nx - 1; a1 = 0; b1 = xhi - sd.nx; xc1 = 1u; }
   s0 = a.start[rowc + a0];
   l0 = a.start[rowc + b0 + 1] - s0;
   k0 = yzc;
@@ -1365,7 +1366,8 @@ k_force_list(ForceArgs a) {
   const PairConst* row = &tab[wv][ti * sd.T];
 
   // ---- candidates, kList at a time -------------------------------------------------
-  const bool do_edges = EDGES && a.edges && sd.edge_on && active;
+  const bool want_edges = EDGES && a.edges && sd.edge_on;  // the same for the whole wave
+  const bool do_edges = want_edges && active;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   int cnt = 0;
   for (uint32_t b0 = 0; b0 < total; b0 += (uint32_t)kList) {
@@ -1399,7 +1401,7 @@ k_force_list(ForceArgs a) {
         const uint32_t q = e & ((1u << kCodeShift) - 1u), code = e >> kCodeShift;
         const float4 pj = a.pos[q];
         // the index travels with the position: an edge candidate then waits for no load
-        const uint32_t jq = (EDGES && do_edges) ? a.id[q] : 0u;
+        const uint32_t jq = want_edges ? a.id[q] : 0u;
         float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
         bool raw;
         if constexpr (!GEN) {
