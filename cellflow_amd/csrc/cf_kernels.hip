@@ -1298,9 +1298,13 @@ k_force_list(ForceArgs a) {
   constexpr int kRows = (2 * R + 1) * (2 * R + 1);
   constexpr int kOwn = kWave / G;
   constexpr int kRpl = (kRows + G - 1) / G;  // stencil rows per lane
-  __shared__ PairConst tab[4][kMaxTypes * kMaxTypes];
-  // list entries: slot | image-shift code << kCodeShift (slots stay below 2^kCodeShift)
-  __shared__ uint32_t lslot[4][kOwn][kList];
+  // the pair table by field (thr, fv, inv_eff: kMaxTypes^2 words each), so the lanes of
+  // different owners, which read rows of different types, fall on different banks
+  constexpr int kTT = kMaxTypes * kMaxTypes;
+  __shared__ float ptab[4][3 * kTT];
+  // list entries: slot | image-shift code << kCodeShift (slots stay below 2^kCodeShift);
+  // owners kList + 8 words apart, so the 8 owners' lanes of one read fall on 8 bank groups
+  __shared__ uint32_t lslot[4][kOwn][kList + 8];
   __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_n[4][kOwn];
@@ -1327,7 +1331,12 @@ k_force_list(ForceArgs a) {
   const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
   const SimDesc sd = a.sims[s];
   if (wave_ok)
-    for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = a.pairs[sd.tab + t];
+    for (int t = lane; t < sd.T * sd.T; t += kWave) {
+      const PairConst pt = a.pairs[sd.tab + t];
+      ptab[wv][t] = pt.thr;
+      ptab[wv][kTT + t] = pt.fv;
+      ptab[wv][2 * kTT + t] = pt.inv_eff;
+    }
   const bool active = wave_ok && (p - sd.p0) < sd.n;
   const float4 pi = c.pi;
   c.ti = active ? __float_as_uint(pi.w) : 0u;
@@ -1363,7 +1372,7 @@ k_force_list(ForceArgs a) {
   const uint32_t first = group_excl_scan<G>(mine, g);
   const uint32_t total = (uint32_t)dpp_group_sum_i<G>((int)mine);
   wave_lds_sync();  // pair table ready
-  const PairConst* row = &tab[wv][ti * sd.T];
+  const float* row = &ptab[wv][ti * sd.T];
 
   // ---- candidates, kList at a time -------------------------------------------------
   const bool want_edges = EDGES && a.edges && sd.edge_on;  // the same for the whole wave
@@ -1420,14 +1429,15 @@ k_force_list(ForceArgs a) {
         }
         const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         const uint32_t tj = __float_as_uint(pj.w);
-        const PairConst pc = row[tj];
+        const float thr = row[tj];
         const float sp = s2 + 1e-4f;
-        if (sp < pc.thr) {
+        if (sp < thr) {
           cnt++;
+          const float fv = row[kTT + tj], ie = row[2 * kTT + tj];
           const float inv = __builtin_amdgcn_rsqf(sp);
-          const float rr = (sp * inv) * pc.inv_eff;
+          const float rr = (sp * inv) * ie;
           const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
-          const float w = (sd.rep * ex - sd.att * rr) * pc.fv * inv;
+          const float w = (sd.rep * ex - sd.att * rr) * fv * inv;
           fx += dx * w;
           fy += dy * w;
           fz += dz * w;
@@ -1455,7 +1465,7 @@ k_force_list(ForceArgs a) {
   fy = dpp_group_sum_f<G>(fy);
   fz = dpp_group_sum_f<G>(fz);
   cnt = dpp_group_sum_i<G>(cnt);
-  if (active && 1e-4f < row[ti].thr) cnt -= 1;  // the particle itself
+  if (active && 1e-4f < row[ti]) cnt -= 1;  // the particle itself
 
   uint32_t ekept = 0u;  // kept edges of the particle (lane g == 0), passed on by CONS
   if (EDGES) {
