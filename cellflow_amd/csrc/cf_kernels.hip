@@ -1360,15 +1360,34 @@ k_force_list(ForceArgs a) {
   // ---- this lane's runs, their place in the particle's flattened candidate order ----
   uint32_t rs[2 * kRpl], rl[2 * kRpl], rk[2 * kRpl];
   uint32_t mine = 0u;
+  if (R == 1 && G == 8 && !c.gen) {
+    // 9 rows over 8 lanes in one pass: lane g takes row g (lane 4 row 8 in place of the
+    // centre row 4), and every lane works out the centre row, which has no y or z wrap
+    // and no y or z gap (so the row itself folds to its x ends), for lane 0 to keep
+    static_assert(R != 1 || G != 8 || kRpl == 2, "two run pairs per lane");
 #pragma unroll
-  for (int i = 0; i < kRpl; ++i) {
-    const int r = g + i * G;
-    rs[2 * i] = rl[2 * i] = rk[2 * i] = rs[2 * i + 1] = rl[2 * i + 1] = rk[2 * i + 1] = 0u;
-    if (active && r < nrows)
-      row_runs<R>(a, sd, c, r, rs[2 * i], rl[2 * i], rk[2 * i], rs[2 * i + 1], rl[2 * i + 1],
-                  rk[2 * i + 1]);
-    mine += rl[2 * i] + rl[2 * i + 1];
+    for (int k = 0; k < 2 * kRpl; ++k) rs[k] = rl[k] = rk[k] = 0u;
+    if (active) {
+      row_runs_fast<R>(a, sd, c, g == 4 ? 8 : g, rs[0], rl[0], rk[0], rs[1], rl[1], rk[1]);
+      row_runs_fast<R>(a, sd, c, 4, rs[2], rl[2], rk[2], rs[3], rl[3], rk[3]);
+      if (g != 0) rl[2] = rl[3] = 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * kRpl; ++k) mine += rl[k];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kRpl; ++i) {
+      const int r = g + i * G;
+      rs[2 * i] = rl[2 * i] = rk[2 * i] = rs[2 * i + 1] = rl[2 * i + 1] = rk[2 * i + 1] = 0u;
+      if (active && r < nrows)
+        row_runs<R>(a, sd, c, r, rs[2 * i], rl[2 * i], rk[2 * i], rs[2 * i + 1], rl[2 * i + 1],
+                    rk[2 * i + 1]);
+      mine += rl[2 * i] + rl[2 * i + 1];
+    }
   }
+  // each run as its first list entry (slot | code << kCodeShift): one register per run
+#pragma unroll
+  for (int k = 0; k < 2 * kRpl; ++k) rs[k] |= rk[k] << kCodeShift;
   const uint32_t first = group_excl_scan<G>(mine, g);
   const uint32_t total = (uint32_t)dpp_group_sum_i<G>((int)mine);
   wave_lds_sync();  // pair table ready
@@ -1389,7 +1408,7 @@ k_force_list(ForceArgs a) {
       // store instruction
       if (lo < hi) {
         uint32_t* d = &lslot[wv][o][lo - b0];
-        uint32_t v = (rs[k] | (rk[k] << kCodeShift)) + (lo - f0);
+        uint32_t v = rs[k] + (lo - f0);
         uint32_t m = hi - lo;
         for (; m >= 2u; m -= 2u, d += 2, v += 2u) {
           d[0] = v;
@@ -1466,6 +1485,8 @@ k_force_list(ForceArgs a) {
   fz = dpp_group_sum_f<G>(fz);
   cnt = dpp_group_sum_i<G>(cnt);
   if (active && 1e-4f < row[ti]) cnt -= 1;  // the particle itself
+  // CONS: the sums go to LDS now, so they take no registers through the edge selection
+  if (CONS && g == 0) cres[wv * kOwn + o] = make_float4(fx, fy, fz, __int_as_float(cnt));
 
   uint32_t ekept = 0u;  // kept edges of the particle (lane g == 0), passed on by CONS
   if (EDGES) {
@@ -1496,13 +1517,121 @@ k_force_list(ForceArgs a) {
         if (rank < (uint32_t)maxc)
           a.estage[(long)(TILE ? idi : (uint32_t)p) * a.estride + rank] = ik;
       }
+    // Owners with more candidates than their LDS list holds (n counts them all): the whole
+    // wave takes them one at a time, in the wave's LDS lists (free now: every fitting
+    // owner's selection above is done).  Its 64 lanes walk the owner's flattened candidates
+    // again, append the qualifying ones (d^2 in el_d's words, index in lslot's, up to
+    // kOvfCap), keep the cap smallest indices and rank those by (d^2, index), as above.
+    // More than kOvfCap qualifying: the owner's lane takes the streaming selection below.
+    uint32_t ovf_e = ~0u;  // kept edges of an owner this handled (its lane g == 0)
+    if constexpr (G == 8) {
+      constexpr int kOvfCap = kOwn * (kList + 8);  // words of the wave's lslot
+      static_assert(kOvfCap <= 2 * kOwn * kListEdges, "d^2 fits the wave's el_d");
+      constexpr int kOvfInfo = G * 2 * kRpl * 2;  // after the runs in the wave's el_i
+      static_assert(kOvfInfo + 7 <= kOwn * kListEdges, "runs and owner fit the wave's el_i");
+      float* ov_d = &el_d[wv][0][0].x;
+      uint32_t* ov_i = &lslot[wv][0][0];
+      uint32_t* ov_r = &el_i[wv][0][0];
+      unsigned long long left = __ballot(do_edges && g == 0 && !fits);
+      if (left) wave_lds_sync();  // the fitting owners' reads of el_d, el_i are done
+      while (left) {
+        const int L = __ffsll((long long)left) - 1;
+        left &= left - 1ull;
+        const int os = L / G;
+        // the owner's runs, position, type, index, candidate count and record go to LDS
+        // (read where used: they take no registers through the walk)
+        if (o == os)
+#pragma unroll
+          for (int k = 0; k < 2 * kRpl; ++k) {
+            ov_r[(g * 2 * kRpl + k) * 2] = rs[k];
+            ov_r[(g * 2 * kRpl + k) * 2 + 1] = rl[k];
+          }
+        if (lane == L) {
+          ov_r[kOvfInfo] = __float_as_uint(pi.x);
+          ov_r[kOvfInfo + 1] = __float_as_uint(pi.y);
+          ov_r[kOvfInfo + 2] = __float_as_uint(pi.z);
+          ov_r[kOvfInfo + 3] = ti;
+          ov_r[kOvfInfo + 4] = idi;
+          ov_r[kOvfInfo + 5] = total;
+          ov_r[kOvfInfo + 6] = TILE ? idi : (uint32_t)p;
+          el_n[wv][os] = 0u;
+        }
+        wave_lds_sync();
+        const uint32_t otot = ov_r[kOvfInfo + 5];
+        for (uint32_t i0 = 0; i0 < otot; i0 += (uint32_t)kWave) {
+          const uint32_t i = i0 + (uint32_t)lane;
+          if (i >= otot) continue;
+          uint32_t off = i, ent = 0u;
+          for (int r = 0; r < G * 2 * kRpl; ++r) {
+            const uint32_t len = ov_r[2 * r + 1];
+            if (off < len) {
+              ent = ov_r[2 * r] + off;
+              break;
+            }
+            off -= len;
+          }
+          const uint32_t q = ent & ((1u << kCodeShift) - 1u);
+          if (!c.gen && (ent >> kCodeShift) != 0u) continue;  // image-shifted: no partner
+          const float4 pj = a.pos[q];
+          if (__float_as_uint(pj.w) != ov_r[kOvfInfo + 3]) continue;
+          const float ux = pj.x - __uint_as_float(ov_r[kOvfInfo]),
+                      uy = pj.y - __uint_as_float(ov_r[kOvfInfo + 1]),
+                      uz = pj.z - __uint_as_float(ov_r[kOvfInfo + 2]);
+          if (c.gen && !(min_image(ux, sd.Lx) == ux && min_image(uy, sd.Ly) == uy &&
+                         min_image(uz, sd.Lz) == uz))
+            continue;
+          const float s2 = fmaf(uz, uz, fmaf(uy, uy, ux * ux));
+          if (!(s2 < sd.edge_d2)) continue;
+          const uint32_t jid = a.id[q];
+          if (jid <= ov_r[kOvfInfo + 4]) continue;
+          const uint32_t k = atomicAdd(&el_n[wv][os], 1u);
+          if (k < (uint32_t)kOvfCap) {
+            ov_d[k] = s2;
+            ov_i[k] = jid;
+          }
+        }
+        wave_lds_sync();
+        const uint32_t m = el_n[wv][os];
+        if (m <= (uint32_t)kOvfCap) {
+          // the cap smallest indices (marked in bit 31; indices stay below 2^26)
+          uint32_t keep = 0u;
+          for (uint32_t k = lane, b = 0; k < m; k += kWave, ++b) {
+            const uint32_t ik = ov_i[k];
+            uint32_t below = 0;
+            for (uint32_t k2 = 0; k2 < m; ++k2) below += (ov_i[k2] < ik) ? 1u : 0u;
+            keep |= (below < (uint32_t)cap ? 1u : 0u) << b;
+          }
+          wave_lds_sync();
+          for (uint32_t k = lane, b = 0; k < m; k += kWave, ++b)
+            if ((keep >> b) & 1u) ov_i[k] |= 0x80000000u;
+          wave_lds_sync();
+          for (uint32_t k = lane; k < m; k += kWave) {
+            const uint32_t ik = ov_i[k];
+            if (!(ik & 0x80000000u)) continue;
+            const float dk = ov_d[k];
+            uint32_t rank = 0;
+            for (uint32_t k2 = 0; k2 < m; ++k2) {
+              const uint32_t i2 = ov_i[k2];
+              const float d2 = ov_d[k2];
+              rank += ((i2 & 0x80000000u) && (d2 < dk || (d2 == dk && i2 < ik))) ? 1u : 0u;
+            }
+            if (rank < (uint32_t)maxc)
+              a.estage[(long)ov_r[kOvfInfo + 6] * a.estride + rank] = ik & 0x7FFFFFFFu;
+          }
+          if (lane == L) ovf_e = (uint32_t)min(min((int)m, cap), maxc);
+        }
+        wave_lds_sync();  // the LDS lists are reused for the next owner
+      }
+    }
     if (do_edges && g == 0) {
       uint32_t e;
       if (fits) {
         e = (uint32_t)min(min((int)n, cap), maxc);
+      } else if (ovf_e != ~0u) {
+        e = ovf_e;
       } else {
-        // more candidates than the LDS list holds: exact streaming selection by one lane,
-        // over the stencil rows again
+        // more candidates than the wave's LDS lists hold: exact streaming selection by one
+        // lane, over the stencil rows again
         EdgeAcc ea;
         ea.q = 0;
         ea.maxid = 0;
@@ -1537,10 +1666,7 @@ k_force_list(ForceArgs a) {
 
   if constexpr (CONS) {
     const bool de = EDGES && a.edges && sd.edge_on && active;
-    if (g == 0) {
-      cres[wv * kOwn + o] = make_float4(fx, fy, fz, __int_as_float(cnt));
-      cedge[wv * kOwn + o] = ekept | (active ? (1u << 30) : 0u) | (de ? (1u << 31) : 0u);
-    }
+    if (g == 0) cedge[wv * kOwn + o] = ekept | (active ? (1u << 30) : 0u) | (de ? (1u << 31) : 0u);
     __syncthreads();
     if (wv != 0 || lane >= 4 * kOwn) return;
     // wave 0: lane L finishes the block's particle L

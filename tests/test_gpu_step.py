@@ -187,11 +187,12 @@ def test_edges_truncation_matches_oracle(orc, maxc):
         assert np.array_equal(edges, ref), f"dist={dist} maxc={maxc}"
 
 
-@pytest.mark.parametrize("variant,lanes", [(0, 4), (0, 16), (3, 8), (3, 16), (3, 32)])
+@pytest.mark.parametrize("variant,lanes", [(0, 4), (0, 16), (3, 8), (3, 16), (3, 32), (4, 8)])
 def test_fused_edges_dense_blobs(orc, variant, lanes):
     """Fused edge candidates on dense blobs: many owners exceed the per-particle LDS edge
-    list and take the streaming fallback (and, in variant 3, hold several candidate-list
-    chunks); the graph must still equal the oracle's."""
+    list (variants 3 and 4 at 8 lanes then select in the wave's LDS lists, the others take
+    the streaming selection) and, in variants 3 and 4, hold several candidate-list chunks;
+    the graph must still equal the oracle's."""
     st, phys, params = clustered_state(n=5000, blobs=4, sd=0.006)
     sim, twin = make_sim(st, phys), make_sim(st, phys)
     for h in (sim, twin):
@@ -201,6 +202,18 @@ def test_fused_edges_dense_blobs(orc, variant, lanes):
             h.setTuning(force_variant=variant, lanes_per_particle=lanes, sort_variant=sort)
         check_step_edges(orc, sim, twin, params, (0.01, 16), steps=2)
         twin.simulate(params, 1)  # back in step with sim
+
+
+@pytest.mark.parametrize("variant", [3, 4])
+def test_fused_edges_past_the_wave_lists(orc, variant):
+    """One blob so dense that most owners have more qualifying partners than the wave's LDS
+    lists hold (320): those take the exact streaming selection, the few with fewer the
+    wave's; the graph must equal the oracle's."""
+    st, phys, params = clustered_state(n=4000, blobs=1, sd=0.002)
+    sim, twin = make_sim(st, phys), make_sim(st, phys)
+    for h in (sim, twin):
+        h.setTuning(force_variant=variant, lanes_per_particle=8)
+    assert check_step_edges(orc, sim, twin, params, (0.01, 16), steps=2) > 0
 
 
 def test_fused_edges_every_step(orc):
