@@ -1701,8 +1701,8 @@ k_force_list(ForceArgs a) {
 // to the owner's LDS list; the two-phase selection then keeps the 2*maxC smallest indices
 // and ranks them by (d^2, index) (:228-246).  An owner with more than kListEdges candidates
 // takes the exact streaming selection of one lane.  Kept partners are staged by owner index
-// (estage[index * estride + rank]), the non-zero per-owner counts go to `ecount` and to the
-// edge-tile counters, and k_tile_sort's edge tiles write the canonical list.
+// (estage[index * estride + rank]), the non-zero per-owner counts go to `ecount`, and
+// k_tile_sort's edge tiles write the canonical list.
 
 constexpr int kEdgeLanes = 8;
 
