@@ -1298,10 +1298,11 @@ k_force_list(ForceArgs a) {
   constexpr int kRows = (2 * R + 1) * (2 * R + 1);
   constexpr int kOwn = kWave / G;
   constexpr int kRpl = (kRows + G - 1) / G;  // stencil rows per lane
-  // the pair table by field (thr, fv, inv_eff: kMaxTypes^2 words each), so the lanes of
-  // different owners, which read rows of different types, fall on different banks
+  // the pair table by field (kMaxTypes^2 words each), so the lanes of different owners,
+  // which read rows of different types, fall on different banks: thr, then the force
+  // constants folded per type pair (rep * fv, att * fv * inv_eff, kexp2 * inv_eff^2)
   constexpr int kTT = kMaxTypes * kMaxTypes;
-  __shared__ float ptab[4][3 * kTT];
+  __shared__ float ptab[4][4 * kTT];
   // list entries: slot | image-shift code << kCodeShift (slots stay below 2^kCodeShift);
   // owners kList + 8 words apart, so the 8 owners' lanes of one read fall on 8 bank groups
   __shared__ uint32_t lslot[4][kOwn][kList + 8];
@@ -1334,8 +1335,9 @@ k_force_list(ForceArgs a) {
     for (int t = lane; t < sd.T * sd.T; t += kWave) {
       const PairConst pt = a.pairs[sd.tab + t];
       ptab[wv][t] = pt.thr;
-      ptab[wv][kTT + t] = pt.fv;
-      ptab[wv][2 * kTT + t] = pt.inv_eff;
+      ptab[wv][kTT + t] = sd.rep * pt.fv;
+      ptab[wv][2 * kTT + t] = sd.att * pt.fv * pt.inv_eff;
+      ptab[wv][3 * kTT + t] = sd.kexp2 * pt.inv_eff * pt.inv_eff;
     }
   const bool active = wave_ok && (p - sd.p0) < sd.n;
   const float4 pi = c.pi;
@@ -1452,14 +1454,16 @@ k_force_list(ForceArgs a) {
         const float sp = s2 + 1e-4f;
         if (sp < thr) {
           cnt++;
-          const float fv = row[kTT + tj], ie = row[2 * kTT + tj];
+          // ParticleSimulation.cu:104-131 with rr = r / effR, r = sqrt(sp):
+          // (rep e^(-k rr^2) - att rr) fv / r, the constants folded per type pair
+          const float rep_fv = row[kTT + tj], att_fv_ie = row[2 * kTT + tj];
+          const float k_ie2 = row[3 * kTT + tj];
           const float inv = __builtin_amdgcn_rsqf(sp);
-          const float rr = (sp * inv) * ie;
-          const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
-          const float w = (sd.rep * ex - sd.att * rr) * fv * inv;
-          fx += dx * w;
-          fy += dy * w;
-          fz += dz * w;
+          const float ex = __builtin_amdgcn_exp2f(k_ie2 * sp);
+          const float w = fmaf(rep_fv, ex, -(att_fv_ie * (sp * inv))) * inv;
+          fx = fmaf(dx, w, fx);
+          fy = fmaf(dy, w, fy);
+          fz = fmaf(dz, w, fz);
         }
         if (EDGES) {
           // edges are non-periodic (ParticleSimulation.cu:222-227): unshifted pairs only
