@@ -905,6 +905,25 @@ __device__ __forceinline__ void row_runs(const ForceArgs& a, const SimDesc& sd, 
   else row_runs_fast<R>(a, sd, c, r, s0, l0, k0, s1, l1, k1);
 }
 
+// A type pair's cutoff and force constants folded for the pair loop: thr, rep * fv,
+// att * fv * inv_eff and kexp2 * inv_eff^2.
+struct PairFold {
+  float thr, rep_fv, att_fv_ie, k_ie2;
+};
+
+__device__ __forceinline__ PairFold fold_pair(const PairConst& pc, const SimDesc& sd) {
+  return PairFold{pc.thr, sd.rep * pc.fv, sd.att * pc.fv * pc.inv_eff,
+                  sd.kexp2 * pc.inv_eff * pc.inv_eff};
+}
+
+// ParticleSimulation.cu:104-131 for a pair in range, sp = d^2 + 1e-4 (r = sqrt(sp),
+// rr = r / effR): (rep e^(-k rr^2) - att rr) fv / r, the factor of the offset (dx, dy, dz).
+__device__ __forceinline__ float pair_weight(const PairFold& pc, float sp) {
+  const float inv = __builtin_amdgcn_rsqf(sp);
+  const float ex = __builtin_amdgcn_exp2f(pc.k_ie2 * sp);
+  return fmaf(pc.rep_fv, ex, -(pc.att_fv_ie * (sp * inv))) * inv;
+}
+
 template <bool EDGES>
 struct GrpAcc {
   float fx, fy, fz;
@@ -918,7 +937,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   constexpr int kRows = (2 * R + 1) * (2 * R + 1);
   constexpr int kRuns = 2 * kRows;
   constexpr int kOwn = kWave / G;  // particles per wave
-  __shared__ PairConst tab[4][kMaxTypes * kMaxTypes];
+  __shared__ PairFold tab[4][kMaxTypes * kMaxTypes];
   __shared__ uint2 runs[4][kOwn][kRuns];  // (start slot, length | shift code << 26)
   __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
   __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kEdgeList : 1];
@@ -933,7 +952,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   const int s = __builtin_amdgcn_readfirstlane(wave_ok ? (int)a.wave_sim[base / kWave] : 0);
   const SimDesc sd = a.sims[s];
   if (wave_ok)
-    for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = a.pairs[sd.tab + t];
+    for (int t = lane; t < sd.T * sd.T; t += kWave) tab[wv][t] = fold_pair(a.pairs[sd.tab + t], sd);
   const int o = lane / G, g = lane % G;
   const int p = (int)base + o;
   const bool active = wave_ok && (p - sd.p0) < sd.n;
@@ -956,7 +975,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
   if (HITS && g == 0) hn[wv][o] = 0u;
   wave_lds_sync();  // pair table ready
 
-  const PairConst* row = &tab[wv][ti * sd.T];
+  const PairFold* row = &tab[wv][ti * sd.T];
   // ---- run table: row_runs of each stencil row, as (start slot, length | code << 26) ----
   {
     Own c;
@@ -968,7 +987,8 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
     c.gen = gen;
     c.nyo = nyo;
     c.nzo = nzo;
-    c.cut2 = row[0].pad;  // largest squared reach of this type, with margins
+    // largest squared reach of this type, with margins
+    c.cut2 = active ? a.pairs[sd.tab + ti * sd.T].pad : 0.0f;
     c.fxr = fminf(fmaxf(pi.x * sd.ihx - (float)cx, 0.0f), 1.0f);
     c.fyr = fminf(fmaxf(pi.y * sd.ihy - (float)cy, 0.0f), 1.0f);
     c.fzr = fminf(fmaxf(pi.z * sd.ihz - (float)cz, 0.0f), 1.0f);
@@ -1007,7 +1027,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
         const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
         const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         const uint32_t tj = __float_as_uint(pj.w);
-        const PairConst pc = row[tj];
+        const PairFold pc = row[tj];
         const float sp = s2 + 1e-4f;
         const bool in = sp < pc.thr;
         if (HITS && in && q != (uint32_t)p) {
@@ -1016,13 +1036,10 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
         }
         if (in) {
           cnt++;
-          const float inv = __builtin_amdgcn_rsqf(sp);
-          const float rr = (sp * inv) * pc.inv_eff;
-          const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
-          const float w = (sd.rep * ex - sd.att * rr) * pc.fv * inv;
-          fx += dx * w;
-          fy += dy * w;
-          fz += dz * w;
+          const float w = pair_weight(pc, sp);
+          fx = fmaf(dx, w, fx);
+          fy = fmaf(dy, w, fy);
+          fz = fmaf(dz, w, fz);
         }
         if (EDGES) {
           if ((in ? ein : eout) && tj == ti && s2 < sd.edge_d2) {
@@ -1060,7 +1077,7 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
         }
         const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         const uint32_t tj = __float_as_uint(pj.w);
-        const PairConst pc = row[tj];
+        const PairFold pc = row[tj];
         const float sp = s2 + 1e-4f;
         if (HITS && sp < pc.thr && q != (uint32_t)p) {
           const uint32_t at = atomicAdd(&hn[wv][o], 1u);
@@ -1068,13 +1085,10 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
         }
         if (sp < pc.thr) {
           cnt++;
-          const float inv = __builtin_amdgcn_rsqf(sp);
-          const float rr = (sp * inv) * pc.inv_eff;
-          const float ex = __builtin_amdgcn_exp2f(sd.kexp2 * (rr * rr));
-          const float w = (sd.rep * ex - sd.att * rr) * pc.fv * inv;
-          fx += dx * w;
-          fy += dy * w;
-          fz += dz * w;
+          const float w = pair_weight(pc, sp);
+          fx = fmaf(dx, w, fx);
+          fy = fmaf(dy, w, fy);
+          fz = fmaf(dz, w, fz);
         }
         if (EDGES) {
           if (do_edges && raw && tj == ti && s2 < sd.edge_d2) {
