@@ -386,6 +386,19 @@ class ParticleSimulation {
     check(cf_preset_write(path, &ui, template_path));
   }
 
+  // Autotuner (no reference counterpart): times the kernel variants, lanes per particle and
+  // step sorts on a copy of the current state for `params` (and the edge options the caller
+  // passes to simulate) and applies the fastest; the state is left as it was.  With
+  // `cachePath`, a JSON cache answers later calls for the same workload bucket.  Untuned, the
+  // engine runs force variant 0 at 16 lanes: at 100k particles about 1.5x slower per step.
+  cf_tuning autotune(const SimulationParams& params, const cf_edge_opts* edges = nullptr,
+                     const char* cachePath = nullptr, int trialSteps = 64) {
+    const cf_params p = params.to_c();
+    cf_tuning t{};
+    check(cf_autotune(h_, &p, edges, trialSteps, cachePath, 0, &t, nullptr, nullptr, 0));
+    return t;
+  }
+
   cf_sim* handle() const { return h_; }
 
  private:
