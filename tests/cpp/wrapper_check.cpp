@@ -5,8 +5,13 @@
 // through another binding and compare bit for bit.  With a preset path as the second
 // argument it also loads that preset, steps, and saves it to <dir>/saved.json.
 //
-// Exit status: 0 success; 10 + cellflow status on a cellflow::Error (12 = no device).
+// With "--tune" as the first argument it instead checks the class's autotune(): the state is
+// left as it was, the chosen tuning is the one applied, and the tuned simulation steps.
+//
+// Exit status: 0 success; 1 a failed check; 10 + cellflow status on a cellflow::Error
+// (12 = no device).
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -22,11 +27,42 @@ void dump(const std::string& path, const T* data, size_t count) {
   std::fclose(f);
 }
 
+int tune_check() {
+  cellflow::ParticleSimulation sim(4000, 0.5f, 0.5f);
+  cellflow::SimulationParams p;
+  p.radius = 0.05f;
+  p.canvasWidth = p.canvasHeight = p.canvasDepth = 0.5f;
+  p.ratioWithLFO = 0.3f;
+  sim.simulate(p, 3);
+  std::vector<cellflow::Particle> before, after;
+  sim.getParticleData(before);
+  const cf_tuning t = sim.autotune(p, nullptr, nullptr, 4);
+  sim.getParticleData(after);
+  if (before.size() != after.size() ||
+      std::memcmp(before.data(), after.data(), before.size() * sizeof(before[0])) != 0) {
+    std::printf("autotune changed the state\n");
+    return 1;
+  }
+  cf_tuning applied{};
+  if (cf_get_tuning(sim.handle(), &applied) != CF_OK ||
+      applied.force_variant != t.force_variant ||
+      applied.lanes_per_particle != t.lanes_per_particle ||
+      applied.sort_variant != t.sort_variant || applied.cell_divisor != t.cell_divisor) {
+    std::printf("the chosen tuning is not the applied one\n");
+    return 1;
+  }
+  sim.simulate(p, 3);
+  std::printf("tuned: variant %d, %d lanes, cell divisor %d, sort %d\n", t.force_variant,
+              t.lanes_per_particle, t.cell_divisor, t.sort_variant);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   const std::string dir = argc > 1 ? argv[1] : ".";
   try {
+    if (dir == "--tune") return tune_check();
     const int n = 6000, frames = 12;
     cellflow::ParticleSimulation sim(n, 0.6f, 0.6f);  // canvas 0.6^3; spawn cube = canvas
     cellflow::SimulationParams p;

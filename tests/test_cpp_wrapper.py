@@ -36,6 +36,17 @@ def test_wrapper_without_device_reports_status(tmp_path):
 
 
 @pytest.mark.gpu
+def test_wrapper_autotune():
+    """The class's autotune(): the state is unchanged, the chosen tuning is applied, the
+    tuned simulation steps."""
+    assert os.path.exists(BIN), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    r = subprocess.run([BIN, "--tune"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "tuned: variant" in r.stdout
+    print(r.stdout.strip())
+
+
+@pytest.mark.gpu
 def test_wrapper_matches_python_binding(tmp_path):
     r = run(tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
