@@ -187,19 +187,19 @@ def test_edges_truncation_matches_oracle(orc, maxc):
         assert np.array_equal(edges, ref), f"dist={dist} maxc={maxc}"
 
 
-@pytest.mark.parametrize("variant,lanes", [(0, 4), (0, 16), (3, 8), (3, 16), (3, 32), (4, 8)])
-def test_fused_edges_dense_blobs(orc, variant, lanes):
+@pytest.mark.parametrize("variant,lanes,divisor", [(0, 4, 1), (0, 16, 1), (3, 8, 1), (3, 16, 1),
+                                                   (3, 32, 1), (4, 8, 1), (3, 8, 2), (4, 8, 2)])
+def test_fused_edges_dense_blobs(orc, variant, lanes, divisor):
     """Fused edge candidates on dense blobs: many owners exceed the per-particle LDS edge
-    list (variants 3 and 4 at 8 lanes then select in the wave's LDS lists, the others take
-    the streaming selection) and, in variants 3 and 4, hold several candidate-list chunks;
-    the graph must still equal the oracle's."""
+    list (variants 3 and 4 at 8 lanes then select in the wave's LDS lists, with the 27- and
+    the 125-cell stencil; the others take the streaming selection) and, in variants 3 and 4,
+    hold several candidate-list chunks; the graph must still equal the oracle's."""
     st, phys, params = clustered_state(n=5000, blobs=4, sd=0.006)
     sim, twin = make_sim(st, phys), make_sim(st, phys)
-    for h in (sim, twin):
-        h.setTuning(force_variant=variant, lanes_per_particle=lanes)
     for sort in (1, 0):
         for h in (sim, twin):
-            h.setTuning(force_variant=variant, lanes_per_particle=lanes, sort_variant=sort)
+            h.setTuning(cell_divisor=divisor, force_variant=variant, lanes_per_particle=lanes,
+                        sort_variant=sort)
         check_step_edges(orc, sim, twin, params, (0.01, 16), steps=2)
         twin.simulate(params, 1)  # back in step with sim
 
