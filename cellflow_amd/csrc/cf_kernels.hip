@@ -1221,12 +1221,17 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
 //
 // k_force_grp walks every run of the stencil in every lane, which at ~20 candidates per
 // particle costs more instructions than the candidates themselves.  Here each lane builds
-// the runs of its own stencil row(s) in registers, a group prefix scan places them in a
-// flattened order, the lanes copy their slots into a per-particle LDS list (chunks of
-// kList), and the G lanes then take list entries g, g+G, ... with one loop body for
-// shifted and unshifted runs alike.  Group sums use DPP mirror butterflies.
+// the runs of its own stencil row(s) in registers, a group prefix scan places them in the
+// particle's flattened order, the lanes copy their slots into the wave's LDS list, and each
+// particle's G lanes then take its list entries g, g+G, ... with one loop body for shifted
+// and unshifted runs alike.  When the wave's particles have at most kWaveList candidates in
+// all, their lists lie back to back and take one pass; otherwise each particle has
+// kWaveList / (64 / G) entries of the list and takes its candidates that many at a time
+// (the particles of a wave always advance together: a split by position would run a
+// long list before the list and after the split one after the other).  Group sums use DPP
+// mirror butterflies.
 
-constexpr int kList = 32;       // candidate slots per particle per chunk
+constexpr int kWaveList = 640;  // candidate slots per wave
 constexpr int kListEdges = 24;  // LDS edge candidates per particle (more: streaming selection)
 
 // Sum over aligned groups of G lanes (G in 4, 8, 16, 32), every lane gets its group's sum.
@@ -1314,12 +1319,16 @@ k_force_list(ForceArgs a) {
   constexpr int kRpl = (kRows + G - 1) / G;  // stencil rows per lane
   // the pair table by field (kMaxTypes^2 words each), so the lanes of different owners,
   // which read rows of different types, fall on different banks: thr, then the force
-  // constants folded per type pair (rep * fv, att * fv * inv_eff, kexp2 * inv_eff^2)
+  // constants folded per type pair (rep * fv, att * fv * inv_eff, kexp2 * inv_eff^2).
+  // One table per block: a block's 4 * kOwn particles (at most 64, aligned) belong to one
+  // simulation, and each wave writes the whole table itself (the same values as the other
+  // waves) before it reads it, so no block barrier is needed.
+  static_assert(4 * kOwn <= kWave, "a block's particles lie in one simulation's 64 slots");
   constexpr int kTT = kMaxTypes * kMaxTypes;
-  __shared__ float ptab[4][4 * kTT];
-  // list entries: slot | image-shift code << kCodeShift (slots stay below 2^kCodeShift);
-  // owners kList + 8 words apart, so the 8 owners' lanes of one read fall on 8 bank groups
-  __shared__ uint32_t lslot[4][kOwn][kList + 8];
+  __shared__ float ptab[4 * kTT];
+  // list entries: slot | image-shift code << kCodeShift (slots stay below 2^kCodeShift),
+  // the wave's particles one after another
+  __shared__ uint32_t lslot[4][kWaveList];
   __shared__ float2 el_d[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_i[EDGES ? 4 : 1][EDGES ? kOwn : 1][EDGES ? kListEdges : 1];
   __shared__ uint32_t el_n[4][kOwn];
@@ -1348,10 +1357,10 @@ k_force_list(ForceArgs a) {
   if (wave_ok)
     for (int t = lane; t < sd.T * sd.T; t += kWave) {
       const PairConst pt = a.pairs[sd.tab + t];
-      ptab[wv][t] = pt.thr;
-      ptab[wv][kTT + t] = sd.rep * pt.fv;
-      ptab[wv][2 * kTT + t] = sd.att * pt.fv * pt.inv_eff;
-      ptab[wv][3 * kTT + t] = sd.kexp2 * pt.inv_eff * pt.inv_eff;
+      ptab[t] = pt.thr;
+      ptab[kTT + t] = sd.rep * pt.fv;
+      ptab[2 * kTT + t] = sd.att * pt.fv * pt.inv_eff;
+      ptab[3 * kTT + t] = sd.kexp2 * pt.inv_eff * pt.inv_eff;
     }
   const bool active = wave_ok && (p - sd.p0) < sd.n;
   const float4 pi = c.pi;
@@ -1406,16 +1415,32 @@ k_force_list(ForceArgs a) {
   for (int k = 0; k < 2 * kRpl; ++k) rs[k] |= rk[k] << kCodeShift;
   const uint32_t first = group_excl_scan<G>(mine, g);
   const uint32_t total = (uint32_t)dpp_group_sum_i<G>((int)mine);
+  // the particle's part of the wave's list: from its first position in the wave's order
+  // (an inclusive scan of the particles' counts over the wave's groups, less its own) when
+  // the wave's candidates fit the list at once, else kQuota entries at o * kQuota.  kQuota
+  // is an odd multiple of G: each lane keeps the entries it takes in one pass (g, g+G, ...)
+  // and the particles' lanes of one read fall on different banks.
+  static_assert(kWaveList % kWave == 0, "list of whole rows");
+  constexpr uint32_t kQuota = (uint32_t)(G * ((kWaveList / kWave - 1) | 1));
+  uint32_t incl = total;
+#pragma unroll
+  for (int d = G; d < kWave; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += y;
+  }
+  const bool packed = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1) <= (uint32_t)kWaveList;
+  const uint32_t obase = packed ? incl - total : (uint32_t)o * kQuota;
+  const uint32_t quota = packed ? (uint32_t)kWaveList : kQuota;
   wave_lds_sync();  // pair table ready
-  const float* row = &ptab[wv][ti * sd.T];
+  const float* row = &ptab[ti * sd.T];
 
-  // ---- candidates, kList at a time -------------------------------------------------
+  // ---- candidates, quota at a time -------------------------------------------------
   const bool want_edges = EDGES && a.edges && sd.edge_on;  // the same for the whole wave
   const bool do_edges = want_edges && active;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   int cnt = 0;
-  for (uint32_t b0 = 0; b0 < total; b0 += (uint32_t)kList) {
-    const uint32_t b1 = b0 + (uint32_t)kList;
+  for (uint32_t b0 = 0; b0 < total; b0 += quota) {
+    const uint32_t b1 = b0 + quota;
     uint32_t f0 = first;
 #pragma unroll
     for (int k = 0; k < 2 * kRpl; ++k) {
@@ -1423,7 +1448,7 @@ k_force_list(ForceArgs a) {
       // entries lo..hi-1 of run k: (slot | code << kCodeShift) + f - f0 (no carry), two per
       // store instruction
       if (lo < hi) {
-        uint32_t* d = &lslot[wv][o][lo - b0];
+        uint32_t* d = &lslot[wv][obase + (lo - b0)];
         uint32_t v = rs[k] + (lo - f0);
         uint32_t m = hi - lo;
         for (; m >= 2u; m -= 2u, d += 2, v += 2u) {
@@ -1435,13 +1460,13 @@ k_force_list(ForceArgs a) {
       f0 += rl[k];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const uint32_t nb = min(total - b0, (uint32_t)kList);
+    const uint32_t nb = min(total - b0, quota);
     // one copy of the loop per grid kind (the choice is the same for the whole wave), so
     // the common kind carries no per-pair minimum-image code
     auto pairs = [&](auto gen_kind) {
       constexpr bool GEN = decltype(gen_kind)::value;
       for (uint32_t k = (uint32_t)g; k < nb; k += (uint32_t)G) {
-        const uint32_t e = lslot[wv][o][k];
+        const uint32_t e = lslot[wv][obase + k];
         const uint32_t q = e & ((1u << kCodeShift) - 1u), code = e >> kCodeShift;
         const float4 pj = a.pos[q];
         // the index travels with the position: an edge candidate then waits for no load
@@ -1543,12 +1568,13 @@ k_force_list(ForceArgs a) {
     // More than kOvfCap qualifying: the owner's lane takes the streaming selection below.
     uint32_t ovf_e = ~0u;  // kept edges of an owner this handled (its lane g == 0)
     if constexpr (G == 8) {
-      constexpr int kOvfCap = kOwn * (kList + 8);  // words of the wave's lslot
+      constexpr int kOvfCap = 320;
+      static_assert(kOvfCap <= kWaveList, "indices fit the wave's lslot");
       static_assert(kOvfCap <= 2 * kOwn * kListEdges, "d^2 fits the wave's el_d");
       constexpr int kOvfInfo = G * 2 * kRpl * 2;  // after the runs in the wave's el_i
       static_assert(kOvfInfo + 7 <= kOwn * kListEdges, "runs and owner fit the wave's el_i");
       float* ov_d = &el_d[wv][0][0].x;
-      uint32_t* ov_i = &lslot[wv][0][0];
+      uint32_t* ov_i = &lslot[wv][0];
       uint32_t* ov_r = &el_i[wv][0][0];
       unsigned long long left = __ballot(do_edges && g == 0 && !fits);
       if (left) wave_lds_sync();  // the fitting owners' reads of el_d, el_i are done
