@@ -19,15 +19,17 @@ def small_ensemble(S=6, seed=3):
     return configs.make_ensemble(cfg, seed=seed, sims=S)
 
 
-@pytest.mark.parametrize("lanes,divisor", [(16, 1), (8, 2)])
-def test_ensemble_members_equal_plain_runs(lanes, divisor):
+@pytest.mark.parametrize("variant,lanes,divisor", [(0, 16, 1), (0, 8, 2), (3, 4, 1), (4, 8, 1)])
+def test_ensemble_members_equal_plain_runs(variant, lanes, divisor):
+    """Also for the list kernels, whose blocks share one pair table (a block's particles
+    always lie in one simulation)."""
     sims = small_ensemble()
     ens = Ensemble(sims)
-    ens.setTuning(cell_divisor=divisor, lanes_per_particle=lanes)
+    ens.setTuning(cell_divisor=divisor, force_variant=variant, lanes_per_particle=lanes)
     ens.simulate(12)
     for s, (ph, st) in enumerate(sims):
         plain = make_sim(st, ph)
-        plain.setTuning(cell_divisor=divisor, lanes_per_particle=lanes)
+        plain.setTuning(cell_divisor=divisor, force_variant=variant, lanes_per_particle=lanes)
         plain.simulate(SimulationParams.from_physics(ph), 12)
         a, b = ens.getParticleData(s), plain.getParticleData()
         assert a.tobytes() == b.tobytes(), f"sim {s}"
