@@ -1225,11 +1225,11 @@ __global__ void __launch_bounds__(256) k_force_grp(ForceArgs a) {
 // particle's flattened order, the lanes copy their slots into the wave's LDS list, and each
 // particle's G lanes then take its list entries g, g+G, ... with one loop body for shifted
 // and unshifted runs alike.  When the wave's particles have at most kWaveList candidates in
-// all, their lists lie back to back and take one pass; otherwise each particle has
-// kWaveList / (64 / G) entries of the list and takes its candidates that many at a time
-// (the particles of a wave always advance together: a split by position would run a
-// long list before the list and after the split one after the other).  Group sums use DPP
-// mirror butterflies.
+// all, their lists lie back to back and take one pass; otherwise each particle has 9 * G
+// entries of the list and takes its candidates that many at a time, so the wave's
+// particles still advance together (split by position instead, a long list before the
+// split and one after it would run one after the other).  Group sums use DPP mirror
+// butterflies.
 
 constexpr int kWaveList = 640;  // candidate slots per wave
 constexpr int kListEdges = 24;  // LDS edge candidates per particle (more: streaming selection)
